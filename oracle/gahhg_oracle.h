@@ -1,0 +1,122 @@
+/*
+ * gahhg_oracle.h -- CPU restatement of the GAHHG point-in-spherical-polygon
+ * algorithm (hierarchy build, leaf classification, query).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against, and the timed CPU baseline.  Nothing in
+ * the product (gahhg_amd/, include/) links, loads or calls it.
+ *
+ * Every routine follows the reference source with the same FP64 operation
+ * order (no FMA contraction: built with -ffp-contract=off).  Citations are to
+ * the reference repository (source/...).  Where the reference is racy or
+ * reads out of bounds, the deterministic semantics chosen here are stated at
+ * the routine.
+ */
+#ifndef GAHHG_ORACLE_H
+#define GAHHG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct og_ctx og_ctx;
+
+/* Level ranges: source/GPISPAHHG.cuh:12-20 (GAHHGGrid). */
+typedef struct {
+    int32_t vstart, vnum, bstart, bnum, cstart, cnum, rnum;
+} og_level;
+
+/* Diagnostics for behaviour the reference leaves undefined, plus roofline
+ * counters.  All counts are exact. */
+typedef struct {
+    int64_t walk_steps_mark;       /* cells visited by all marking walks      */
+    int64_t walk_steps_dispatch;   /* cells visited by the counting walk       */
+    int64_t walk_partner_missing;  /* X/T1/OVERLAP exit through partner -1    */
+    int64_t walk_step_cap_hits;    /* walks stopped by the step cap            */
+    int64_t walk_unlocated;        /* edges whose p0 was not located           */
+    int64_t overlap_bad_index;     /* marking OVERLAP index outside the array  */
+    int64_t zero_ref_edges;        /* edges with 0 bottom-level refs           */
+    int64_t undefined_leaves_snapshot; /* leaves a single classification pass
+                                          leaves UNDEFINED                     */
+    int64_t undefined_leaves_final;    /* leaves UNDEFINED after fixpoint      */
+    int64_t partner_conflicts;     /* setSCPartner writes that disagree        */
+    int32_t mark_passes;           /* marking passes run (M)                   */
+    int32_t pad;
+} og_diag;
+
+/* Per-query work counters (roofline numerators). */
+typedef struct {
+    int64_t plane_tests;     /* T: n.p plane evaluations in the descent      */
+    int64_t boundary_hits;   /* queries ending in a leaf with rnum > 0       */
+    int64_t ref_tests;       /* isEEInter evaluations                        */
+    int64_t bnum_sum;        /* sum of bnum over boundary leaves hit         */
+    int64_t unlocated;       /* queries that reached no leaf                 */
+    int64_t inside;          /* labels == IN                                 */
+} og_qcount;
+
+og_ctx* og_new(void);
+void og_free(og_ctx* c);
+const char* og_last_error(og_ctx* c);
+
+/* Polygon input.  TXT format: source/GPISPPolygon.cu:32-278. */
+int og_load_polygon_txt(og_ctx* c, const char* path);
+/* lon/lat rings, each ring including its closing duplicate vertex;
+ * ring_off has n_rings+1 entries (vertex offsets), ring 0 is the outer ring. */
+int og_set_polygon_lonlat(og_ctx* c, const double* lon, const double* lat,
+                          const int64_t* ring_off, int32_t n_rings);
+int32_t og_num_edges(og_ctx* c);
+int32_t og_orientation_flips(og_ctx* c);  /* bit r set: ring r was reversed */
+void og_get_polygon(og_ctx* c, double* pv /*3N*/, int32_t* pe /*2N*/);
+
+/* Point conversions: source/GPISPPolygon.cuh:21-30 and :72,82-92. */
+void og_ll2xyz(const double* lon, const double* lat, int64_t n, double* xyz);
+void og_query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n,
+                            double* xyz);
+/* Query CSV "lon,lat" lines: source/GPISPTP.cu:13-79.  Returns count or -1;
+ * call with lon==NULL to count only. */
+int64_t og_load_query_csv(og_ctx* c, const char* path, double* lon, double* lat,
+                          int64_t cap);
+
+/* Level-0 grid: 42 cells / 80 vertices / 240 borders (restated from the
+ * geometry the reference's closed grid-0 library implies). */
+int og_grid0(double* v /*80*3*/, int32_t* border_i /*240*6*/,
+             double* border_n /*240*3*/, int32_t* cell_i /*42*7*/);
+
+/* Build + classify (source/GPISPAHHG.cu:1947-2059).  max_depth as in
+ * GPISPParam (default 15); auto_depth 1 = the U/N density stop rule. */
+int og_build(og_ctx* c, int32_t max_depth, int32_t auto_depth);
+/* Build only up to (and including) dispatch, no classification. */
+int og_build_create_only(og_ctx* c, int32_t max_depth, int32_t auto_depth);
+int og_classify(og_ctx* c);
+
+/* Counts: [0]=gnum [1]=V [2]=B [3]=C [4]=R; gamma via og_gamma. */
+void og_counts(og_ctx* c, int64_t* out5);
+float og_gamma(og_ctx* c);
+void og_levels(og_ctx* c, og_level* out /*gnum*/);
+void og_diagnostics(og_ctx* c, og_diag* out);
+/* Grid export.  cells: 7 ints per cell (bstart,bnum,rstart,rnum,upcell,
+ * subcell,type); borders: 6 ints (parent,partner,v0,v1,subcell,id) + 3
+ * doubles (n); refs: R ints with each cell's run sorted ascending. */
+void og_export(og_ctx* c, double* v, int32_t* cells, int32_t* borders_i,
+               double* borders_n, int32_t* refs);
+
+/* Query (source/GPISPAHHG.cu:1422-1504).  Labels: 2 IN, 1 OUT, 0 undefined
+ * or unlocated.  nthreads <= 0 uses all OpenMP threads. */
+void og_query_xyz(og_ctx* c, const double* xyz, int64_t n, int32_t* labels,
+                  int32_t nthreads, og_qcount* counts /*nullable*/);
+
+/* Exact FP64 primitives exposed for known-answer tests. */
+void og_kat_normalize(const double* in3, double* out3);
+void og_kat_cross(const double* a3, const double* b3, double* out3);
+int32_t og_kat_is_ee_inter(const double* p0, const double* p1,
+                           const double* v0, const double* v1);
+int32_t og_kat_edge_border_inter(const double* p0, const double* p1,
+                                 const double* v0, const double* v1,
+                                 int32_t* reserved);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
