@@ -1,0 +1,297 @@
+"""gahhg_amd -- MI355X (gfx950) GAHHG point-in-spherical-polygon library.
+
+The product is the native library lib/libgahhg.so (C ABI in include/gahhg.h,
+host C++ + hand-written HIP kernels).  This module is a thin ctypes binding
+whose class mirrors the reference's driver sequence (source/GPISP.cu:47-67):
+
+    GPISP.load_polygon_txt   <- GPISPPolygon::loadPolyFromTXT
+    GPISP.build              <- GPISP::createAHHGInit + createAHHG + classifyAHHG
+    GPISP.query_lonlat/xyz   <- GPISPTP::loadTPFromCSV + GPISP::test
+    GPISP.stats              <- GPISP::getStat
+    GPISP.write_report       <- GPISP::exportStat2TXT
+    write_result_csv         <- GPISPTP::exportResult2CSV
+
+There is no CPU fallback: if the library is missing or no GPU is visible the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libgahhg.so")
+CSRC = os.path.join(PKG_DIR, "csrc")
+
+LABEL_UNDEFINED, LABEL_OUT, LABEL_IN = 0, 1, 2
+
+ERRORS = {-1: "bad argument", -2: "cannot open file", -3: "malformed file",
+          -4: "call out of order", -5: "HIP runtime error", -6: "out of memory",
+          -7: "size beyond int32 indexing"}
+
+
+class GahhgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gahhg error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
+        "device", "verbose", "query_layout")]
+
+
+_STAT_FIELDS = [
+    ("pre_ms", ctypes.c_double), ("create_ms", ctypes.c_double), ("classify_ms", ctypes.c_double),
+    ("query_ms", ctypes.c_double), ("query_unit_s", ctypes.c_double), ("depth", ctypes.c_int32),
+    ("gamma", ctypes.c_float), ("n_edges", ctypes.c_int64), ("n_queries", ctypes.c_int64),
+    ("vnum", ctypes.c_int64), ("bnum", ctypes.c_int64), ("cnum", ctypes.c_int64), ("rnum", ctypes.c_int64),
+    ("vspace", ctypes.c_int64), ("bspace", ctypes.c_int64), ("cspace", ctypes.c_int64),
+    ("rspace", ctypes.c_int64), ("gpu_space_bytes", ctypes.c_int64), ("cpu_space_bytes", ctypes.c_int64),
+    ("device_bytes", ctypes.c_int64), ("inside", ctypes.c_int64), ("outside", ctypes.c_int64),
+    ("unlocated", ctypes.c_int64), ("orientation_flipped", ctypes.c_int32), ("mark_passes", ctypes.c_int32),
+    ("walk_steps_mark", ctypes.c_int64), ("walk_steps_dispatch", ctypes.c_int64),
+    ("walk_partner_missing", ctypes.c_int64), ("walk_step_cap_hits", ctypes.c_int64),
+    ("walk_unlocated", ctypes.c_int64), ("zero_ref_edges", ctypes.c_int64),
+    ("undefined_leaves_snapshot", ctypes.c_int64), ("undefined_leaves_final", ctypes.c_int64),
+]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = _STAT_FIELDS
+
+
+# Every symbol include/gahhg.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "gahhg_params_default", "gahhg_create", "gahhg_destroy", "gahhg_last_error",
+    "gahhg_load_polygon_txt", "gahhg_set_polygon_lonlat", "gahhg_build", "gahhg_query_lonlat",
+    "gahhg_query_xyz", "gahhg_query_xyz_device", "gahhg_read_query_csv", "gahhg_query_lonlat_to_xyz",
+    "gahhg_get_stats", "gahhg_write_report", "gahhg_write_result_csv", "gahhg_device_alloc",
+    "gahhg_device_free", "gahhg_memcpy_h2d", "gahhg_memcpy_d2h", "gahhg_grid_counts",
+    "gahhg_export_grid", "gahhg_kat_device",
+]
+
+_lib = None
+
+
+def build_library(jobs=8):
+    import subprocess
+    subprocess.check_call(["make", "-s", f"-j{jobs}", "-C", CSRC])
+
+
+def lib():
+    """Loads lib/libgahhg.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GahhgError(-4, f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    i32, i64 = ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "gahhg_params_default": ([ctypes.POINTER(Params)], ctypes.c_int),
+        "gahhg_create": ([ctypes.POINTER(P), ctypes.POINTER(Params)], ctypes.c_int),
+        "gahhg_destroy": ([P], None),
+        "gahhg_last_error": ([P], ctypes.c_char_p),
+        "gahhg_load_polygon_txt": ([P, ctypes.c_char_p], ctypes.c_int),
+        "gahhg_set_polygon_lonlat": ([P, dp, dp, lp, i32], ctypes.c_int),
+        "gahhg_build": ([P], ctypes.c_int),
+        "gahhg_query_lonlat": ([P, dp, dp, i64, ip], ctypes.c_int),
+        "gahhg_query_xyz": ([P, dp, i64, ip], ctypes.c_int),
+        "gahhg_query_xyz_device": ([P, P, i64, P], ctypes.c_int),
+        "gahhg_read_query_csv": ([ctypes.c_char_p, P, P, i64, ctypes.POINTER(i64)], ctypes.c_int),
+        "gahhg_query_lonlat_to_xyz": ([dp, dp, i64, dp], ctypes.c_int),
+        "gahhg_get_stats": ([P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gahhg_write_report": ([P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+        "gahhg_write_result_csv": ([ctypes.c_char_p, ip, i64], ctypes.c_int),
+        "gahhg_device_alloc": ([P, i64, ctypes.POINTER(P)], ctypes.c_int),
+        "gahhg_device_free": ([P, P], ctypes.c_int),
+        "gahhg_memcpy_h2d": ([P, P, P, i64], ctypes.c_int),
+        "gahhg_memcpy_d2h": ([P, P, P, i64], ctypes.c_int),
+        "gahhg_grid_counts": ([P, lp], ctypes.c_int),
+        "gahhg_export_grid": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "gahhg_kat_device": ([P, i32, dp, i64, dp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def default_params(**kw):
+    p = Params()
+    lib().gahhg_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, int(v))
+    return p
+
+
+def query_lonlat_to_xyz(lon, lat):
+    """Reference test-point conversion (FakeLLVertex round trip), host side."""
+    lon = np.ascontiguousarray(lon, dtype=np.float64)
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    out = np.empty(lon.size * 3)
+    rc = lib().gahhg_query_lonlat_to_xyz(lon, lat, lon.size, out)
+    if rc:
+        raise GahhgError(rc, "lonlat_to_xyz")
+    return out.reshape(-1, 3)
+
+
+def read_query_csv(path):
+    L = lib()
+    n = ctypes.c_int64(0)
+    rc = L.gahhg_read_query_csv(str(path).encode(), None, None, 0, ctypes.byref(n))
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    lon = np.zeros(n.value)
+    lat = np.zeros(n.value)
+    rc = L.gahhg_read_query_csv(str(path).encode(), lon.ctypes.data, lat.ctypes.data, n.value,
+                                ctypes.byref(n))
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    return lon, lat
+
+
+def write_result_csv(path, labels):
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    rc = lib().gahhg_write_result_csv(str(path).encode(), labels, labels.size)
+    if rc:
+        raise GahhgError(rc, f"write {path}")
+
+
+class DeviceBuffer:
+    """Device allocation owned by a context (for HBM-resident inputs)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._check(ctx.L.gahhg_device_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.L.gahhg_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, arr):
+        assert arr.flags["C_CONTIGUOUS"] and arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.L.gahhg_memcpy_d2h(self.ctx.h, arr.ctypes.data, self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h:
+            self.ctx._check(self.ctx.L.gahhg_device_free(self.ctx.h, self.ptr))
+        self.ptr = None
+
+
+class GPISP:
+    """One polygon's hierarchy on one GPU (the reference's GPISP object)."""
+
+    def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
+                 query_layout=0):
+        self.L = lib()
+        prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
+                             verbose=1 if verbose else 0, block_query=block_query,
+                             query_layout=query_layout)
+        h = ctypes.c_void_p()
+        rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
+        if rc:
+            raise GahhgError(rc, "gahhg_create (is a GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gahhg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise GahhgError(rc, self.L.gahhg_last_error(self.h).decode())
+
+    # -- polygon -----------------------------------------------------------
+    def load_polygon_txt(self, path):
+        self._check(self.L.gahhg_load_polygon_txt(self.h, str(path).encode()))
+        return self
+
+    loadPolyFromTXT = load_polygon_txt
+
+    def set_polygon_lonlat(self, lon, lat, ring_off):
+        self._check(self.L.gahhg_set_polygon_lonlat(
+            self.h, np.ascontiguousarray(lon, dtype=np.float64), np.ascontiguousarray(lat, dtype=np.float64),
+            np.ascontiguousarray(ring_off, dtype=np.int64), len(ring_off) - 1))
+        return self
+
+    # -- build -------------------------------------------------------------
+    def build(self):
+        self._check(self.L.gahhg_build(self.h))
+        return self
+
+    # -- query -------------------------------------------------------------
+    def query_xyz(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        n = xyz.size // 3
+        labels = np.zeros(n, dtype=np.int32)
+        self._check(self.L.gahhg_query_xyz(self.h, xyz, n, labels))
+        return labels
+
+    def query_lonlat(self, lon, lat):
+        lon = np.ascontiguousarray(lon, dtype=np.float64)
+        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        labels = np.zeros(lon.size, dtype=np.int32)
+        self._check(self.L.gahhg_query_lonlat(self.h, lon, lat, lon.size, labels))
+        return labels
+
+    def query_device(self, d_xyz, n, d_labels):
+        self._check(self.L.gahhg_query_xyz_device(self.h, d_xyz.ptr, int(n), d_labels.ptr))
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    # -- stats / export ----------------------------------------------------
+    def stats(self):
+        s = Stats()
+        self._check(self.L.gahhg_get_stats(self.h, ctypes.byref(s)))
+        return dict((f, getattr(s, f)) for f, _ in _STAT_FIELDS)
+
+    def write_report(self, path, poly_name="", query_name=""):
+        self._check(self.L.gahhg_write_report(self.h, str(path).encode(), poly_name.encode(),
+                                              query_name.encode()))
+
+    def grid(self):
+        cnt = np.zeros(5, dtype=np.int64)
+        self._check(self.L.gahhg_grid_counts(self.h, cnt))
+        d, V, B, C, R = (int(x) for x in cnt)
+        v = np.zeros(V * 3)
+        cells = np.zeros(C * 7, dtype=np.int32)
+        borders = np.zeros(B * 6, dtype=np.int32)
+        normals = np.zeros(B * 3)
+        refs = np.zeros(max(R, 1), dtype=np.int32)
+        levels = np.zeros(7 * d, dtype=np.int32)
+        self._check(self.L.gahhg_export_grid(self.h, v.ctypes.data, cells.ctypes.data, borders.ctypes.data,
+                                             normals.ctypes.data, refs.ctypes.data, levels.ctypes.data))
+        return dict(depth=d, v=v.reshape(-1, 3), cells=cells.reshape(-1, 7), borders=borders.reshape(-1, 6),
+                    normals=normals.reshape(-1, 3), refs=refs[:R], levels=levels.reshape(-1, 7))
+
+    def kat(self, op, inputs):
+        inputs = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 12)
+        out = np.zeros((inputs.shape[0], 3))
+        self._check(self.L.gahhg_kat_device(self.h, op, inputs.reshape(-1), inputs.shape[0], out.reshape(-1)))
+        return out

@@ -1,0 +1,447 @@
+// gahhg_capi.hip -- the extern "C" boundary (include/gahhg.h).
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gahhg_internal.h"
+
+namespace gahhg {
+int count_labels(gahhg_ctx* c, const int32_t* d_labels, int64_t n, unsigned long long* d_out2);
+}
+
+using namespace gahhg;
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define RCK(x)                       \
+  do {                               \
+    int _rc = (x);                   \
+    if (_rc != GAHHG_OK) return _rc; \
+  } while (0)
+
+int ctx_ok(gahhg_ctx* c) { return c ? GAHHG_OK : GAHHG_E_ARG; }
+
+int select_device(gahhg_ctx* c) { return hip_check(c, hipSetDevice(c->device), "hipSetDevice"); }
+
+void free_grid(gahhg_ctx* c) {
+  DeviceGrid& g = c->g;
+  void* ptrs[] = {g.v, g.b_parent, g.b_partner, g.b_v0, g.b_v1, g.b_subcell, g.b_id, g.b_n,
+                  g.c_bstart, g.c_bnum, g.c_rstart, g.c_rnum, g.c_upcell, g.c_subcell, g.c_type, g.refs};
+  for (void* p : ptrs) dfree(p);
+  g = DeviceGrid();
+  QueryLayout& q = c->q;
+  dfree(q.nd);
+  dfree(q.qc);
+  dfree(q.qe);
+  dfree(q.qside);
+  q = QueryLayout();
+  c->built = false;
+  c->gnum = 0;
+}
+
+void free_scratch(gahhg_ctx* c) {
+  void* ptrs[] = {c->d_f0, c->d_f1, c->d_o0, c->d_o1, c->d_scan, c->d_ecnt, c->d_eoff,
+                  c->d_rcell, c->d_rflag, c->d_redge, c->d_loc_cur, c->d_loc_prev};
+  for (void* p : ptrs) dfree(p);
+  c->d_f0 = c->d_f1 = c->d_o0 = c->d_o1 = c->d_scan = c->d_ecnt = nullptr;
+  c->d_eoff = nullptr;
+  c->d_rcell = c->d_rflag = c->d_redge = c->d_loc_cur = c->d_loc_prev = nullptr;
+  c->fcap = c->scap = c->ecap = c->r2cap = 0;
+}
+
+int upload_polygon(gahhg_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  free_grid(c);
+  free_scratch(c);
+  dfree(c->d_pv);
+  dfree(c->d_pe);
+  dfree(c->d_edge);
+  c->d_pv = nullptr;
+  c->d_pe = nullptr;
+  c->d_edge = nullptr;
+  const int64_t N = c->poly.N;
+  RCK(dalloc(c, (void**)&c->d_pv, N * 24));
+  RCK(dalloc(c, (void**)&c->d_pe, N * 8));
+  RCK(dalloc(c, (void**)&c->d_edge, N * 72));
+  RCK(dalloc(c, (void**)&c->d_loc_cur, N * 4));
+  RCK(dalloc(c, (void**)&c->d_loc_prev, N * 4));
+  RCK(hip_check(c, hipMemcpy(c->d_pv, c->poly.pv.data(), N * 24, hipMemcpyHostToDevice), "polygon H2D"));
+  RCK(hip_check(c, hipMemcpy(c->d_pe, c->poly.pe.data(), N * 8, hipMemcpyHostToDevice), "polygon H2D"));
+  c->have_poly = true;
+  return GAHHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gahhg_params_default(gahhg_params* p) {
+  if (!p) return GAHHG_E_ARG;
+  memset(p, 0, sizeof(*p));
+  p->max_depth = 15;
+  p->auto_depth = 1;
+  p->block_create = 256;
+  p->block_classify = 256;
+  p->block_query = 256;
+  p->device = 0;
+  p->verbose = 0;
+  p->query_layout = 0;
+  return GAHHG_OK;
+}
+
+int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
+  if (!out) return GAHHG_E_ARG;
+  *out = nullptr;
+  gahhg_ctx* c = new (std::nothrow) gahhg_ctx();
+  if (!c) return GAHHG_E_NOMEM;
+  if (p)
+    c->prm = *p;
+  else
+    gahhg_params_default(&c->prm);
+  if (c->prm.max_depth < 1 || c->prm.max_depth > kMaxLevels - 1 || c->prm.block_query < 64 ||
+      c->prm.block_query > 1024 || (c->prm.block_query % 64) != 0) {
+    delete c;
+    return GAHHG_E_ARG;
+  }
+  c->device = c->prm.device;
+  memset(&c->st, 0, sizeof(c->st));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
+    delete c;
+    return GAHHG_E_HIP;
+  }
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return GAHHG_E_HIP;
+  }
+  for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
+  if (hipMalloc((void**)&c->d_st, sizeof(DevState)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return GAHHG_E_HIP;
+  }
+  *out = c;
+  return GAHHG_OK;
+}
+
+void gahhg_destroy(gahhg_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_grid(c);
+  free_scratch(c);
+  dfree(c->d_pv);
+  dfree(c->d_pe);
+  dfree(c->d_edge);
+  dfree(c->d_st);
+  for (void* p : c->user_allocs) dfree(p);
+  if (c->h_st) (void)hipHostFree(c->h_st);
+  for (int i = 0; i < 4; i++)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* gahhg_last_error(const gahhg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
+  RCK(ctx_ok(c));
+  if (!path) return set_err(c, GAHHG_E_ARG, "null path");
+  RCK(select_device(c));
+  Polygon p;
+  int rc = load_polygon_txt(path, &p, &c->err);
+  if (rc) return rc;
+  c->poly = std::move(p);
+  return upload_polygon(c);
+}
+
+int gahhg_set_polygon_lonlat(gahhg_ctx* c, const double* lon, const double* lat, const int64_t* ring_off,
+                             int32_t n_rings) {
+  RCK(ctx_ok(c));
+  RCK(select_device(c));
+  Polygon p;
+  int rc = polygon_from_lonlat(lon, lat, ring_off, n_rings, &p, &c->err);
+  if (rc) return rc;
+  c->poly = std::move(p);
+  return upload_polygon(c);
+}
+
+int gahhg_build(gahhg_ctx* c) {
+  RCK(ctx_ok(c));
+  if (!c->have_poly) return set_err(c, GAHHG_E_STATE, "build: no polygon loaded");
+  RCK(select_device(c));
+  (void)hipStreamSynchronize(c->stream);
+  free_grid(c);
+  const double t0 = now_ms();
+  RCK(build_hierarchy(c));
+  const double t1 = now_ms();
+  RCK(classify_hierarchy(c));
+  RCK(pack_query_layout(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  const double t2 = now_ms();
+  c->built = true;
+  c->st.create_ms = t1 - t0;
+  c->st.classify_ms = t2 - t1;
+  c->st.pre_ms = t2 - t0;
+  c->st.depth = c->gnum;
+  c->st.gamma = c->gamma;
+  c->st.n_edges = c->poly.N;
+  const LevelDesc& L = c->lv[c->gnum - 1];
+  c->st.vnum = (int64_t)L.vstart + L.vnum;
+  c->st.bnum = (int64_t)L.bstart + L.bnum;
+  c->st.cnum = (int64_t)L.cstart + L.cnum;
+  c->st.rnum = L.rnum;
+  c->st.vspace = 24 * c->st.vnum;
+  c->st.bspace = 48 * c->st.bnum;
+  c->st.cspace = 28 * c->st.cnum;
+  c->st.rspace = 4 * c->st.rnum;
+  c->st.gpu_space_bytes = c->st.vspace + c->st.bspace + c->st.cspace + c->st.rspace;
+  // sizeof(GAHHG) + gnum * sizeof(GAHHGGrid) + sizeof(GPISPParam) +
+  // sizeof(GPISPStat) on the reference's 64-bit build (144 + 28 gnum + 552
+  // + 96), GPISPAHHG.cu:2100.
+  c->st.cpu_space_bytes = 792 + 28LL * c->gnum;
+  c->st.orientation_flipped = c->poly.flips;
+  const DevState& s = *c->h_st;
+  c->st.walk_steps_mark = (int64_t)s.steps_mark;
+  c->st.walk_steps_dispatch = (int64_t)s.steps_dispatch;
+  c->st.walk_partner_missing = (int64_t)s.partner_missing;
+  c->st.walk_step_cap_hits = (int64_t)s.cap_hits;
+  c->st.walk_unlocated = (int64_t)s.unlocated;
+  c->st.mark_passes = c->gnum - 1 + ((c->gnum < c->prm.max_depth || c->prm.max_depth == 1) ? 1 : 0);
+  if (c->prm.max_depth == 1) c->st.mark_passes = 0;
+  return GAHHG_OK;
+}
+
+int gahhg_query_xyz_device(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
+  if (n < 0 || (n > 0 && (!d_xyz || !d_labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  RCK(select_device(c));
+  RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
+  RCK(run_query(c, d_xyz, n, d_labels));
+  RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
+  RCK(hip_check(c, hipEventSynchronize(c->ev[1]), "query sync"));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+  c->st.query_ms = ms;
+  c->st.n_queries = n;
+  c->st.query_unit_s = n > 0 ? ms * 1e-3 / (double)n : 0.0;
+  // label census (outside the timed region)
+  unsigned long long* d_two = nullptr;
+  RCK(dalloc(c, (void**)&d_two, 16));
+  (void)hipMemsetAsync(d_two, 0, 16, c->stream);
+  int rc = count_labels(c, d_labels, n, d_two);
+  unsigned long long h_two[2] = {0, 0};
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(h_two, d_two, 16, hipMemcpyDeviceToHost, c->stream), "census");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "census sync");
+  dfree(d_two);
+  if (rc) return rc;
+  c->st.inside = (int64_t)h_two[0];
+  c->st.unlocated = (int64_t)h_two[1];
+  c->st.outside = n - c->st.inside;
+  return GAHHG_OK;
+}
+
+int gahhg_query_xyz(gahhg_ctx* c, const double* xyz, int64_t n, int32_t* labels) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
+  if (n < 0 || (n > 0 && (!xyz || !labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  if (n == 0) return GAHHG_OK;
+  RCK(select_device(c));
+  double* d_xyz = nullptr;
+  int32_t* d_lab = nullptr;
+  RCK(dalloc(c, (void**)&d_xyz, n * 24));
+  int rc = dalloc(c, (void**)&d_lab, n * 4);
+  if (!rc) rc = hip_check(c, hipMemcpy(d_xyz, xyz, n * 24, hipMemcpyHostToDevice), "query H2D");
+  if (!rc) rc = gahhg_query_xyz_device(c, d_xyz, n, d_lab);
+  if (!rc) rc = hip_check(c, hipMemcpy(labels, d_lab, n * 4, hipMemcpyDeviceToHost), "labels D2H");
+  dfree(d_xyz);
+  dfree(d_lab);
+  return rc;
+}
+
+int gahhg_query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz) {
+  if (n < 0 || (n > 0 && (!lon || !lat || !xyz))) return GAHHG_E_ARG;
+  query_lonlat_to_xyz(lon, lat, n, xyz);
+  return GAHHG_OK;
+}
+
+int gahhg_query_lonlat(gahhg_ctx* c, const double* lon, const double* lat, int64_t n, int32_t* labels) {
+  RCK(ctx_ok(c));
+  if (n < 0 || (n > 0 && (!lon || !lat || !labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  std::vector<double> xyz((size_t)(3 * n));
+  query_lonlat_to_xyz(lon, lat, n, xyz.data());
+  return gahhg_query_xyz(c, xyz.data(), n, labels);
+}
+
+int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n) {
+  if (!path || !n) return GAHHG_E_ARG;
+  std::vector<double> a, b;
+  std::string err;
+  int rc = read_query_csv(path, &a, &b, &err);
+  if (rc) return rc;
+  *n = (int64_t)a.size();
+  if (lon && lat) {
+    if (cap < (int64_t)a.size()) return GAHHG_E_ARG;
+    std::copy(a.begin(), a.end(), lon);
+    std::copy(b.begin(), b.end(), lat);
+  }
+  return GAHHG_OK;
+}
+
+int gahhg_get_stats(gahhg_ctx* c, gahhg_stats* out) {
+  RCK(ctx_ok(c));
+  if (!out) return GAHHG_E_ARG;
+  *out = c->st;
+  int64_t bytes = 0;
+  bytes += c->g.vcap * 24 + c->g.bcap * (6 * 4 + 24) + c->g.ccap * 7 * 4 + c->g.rcap * 4;
+  bytes += c->q.bytes;
+  bytes += (int64_t)c->poly.N * (24 + 8 + 72 + 8);
+  out->device_bytes = bytes;
+  return GAHHG_OK;
+}
+
+int gahhg_write_report(gahhg_ctx* c, const char* path, const char* poly_name, const char* query_name) {
+  RCK(ctx_ok(c));
+  FILE* fp = path ? fopen(path, "a") : stdout;
+  if (!fp) return set_err(c, GAHHG_E_IO, "Can not open file %s", path);
+  time_t t = time(nullptr);
+  struct tm lt;
+  localtime_r(&t, &lt);
+  fprintf(fp, "\n----------------------------------------%d/%d/%d %d:%d:%d-----------------------------------------\n",
+          lt.tm_year + 1900, lt.tm_mon + 1, lt.tm_mday, lt.tm_hour, lt.tm_min, lt.tm_sec);
+  fprintf(fp, "\nPolygon file: %s\n", poly_name ? poly_name : "");
+  fprintf(fp, "Number of polygon vertice: %d\n", c->poly.N);
+  fprintf(fp, "Query point file: %s\n", query_name ? query_name : "");
+  fprintf(fp, "Number of query points: %lld\n", (long long)c->st.n_queries);
+  fprintf(fp, "Depth of GAHHG: %d\n", c->st.depth);
+  fprintf(fp, "Time of preprocessing: %f seconds (creat GAHHG: %f seconds, classify GAHHG: %f seconds)\n",
+          c->st.pre_ms * 1e-3, c->st.create_ms * 1e-3, c->st.classify_ms * 1e-3);
+  fprintf(fp, "Storage overhead of GAHHG on CPU: %lld Bytes\n", (long long)c->st.cpu_space_bytes);
+  fprintf(fp, "Storage overhead of GAHHG on GPU: %lld KB\n", (long long)(c->st.gpu_space_bytes / 1000));
+  fprintf(fp, "Total test time for all query points: %f seconds\n", c->st.query_ms * 1e-3);
+  fprintf(fp, "Average test time per query point: %.5e seconds\n", c->st.query_unit_s);
+  fprintf(fp, "Test result: inside %lld, outside %lld\n", (long long)c->st.inside, (long long)c->st.outside);
+  fprintf(fp, "\n---------------------------------------------------------------------------------------------------\n");
+  if (path) fclose(fp);
+  return GAHHG_OK;
+}
+
+int gahhg_write_result_csv(const char* path, const int32_t* labels, int64_t n) {
+  if (!path || (n > 0 && !labels)) return GAHHG_E_ARG;
+  FILE* fp = fopen(path, "w");
+  if (!fp) return GAHHG_E_IO;
+  for (int64_t i = 0; i < n; i++) fputs(labels[i] == kIn ? "1\n" : "0\n", fp);
+  fclose(fp);
+  return GAHHG_OK;
+}
+
+int gahhg_device_alloc(gahhg_ctx* c, int64_t bytes, void** d_ptr) {
+  RCK(ctx_ok(c));
+  if (!d_ptr || bytes < 0) return GAHHG_E_ARG;
+  RCK(select_device(c));
+  RCK(dalloc(c, d_ptr, bytes));
+  c->user_allocs.push_back(*d_ptr);
+  return GAHHG_OK;
+}
+
+int gahhg_device_free(gahhg_ctx* c, void* p) {
+  RCK(ctx_ok(c));
+  auto it = std::find(c->user_allocs.begin(), c->user_allocs.end(), p);
+  if (it == c->user_allocs.end()) return set_err(c, GAHHG_E_ARG, "device_free: unknown pointer");
+  c->user_allocs.erase(it);
+  RCK(select_device(c));
+  (void)hipStreamSynchronize(c->stream);
+  dfree(p);
+  return GAHHG_OK;
+}
+
+int gahhg_memcpy_h2d(gahhg_ctx* c, void* d, const void* h, int64_t bytes) {
+  RCK(ctx_ok(c));
+  RCK(select_device(c));
+  return hip_check(c, hipMemcpy(d, h, (size_t)bytes, hipMemcpyHostToDevice), "memcpy H2D");
+}
+
+int gahhg_memcpy_d2h(gahhg_ctx* c, void* h, const void* d, int64_t bytes) {
+  RCK(ctx_ok(c));
+  RCK(select_device(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  return hip_check(c, hipMemcpy(h, d, (size_t)bytes, hipMemcpyDeviceToHost), "memcpy D2H");
+}
+
+int gahhg_grid_counts(gahhg_ctx* c, int64_t* counts) {
+  RCK(ctx_ok(c));
+  if (!counts) return GAHHG_E_ARG;
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
+  counts[0] = c->gnum;
+  counts[1] = c->st.vnum;
+  counts[2] = c->st.bnum;
+  counts[3] = c->st.cnum;
+  counts[4] = c->st.rnum;
+  return GAHHG_OK;
+}
+
+int gahhg_export_grid(gahhg_ctx* c, double* v, int32_t* cells, int32_t* borders, double* normals,
+                      int32_t* refs, int32_t* levels) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
+  RCK(select_device(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  const int64_t V = c->st.vnum, B = c->st.bnum, C = c->st.cnum, R = c->st.rnum;
+  auto d2h = [&](void* h, const void* d, int64_t bytes) {
+    return hip_check(c, hipMemcpy(h, d, (size_t)bytes, hipMemcpyDeviceToHost), "export");
+  };
+  if (v) RCK(d2h(v, c->g.v, V * 24));
+  if (normals) RCK(d2h(normals, c->g.b_n, B * 24));
+  if (refs && R > 0) RCK(d2h(refs, c->g.refs, R * 4));
+  std::vector<int32_t> col;
+  if (cells) {
+    col.resize((size_t)std::max<int64_t>(C, 1));
+    int32_t* cs[7] = {c->g.c_bstart, c->g.c_bnum, c->g.c_rstart, c->g.c_rnum, c->g.c_upcell, c->g.c_subcell,
+                      c->g.c_type};
+    for (int k = 0; k < 7; k++) {
+      RCK(d2h(col.data(), cs[k], C * 4));
+      for (int64_t i = 0; i < C; i++) cells[7 * i + k] = col[i];
+    }
+  }
+  if (borders) {
+    col.resize((size_t)std::max<int64_t>(B, 1));
+    int32_t* bs[6] = {c->g.b_parent, c->g.b_partner, c->g.b_v0, c->g.b_v1, c->g.b_subcell, c->g.b_id};
+    for (int k = 0; k < 6; k++) {
+      RCK(d2h(col.data(), bs[k], B * 4));
+      for (int64_t i = 0; i < B; i++) borders[6 * i + k] = col[i];
+    }
+  }
+  if (levels)
+    for (int l = 0; l < c->gnum; l++) memcpy(levels + 7 * l, &c->lv[l], 7 * sizeof(int32_t));
+  return GAHHG_OK;
+}
+
+int gahhg_kat_device(gahhg_ctx* c, int32_t op, const double* in12, int64_t n, double* out3) {
+  RCK(ctx_ok(c));
+  if (op < 0 || op > 3 || n < 0 || (n > 0 && (!in12 || !out3))) return GAHHG_E_ARG;
+  if (n == 0) return GAHHG_OK;
+  RCK(select_device(c));
+  double *di = nullptr, *dout = nullptr;
+  RCK(dalloc(c, (void**)&di, n * 96));
+  int rc = dalloc(c, (void**)&dout, n * 24);
+  if (!rc) rc = hip_check(c, hipMemcpy(di, in12, n * 96, hipMemcpyHostToDevice), "kat H2D");
+  if (!rc) rc = run_kat(c, op, di, n, dout);
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "kat sync");
+  if (!rc) rc = hip_check(c, hipMemcpy(out3, dout, n * 24, hipMemcpyDeviceToHost), "kat D2H");
+  dfree(di);
+  dfree(dout);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,336 @@
+// gahhg_classify.hip -- leaf labelling on gfx950.
+//
+// GPISP::classifyAHHG / GAHHG::classifyBottomGrid
+// (source/GPISPAHHG.cu:1787-1817,2018-2059) with kernels
+// setBottomNECell (:1046-1071), setBottomDECell (:1074-1124),
+// setDECell (:1292-1363), setIECell (:1366-1392), setTopECell (:1395-1419).
+//
+// The reference's propagation kernels read neighbour types while other
+// threads write them, so which cells end up labelled depends on scheduling.
+// Here every propagation pass reads a snapshot taken before the pass
+// (double-buffered types), and the adoption passes repeat until nothing
+// changes.  The values written are the reference's; only coverage is made
+// deterministic (and complete where the reference's single pass stops).
+#include <float.h>
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "gahhg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gahhg {
+
+constexpr int kCBlock = 256;
+
+static inline unsigned cblocks(int64_t n) {
+  return (unsigned)std::max<int64_t>((n + kCBlock - 1) / kCBlock, 1);
+}
+
+__device__ __forceinline__ V3 cvtx(const GridView& g, int i) { return load3(g.v + 3 * (int64_t)i); }
+
+// Leaf centre as the query and classification kernels compute it: the mean
+// of the border start vertices, normalised (:1062-1066, :1481-1485).
+__device__ __forceinline__ V3 mean_center(const GridView& g, int cid) {
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  V3 c = v3(0.0, 0.0, 0.0);
+  for (int i = 0; i < bn; i++) c = add(c, cvtx(g, g.b_v0[bs + i]));
+  c = divk(c, bn);
+  return unit(c);
+}
+
+// device_getCPProp, :61-171: side of the centre relative to the nearest
+// polygon feature among the cell's refs.
+__device__ int cp_prop(const GridView& g, const double* __restrict__ pv, const int32_t* __restrict__ pe,
+                       const double* __restrict__ edge, int cell, V3 center) {
+  double mindist = FLT_MAX;
+  const int rs = g.c_rstart[cell], rn = g.c_rnum[cell];
+  int ne = g.refs[rs];
+  V3 ne_normal = v3(0.0, 0.0, 0.0);
+  int nv_index = -1;
+  for (int i = 0; i < rn; i++) {
+    const int se = g.refs[rs + i];
+    const double* er = edge + 9 * (int64_t)se;
+    const V3 v0 = load3(er), v1 = load3(er + 3), n0 = load3(er + 6);
+    double curdist;
+    V3 n1 = unit(cross(center, n0));
+    V3 inter = unit(cross(n0, n1));
+    if (lt0(dot(inter, v0))) inter = neg(inter);
+    V3 t0 = unit(sub(v0, inter));
+    V3 t1 = unit(sub(v1, inter));
+    const double t3 = dot(t0, t1);
+    int cur_nv;
+    if (lt0(t3)) {
+      curdist = sqlen(sub(inter, center));
+      cur_nv = -1;
+    } else {
+      curdist = sqlen(sub(center, v0));
+      const double vd = sqlen(sub(center, v1));
+      if (vd < curdist) {
+        curdist = vd;
+        cur_nv = pe[2 * se + 1];
+      } else {
+        cur_nv = pe[2 * se];
+      }
+    }
+    if (nv_index != -1 && nv_index == cur_nv) {
+      double a = dot(ne_normal, center);
+      double b = dot(n0, center);
+      if ((le0(a) && ge0(b)) || (ge0(a) && le0(b))) {
+        // both nearest features are the shared vertex: keep the edge whose
+        // far end leans more towards the centre
+        const int oth0 = (pe[2 * ne] == nv_index) ? pe[2 * ne + 1] : pe[2 * ne];
+        const V3 pnv = load3(pv + 3 * (int64_t)nv_index);
+        V3 a0 = sub(load3(pv + 3 * (int64_t)oth0), pnv);
+        const int oth1 = (pe[2 * se] == nv_index) ? pe[2 * se + 1] : pe[2 * se];
+        V3 a1 = sub(load3(pv + 3 * (int64_t)oth1), pnv);
+        V3 bb = sub(center, pnv);
+        a0 = unit(a0);
+        a1 = unit(a1);
+        a = dot(a0, bb);
+        b = dot(a1, bb);
+        if (b > a) {
+          mindist = curdist;
+          ne = se;
+          ne_normal = n0;
+          nv_index = cur_nv;
+          continue;
+        }
+      }
+    }
+    if (curdist < mindist) {
+      mindist = curdist;
+      ne = se;
+      ne_normal = n0;
+      nv_index = cur_nv;
+    }
+  }
+  return gt0(dot(center, ne_normal)) ? kIn : kOut;
+}
+
+// Crossing parity from the centre of `from` to the midpoint of border `mb`
+// over the refs of `from` (dev_isInterSeg, :174-205), giving the label on
+// the far side.
+__device__ int parity_label(const GridView& g, const double* __restrict__ edge, int from, int mb,
+                            int from_type) {
+  const V3 m = unit(mul(add(cvtx(g, g.b_v0[mb]), cvtx(g, g.b_v1[mb])), 0.5));
+  const V3 c = mean_center(g, from);
+  const V3 n1 = arc_normal(c, m);
+  int cnt = 0;
+  const int rs = g.c_rstart[from], rn = g.c_rnum[from];
+  for (int i = 0; i < rn; i++) {
+    const double* er = edge + 9 * (int64_t)g.refs[rs + i];
+    const V3 u0 = load3(er), u1 = load3(er + 3), n0 = load3(er + 6);
+    if (same_side(dot(n0, c), dot(n0, m))) continue;
+    if (same_side(dot(n1, u0), dot(n1, u1))) continue;
+    cnt++;
+  }
+  return (cnt % 2 == 0) ? from_type : 3 - from_type;
+}
+
+__global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const double* pv, const int32_t* pe,
+                                                const double* edge, int cstart, int cnum) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  if (g.c_rnum[cid] != 0) g.c_type[cid] = cp_prop(g, pv, pe, edge, cid, mean_center(g, cid));
+}
+
+__global__ void k_snap(const int32_t* type, int cstart, int cnum, int32_t* snap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < cnum) snap[t] = type[cstart + t];
+}
+
+// setBottomDECell: every border whose neighbour was labelled in the
+// snapshot decides in turn; the last one wins, as in the reference loop.
+__global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const double* edge, int cstart,
+                                                       int cnum, const int32_t* snap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum || snap[t] != kUndefined) return;
+  const int cid = cstart + t;
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  int lab = kUndefined;
+  bool set = false;
+  for (int i = 0; i < bn; i++) {
+    const int pe_ = g.b_partner[bs + i];
+    if (pe_ == -1) continue;
+    const int nb = g.b_parent[pe_];
+    const int nt = snap[nb - cstart];
+    if (nt == kUndefined) continue;
+    lab = parity_label(g, edge, nb, pe_, nt);
+    set = true;
+  }
+  if (set) g.c_type[cid] = lab;
+}
+
+// setIECell round: an unlabelled leaf adopts the label of the first
+// neighbour that is an empty labelled cell.
+__global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum, const int32_t* snap,
+                                                DevState* st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  if (snap[t] != kUndefined || g.c_subcell[cid] != -1) return;
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  for (int i = 0; i < bn; i++) {
+    const int pe_ = g.b_partner[bs + i];
+    if (pe_ == -1) continue;
+    const int pc = g.b_parent[pe_];
+    const int pt = snap[pc - cstart];
+    if (g.c_rnum[pc] == 0 && pt != kUndefined) {
+      g.c_type[cid] = pt;
+      st->changed = 1;
+      return;
+    }
+  }
+}
+
+// setDECell: an empty unlabelled cell above the bottom takes its label from
+// the leaf reached through the first refined neighbour.
+__global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const double* edge, int cstart, int cnum) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  if (!(g.c_rnum[cid] == 0 && g.c_type[cid] == kUndefined)) return;
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  int pe_ = -1, pc = -1;
+  for (int i = 0; i < bn; i++) {
+    pe_ = g.b_partner[bs + i];
+    if (pe_ != -1 && g.c_subcell[g.b_parent[pe_]] != -1) {
+      pc = g.b_parent[pe_];
+      break;
+    }
+  }
+  if (pc == -1) return;
+  int target = g.b_subcell[pe_];
+  if (target < 0) return;
+  while (g.c_subcell[target] != -1) target = g.c_subcell[target];
+  if (g.c_rnum[target] == 0)
+    g.c_type[cid] = g.c_type[target];
+  else
+    g.c_type[cid] = parity_label(g, edge, target, pe_, g.c_type[target]);
+}
+
+// setTopECell on the 42 level-0 cells, repeated to a fixpoint in one block.
+__global__ void k_top(GridView g, int cnum) {
+  __shared__ int snap[64];
+  __shared__ int changed;
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t < cnum) snap[t] = g.c_type[t];
+    if (t == 0) changed = 0;
+    __syncthreads();
+    if (t < cnum && g.c_rnum[t] == 0 && snap[t] == kUndefined) {
+      const int bs = g.c_bstart[t], bn = g.c_bnum[t];
+      for (int i = 0; i < bn; i++) {
+        const int pe_ = g.b_partner[bs + i];
+        if (pe_ != -1 && snap[g.b_parent[pe_]] != kUndefined) {
+          g.c_type[t] = snap[g.b_parent[pe_]];
+          changed = 1;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    const int ch = changed;
+    __syncthreads();
+    if (!ch) break;
+  }
+}
+
+// Counts leaves of one level left unlabelled (diagnostic).
+__global__ void k_count_undef(GridView g, int cstart, int cnum, DevState* st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  if (g.c_subcell[cid] == -1 && g.c_type[cid] == kUndefined) atomicAdd(&st->undefined_leaves, 1ULL);
+}
+
+#define CCK(x)                       \
+  do {                               \
+    int _rc = (x);                   \
+    if (_rc != GAHHG_OK) return _rc; \
+  } while (0)
+
+__global__ void k_clear_changed(DevState* st) { st->changed = 0; }
+__global__ void k_clear_undef(DevState* st) { st->undefined_leaves = 0; }
+
+// setIECell to a fixpoint.  Rounds run in batches; the change flag is
+// cleared before the last round of a batch, so a clear flag after the batch
+// means the last round changed nothing.  Returns the number of leaves still
+// unlabelled after the first round via *after_first.
+static int ie_fixpoint(gahhg_ctx* c, int lv, int32_t* snap, unsigned long long* after_first) {
+  const LevelDesc L = c->lv[lv];
+  GridView g = view(c);
+  bool first = true;
+  for (int batch = 0; batch < 1 << 20; batch++) {
+    const int rounds = first ? 1 : 4;
+    for (int r = 0; r < rounds; r++) {
+      if (r == rounds - 1) hipLaunchKernelGGL(k_clear_changed, dim3(1), dim3(1), 0, c->stream, c->d_st);
+      hipLaunchKernelGGL(k_snap, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream,
+                         (const int32_t*)c->g.c_type, L.cstart, L.cnum, snap);
+      hipLaunchKernelGGL(k_ie, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, L.cstart, L.cnum,
+                         (const int32_t*)snap, c->d_st);
+    }
+    if (first) {
+      hipLaunchKernelGGL(k_clear_undef, dim3(1), dim3(1), 0, c->stream, c->d_st);
+      hipLaunchKernelGGL(k_count_undef, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, L.cstart,
+                         L.cnum, c->d_st);
+    }
+    CCK(hip_check(c, hipGetLastError(), "classify fixpoint"));
+    CCK(hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream),
+                  "readback"));
+    CCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+    if (first) *after_first = c->h_st->undefined_leaves;
+    first = false;
+    if (!c->h_st->changed) return GAHHG_OK;
+  }
+  return set_err(c, GAHHG_E_STATE, "classification did not converge");
+}
+
+int classify_hierarchy(gahhg_ctx* c) {
+  const int Lb = c->gnum - 1;
+  const LevelDesc LB = c->lv[Lb];
+  GridView g = view(c);
+  int64_t maxc = 0;
+  for (int l = 0; l < c->gnum; l++) maxc = std::max<int64_t>(maxc, c->lv[l].cnum);
+  // snapshot buffer: the flag scratch holds at least max(cnum) entries
+  int32_t* snap = c->d_f0;
+  if (c->fcap < maxc) return set_err(c, GAHHG_E_STATE, "classify: scratch too small");
+  unsigned long long und_snap = 0, und1 = 0;
+  if (c->prm.verbose) printf("Classify level %d cells.\n", Lb);
+  hipLaunchKernelGGL(k_ne, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream, g, (const double*)c->d_pv,
+                     (const int32_t*)c->d_pe, (const double*)c->d_edge, LB.cstart, LB.cnum);
+  hipLaunchKernelGGL(k_snap, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream,
+                     (const int32_t*)c->g.c_type, LB.cstart, LB.cnum, snap);
+  hipLaunchKernelGGL(k_de_bottom, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream, g,
+                     (const double*)c->d_edge, LB.cstart, LB.cnum, (const int32_t*)snap);
+  CCK(hip_check(c, hipGetLastError(), "classify bottom"));
+  CCK(ie_fixpoint(c, Lb, snap, &und1));
+  und_snap += und1;
+  for (int lv = Lb - 1; lv >= 0; lv--) {
+    if (c->prm.verbose) printf("Classify level %d cells.\n", lv);
+    const LevelDesc L = c->lv[lv];
+    hipLaunchKernelGGL(k_de_upper, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g,
+                       (const double*)c->d_edge, L.cstart, L.cnum);
+    CCK(hip_check(c, hipGetLastError(), "classify upper"));
+    CCK(ie_fixpoint(c, lv, snap, &und1));
+    und_snap += und1;
+  }
+  hipLaunchKernelGGL(k_top, dim3(1), dim3(64), 0, c->stream, g, c->lv[0].cnum);
+  hipLaunchKernelGGL(k_clear_undef, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  for (int l = 0; l < c->gnum; l++)
+    hipLaunchKernelGGL(k_count_undef, dim3(cblocks(c->lv[l].cnum)), dim3(kCBlock), 0, c->stream, g,
+                       c->lv[l].cstart, c->lv[l].cnum, c->d_st);
+  CCK(hip_check(c, hipGetLastError(), "classify top"));
+  CCK(hip_check(c, hipEventRecord(c->ev[2], c->stream), "event"));
+  CCK(hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream),
+                "readback"));
+  CCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  c->st.undefined_leaves_snapshot = (int64_t)und_snap;
+  c->st.undefined_leaves_final = (int64_t)c->h_st->undefined_leaves;
+  return GAHHG_OK;
+}
+
+}  // namespace gahhg

@@ -1,0 +1,195 @@
+// gahhg_internal.h -- context, device grid layout and cross-file entry points
+// of the gfx950 GAHHG library.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gahhg.h"
+#include "gahhg_geom.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#else
+typedef struct ihipStream_t* hipStream_t;
+typedef struct ihipEvent_t* hipEvent_t;
+#endif
+
+namespace gahhg {
+
+constexpr int kMaxLevels = 32;
+constexpr int kGrid0Cells = 42;
+constexpr int kGrid0Verts = 80;
+constexpr int kGrid0Borders = 240;
+
+// Per-level index ranges; GAHHGGrid, source/GPISPAHHG.cuh:12-20.
+struct LevelDesc {
+  int32_t vstart, vnum, bstart, bnum, cstart, cnum, rnum;
+};
+
+// Device-side counters and the level table, read back by one copy per level.
+struct DevState {
+  int32_t U;        // crossed cells at the level being refined
+  int32_t nv;       // sum of bnum over them (= new vertices = centre borders)
+  int32_t e0;       // borders of border children
+  int32_t c0;       // border children
+  int32_t R;        // bottom-level refs
+  int32_t changed;  // classification round changed something
+  int32_t pad[2];
+  unsigned long long steps_mark, steps_dispatch, partner_missing, cap_hits, unlocated, bad_index;
+  unsigned long long zero_ref_edges, undefined_leaves;
+  LevelDesc lv[kMaxLevels];
+};
+
+// Structure-of-arrays hierarchy in HBM.  Vertices and border normals are
+// stored as packed xyz triples (24 B) because every consumer gathers whole
+// points; topology fields are separate int32 arrays.
+struct DeviceGrid {
+  int64_t vcap = 0, bcap = 0, ccap = 0;
+  double* v = nullptr;  // 3 per vertex
+  int32_t* b_parent = nullptr;
+  int32_t* b_partner = nullptr;
+  int32_t* b_v0 = nullptr;
+  int32_t* b_v1 = nullptr;
+  int32_t* b_subcell = nullptr;
+  int32_t* b_id = nullptr;
+  double* b_n = nullptr;  // 3 per border
+  int32_t* c_bstart = nullptr;
+  int32_t* c_bnum = nullptr;
+  int32_t* c_rstart = nullptr;
+  int32_t* c_rnum = nullptr;
+  int32_t* c_upcell = nullptr;
+  int32_t* c_subcell = nullptr;
+  int32_t* c_type = nullptr;
+  int64_t rcap = 0;
+  int32_t* refs = nullptr;
+};
+
+// Raw pointer bundle passed to kernels by value.
+struct GridView {
+  const double* v;
+  int32_t* b_parent;
+  int32_t* b_partner;
+  int32_t* b_v0;
+  int32_t* b_v1;
+  int32_t* b_subcell;
+  int32_t* b_id;
+  double* b_n;
+  int32_t* c_bstart;
+  int32_t* c_bnum;
+  int32_t* c_rstart;
+  int32_t* c_rnum;
+  int32_t* c_upcell;
+  int32_t* c_subcell;
+  int32_t* c_type;
+  int32_t* refs;
+};
+
+// Packed, read-only layout the query kernel walks (built after
+// classification).  One 32-byte node per cell:
+//   nd[0] = bstart, nd[1] = bnum | (rnum > 0) << 3 | type << 4,
+//   nd[2] = subcell (centre child) or -1, nd[3] = first ref slot,
+//   nd[4] = rnum, nd[5..7] = unused
+// Border normals are copied per cell in border order (qn, 3 doubles each)
+// with the child-cell ids (qsub).  Boundary leaves carry their centre (qc)
+// and an edge list (qe: a, b, n per ref; 9 doubles) plus, per ref, the
+// edge-plane side of the centre (qside: bit0 = ret2 < -eps, bit1 = ret2 >
+// eps), all computed with the same operations the reference uses inline.
+struct QueryLayout {
+  int64_t ncell = 0, nref = 0, nleaf = 0;
+  int32_t* nd = nullptr;     // 8 ints per cell
+  double* qc = nullptr;      // 4 doubles per cell (centre xyz, pad)
+  double* qe = nullptr;      // 9 doubles per ref slot
+  uint8_t* qside = nullptr;  // per ref slot
+  int64_t bytes = 0;
+};
+
+struct Polygon {
+  int32_t N = 0;
+  int32_t flips = 0;
+  std::vector<double> pv;  // 3N
+  std::vector<int32_t> pe; // 2N
+};
+
+// ---- host (gahhg_host.cpp) ----
+int load_polygon_txt(const char* path, Polygon* out, std::string* err);
+int polygon_from_lonlat(const double* lon, const double* lat, const int64_t* ring_off,
+                        int32_t n_rings, Polygon* out, std::string* err);
+void build_grid0(double* v80, int32_t* bi240x6, double* bn240x3, int32_t* ci42x7);
+void query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
+int read_query_csv(const char* path, std::vector<double>* lon, std::vector<double>* lat,
+                   std::string* err);
+
+}  // namespace gahhg
+
+struct gahhg_ctx {
+  gahhg_params prm;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  gahhg::Polygon poly;
+  bool have_poly = false;
+  double* d_pv = nullptr;
+  int32_t* d_pe = nullptr;
+  double* d_edge = nullptr;  // 9 per edge: a, b, arc_normal(a, b)
+
+  gahhg::DeviceGrid g;
+  gahhg::LevelDesc lv[gahhg::kMaxLevels];
+  int32_t gnum = 0;
+  float gamma = 0.f;
+  gahhg::DevState* d_st = nullptr;
+  gahhg::DevState* h_st = nullptr;  // pinned mirror
+  int32_t* d_loc_cur = nullptr;     // per edge: start cell at the last level
+  int32_t* d_loc_prev = nullptr;    // per edge: start cell one level up
+
+  // scratch
+  int64_t fcap = 0;
+  int32_t *d_f0 = nullptr, *d_f1 = nullptr, *d_o0 = nullptr, *d_o1 = nullptr;
+  int64_t scap = 0;
+  int32_t* d_scan = nullptr;
+  int64_t ecap = 0;
+  int32_t* d_ecnt = nullptr;  // per edge
+  int64_t* d_eoff = nullptr;  // unused placeholder for >2^31 refs
+  int64_t r2cap = 0;
+  int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr;
+
+  gahhg::QueryLayout q;
+  bool built = false;
+  gahhg_stats st;
+  double last_query_ms = 0.0;
+  std::vector<void*> user_allocs;
+};
+
+namespace gahhg {
+
+// ---- scan (gahhg_scan.hip) ----
+// Exclusive prefix sum of in[0..n) into out; n is n_bound, or *d_n when
+// d_n is non-null (elements past it read as 0 and are not written).  The
+// total goes to *d_total when non-null.  inclusive != 0 gives the inclusive
+// sum instead.
+int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, const int32_t* d_n,
+             int32_t* d_total, int inclusive);
+
+// ---- build / classify / query (gahhg_build.hip etc.) ----
+int build_hierarchy(gahhg_ctx* c);
+int classify_hierarchy(gahhg_ctx* c);
+int pack_query_layout(gahhg_ctx* c);
+int run_query(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels);
+int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
+
+GridView view(gahhg_ctx* c);
+int set_err(gahhg_ctx* c, int code, const char* fmt, ...);
+int hip_check(gahhg_ctx* c, int hip_status, const char* what);
+int dalloc(gahhg_ctx* c, void** p, int64_t bytes);
+void dfree(void* p);
+
+// Fills n int32 with v on the context stream (a kernel, so the call stays
+// graph-capturable).
+int fill_i32(gahhg_ctx* c, int32_t* p, int64_t n, int32_t v);
+int copy_i32(gahhg_ctx* c, int32_t* dst, const int32_t* src, int64_t n);
+
+}  // namespace gahhg

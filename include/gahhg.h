@@ -1,0 +1,161 @@
+/*
+ * gahhg.h -- C ABI of the MI355X (gfx950) GAHHG point-in-spherical-polygon
+ * library.
+ *
+ * The reference (superredlark2/GAHHG) has no FFI: its boundary is the
+ * in-process call sequence of source/GPISP.cu:47-67
+ *   loadPolyFromTXT -> loadTPFromCSV -> createAHHGInit -> createAHHG(maxDepth)
+ *   -> classifyAHHG -> test -> getStat -> exportStat2TXT -> exportResult2CSV
+ * plus its file formats.  Each entry point below names the reference routine
+ * it replaces.  Conventions: every function returns 0 or a negative
+ * GAHHG_E_* code and never aborts; gahhg_last_error() gives the message.
+ * Callers own host buffers; a context owns its device memory and one HIP
+ * stream.  A context is not thread-safe; separate contexts are independent.
+ * Labels use the reference's cell codes: 2 = IN, 1 = OUT, 0 = undefined or
+ * unlocated (source/GPISPParam.h:9-11).
+ */
+#ifndef GAHHG_H
+#define GAHHG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAHHG_OK 0
+#define GAHHG_E_ARG -1       /* bad argument                                */
+#define GAHHG_E_IO -2        /* file could not be opened                    */
+#define GAHHG_E_FORMAT -3    /* malformed polygon / query file              */
+#define GAHHG_E_STATE -4     /* call out of order (e.g. query before build) */
+#define GAHHG_E_HIP -5       /* HIP runtime error / no device               */
+#define GAHHG_E_NOMEM -6     /* allocation failed                           */
+#define GAHHG_E_LIMIT -7     /* sizes beyond int32 indexing                 */
+
+#define GAHHG_LABEL_UNDEFINED 0
+#define GAHHG_LABEL_OUT 1
+#define GAHHG_LABEL_IN 2
+
+typedef struct gahhg_ctx gahhg_ctx;
+
+/* Parameters: GPISPParam, source/GPISPParam.h:49-78 (maxDepth=15,
+ * autoDepth=true, blocksizes 256). */
+typedef struct {
+  int32_t max_depth;       /* levels, 1..15 (reference default 15)            */
+  int32_t auto_depth;      /* 1: stop refining once crossed cells / N > ~0.09 */
+  int32_t block_create;    /* threads per block of build kernels              */
+  int32_t block_classify;  /* threads per block of classification kernels     */
+  int32_t block_query;     /* threads per block of the query kernel           */
+  int32_t device;          /* HIP device ordinal                              */
+  int32_t verbose;         /* 1: print per-level progress like the reference  */
+  int32_t query_layout;    /* 0: packed query layout (default), 1: build SoA  */
+} gahhg_params;
+
+/* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by
+ * GPISP::getStat (source/GPISPAHHG.cu:2088-2113), plus diagnostics. */
+typedef struct {
+  double pre_ms;          /* create_ms + classify_ms                           */
+  double create_ms;       /* createAHHG: level loop + edge dispatch            */
+  double classify_ms;     /* classifyAHHG                                      */
+  double query_ms;        /* last query: kernel time only                      */
+  double query_unit_s;    /* query seconds per point                           */
+  int32_t depth;          /* levels built (gnum)                               */
+  float gamma;            /* last crossed-cells / N ratio                      */
+  int64_t n_edges;        /* polygon edges N                                   */
+  int64_t n_queries;      /* points in the last query                          */
+  int64_t vnum, bnum, cnum, rnum;
+  int64_t vspace, bspace, cspace, rspace; /* 24 V, 48 B, 28 C, 4 R bytes      */
+  int64_t gpu_space_bytes;/* reference space formula sum                       */
+  int64_t cpu_space_bytes;/* reference host bookkeeping bytes                  */
+  int64_t device_bytes;   /* device memory this context holds                  */
+  int64_t inside;         /* labels == IN in the last query                    */
+  int64_t outside;        /* n_queries - inside                                */
+  int64_t unlocated;      /* points that reached no leaf                       */
+  int32_t orientation_flipped; /* bit r: ring r reversed on load               */
+  int32_t mark_passes;    /* marking walks run                                 */
+  int64_t walk_steps_mark;
+  int64_t walk_steps_dispatch;
+  int64_t walk_partner_missing;
+  int64_t walk_step_cap_hits;
+  int64_t walk_unlocated;
+  int64_t zero_ref_edges;
+  int64_t undefined_leaves_snapshot;
+  int64_t undefined_leaves_final;
+} gahhg_stats;
+
+/* Fills defaults (GPISPParam::GPISPParam, source/GPISPParam.h:66-77). */
+int gahhg_params_default(gahhg_params* p);
+
+/* Creates a context on p->device (NULL = defaults). */
+int gahhg_create(gahhg_ctx** out, const gahhg_params* p);
+void gahhg_destroy(gahhg_ctx* ctx);
+const char* gahhg_last_error(const gahhg_ctx* ctx);
+
+/* Replaces GPISPPolygon::loadPolyFromTXT (source/GPISPPolygon.cu:32-278):
+ * polygon ring TXT file, outer ring first, orientation normalised. */
+int gahhg_load_polygon_txt(gahhg_ctx* ctx, const char* path);
+
+/* Same polygon from arrays: lon/lat degrees; ring_off[n_rings+1] vertex
+ * offsets; each ring includes its closing duplicate vertex (as in the file);
+ * ring 0 is the outer ring. */
+int gahhg_set_polygon_lonlat(gahhg_ctx* ctx, const double* lon, const double* lat,
+                             const int64_t* ring_off, int32_t n_rings);
+
+/* Replaces GPISP::createAHHGInit + createAHHG(maxDepth) + classifyAHHG
+ * (source/GPISPAHHG.cu:1947-2059): builds and classifies the hierarchy on
+ * the GPU. */
+int gahhg_build(gahhg_ctx* ctx);
+
+/* Replaces GPISPTP::loadTPFromCSV (source/GPISPTP.cu:13-79) + GPISP::test
+ * (source/GPISPAHHG.cu:2062-2085).  Host lon/lat degrees in, labels out. */
+int gahhg_query_lonlat(gahhg_ctx* ctx, const double* lon, const double* lat, int64_t n,
+                       int32_t* labels);
+/* Host XYZ (n x 3 doubles, y up) in, labels out. */
+int gahhg_query_xyz(gahhg_ctx* ctx, const double* xyz, int64_t n, int32_t* labels);
+/* Device-resident XYZ and labels (pointers from gahhg_device_alloc or any
+ * device allocation on the context's device); runs on the context stream
+ * and waits for it.  This is GPISP::test's timed region. */
+int gahhg_query_xyz_device(gahhg_ctx* ctx, const double* d_xyz, int64_t n, int32_t* d_labels);
+
+/* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
+ * NULL it returns the count in *n. */
+int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);
+/* Query lon/lat -> XYZ exactly as the reference converts test points
+ * (FakeLLVertex round trip, source/GPISPPolygon.cuh:72,82-92). */
+int gahhg_query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
+
+/* GPISP::getStat (source/GPISPAHHG.cu:2088-2113). */
+int gahhg_get_stats(gahhg_ctx* ctx, gahhg_stats* out);
+/* GPISP::exportStat2TXT (source/GPISPAHHG.cu:2115-2161): appends the
+ * report block; names are the polygon / query file names to print. */
+int gahhg_write_report(gahhg_ctx* ctx, const char* path, const char* poly_name,
+                       const char* query_name);
+/* GPISPTP::exportResult2CSV (source/GPISPTP.cu:81-99): "1" for IN else "0". */
+int gahhg_write_result_csv(const char* path, const int32_t* labels, int64_t n);
+
+/* Device memory helpers for callers without their own allocator. */
+int gahhg_device_alloc(gahhg_ctx* ctx, int64_t bytes, void** d_ptr);
+int gahhg_device_free(gahhg_ctx* ctx, void* d_ptr);
+int gahhg_memcpy_h2d(gahhg_ctx* ctx, void* d_dst, const void* h_src, int64_t bytes);
+int gahhg_memcpy_d2h(gahhg_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
+
+/* Grid export (for parity checks and serialisation): counts[5] =
+ * {depth, V, B, C, R}; then host arrays sized from them: v[3V],
+ * cells[7C] = (bstart,bnum,rstart,rnum,upcell,subcell,type),
+ * borders[6B] = (parent,partner,v0,v1,subcell,id), normals[3B], refs[R]
+ * (each cell's refs ascending), levels[7*depth] (vstart,vnum,bstart,bnum,
+ * cstart,cnum,rnum).  Any output pointer may be NULL. */
+int gahhg_grid_counts(gahhg_ctx* ctx, int64_t* counts5);
+int gahhg_export_grid(gahhg_ctx* ctx, double* v, int32_t* cells, int32_t* borders,
+                      double* normals, int32_t* refs, int32_t* levels);
+
+/* Known-answer hooks: run the device predicates on n inputs.
+ * op 0: normalize(a) -> out[3]; op 1: cross(a, b) -> out[3];
+ * op 2: isEEInter(a, b, c, d) -> out[0] in {0,1};
+ * op 3: isEdgeBorderInter(a, b, c, d) -> out[0] = class, out[1] = reserved. */
+int gahhg_kat_device(gahhg_ctx* ctx, int32_t op, const double* in12, int64_t n, double* out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

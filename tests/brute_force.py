@@ -2,7 +2,7 @@
 
 Independent of the oracle: the label of p is the crossing parity of the
 great-circle path r -> m -> p against every polygon edge, where r is a point
-known to be inside (just left of the midpoint of the first outer-ring edge,
+known to be inside (just left of the midpoint of a long outer-ring edge,
 the side the reference treats as inside after orientation normalisation) and
 m is the normalised midpoint of r and p.  Arc intersection uses the exact
 geometric test (both straddle conditions and the crossing point on both
@@ -52,16 +52,25 @@ def arc_distance(p, a, b):
     return np.where(inside, dplane, np.minimum(da, db)).min(axis=1)
 
 
-def labels(pv, pe, pts, chunk=1000, offset=1e-4):
+def labels(pv, pe, pts, chunk=1000):
     """Returns (labels 2=IN/1=OUT, distance to nearest edge) for pts[q,3]."""
     a = pv[pe[:, 0]]
     b = pv[pe[:, 1]]
-    a0, b0 = a[0], b[0]
-    n0 = _unit(np.cross(a0, b0))
-    r = _unit(_unit(a0 + b0) + offset * n0)
-    rd = arc_distance(r[None], a, b)[0]
-    if rd < offset * 0.25:
-        raise ValueError("reference point too close to an edge")
+    # reference point: just left of the midpoint of a long edge of ring 0
+    # (edge 0 starts ring 0; ring 0 ends where an edge returns to its first
+    # vertex), offset by a fraction of that edge's length
+    ring0_end = int(np.nonzero(pe[:, 1] == pe[0, 0])[0][0]) + 1
+    lens = np.linalg.norm(b[:ring0_end] - a[:ring0_end], axis=1)
+    r = None
+    for e in np.argsort(-lens)[:32]:
+        n0 = _unit(np.cross(a[e], b[e]))
+        off = 1e-3 * lens[e]
+        cand = _unit(_unit(a[e] + b[e]) + off * n0)
+        if arc_distance(cand[None], a, b)[0] > 0.25 * off:
+            r = cand
+            break
+    if r is None:
+        raise ValueError("no clear reference point")
     out = np.zeros(len(pts), dtype=np.int32)
     dist = np.zeros(len(pts))
     for s in range(0, len(pts), chunk):

@@ -1,0 +1,201 @@
+"""GPU parity: the gfx950 library against the CPU oracle, bit for bit.
+
+P0  device FP64 primitives and predicates equal the oracle's bitwise;
+P1  the hierarchy arrays built on the GPU equal the oracle's;
+P2  query labels equal the oracle's (0 mismatches).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import gahhg_amd
+from gahhg_amd import datasets
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+REF = os.path.join(os.path.dirname(__file__), "golden", "reference")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def refpath(n):
+    return os.path.join(REF, n)
+
+
+def _rand_unit(rng, n):
+    x = rng.standard_normal((n, 3))
+    return x / np.linalg.norm(x, axis=1)[:, None]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    g = gahhg_amd.GPISP(device=0)
+    yield g
+    g.close()
+
+
+def test_kat_bitwise(dev):
+    say("kat: 4 ops x 20000 cases")
+    rng = np.random.default_rng(42)
+    n = 20000
+    a, b, c, d = (_rand_unit(rng, n) for _ in range(4))
+    # near-degenerate cases: short arcs, points on the edge's great circle
+    b[:2000] = a[:2000] + 1e-9 * _rand_unit(rng, 2000)
+    m = a[2000:4000] + b[2000:4000]
+    d[2000:4000] = m / np.linalg.norm(m, axis=1)[:, None]
+    c[4000:6000] = a[4000:6000]
+    d[6000:8000] = b[6000:8000] * (1 + 1e-12)
+    x = np.concatenate([a, b, c, d], axis=1)
+    L = ol.lib()
+    out0 = dev.kat(0, x)
+    out1 = dev.kat(1, x)
+    out2 = dev.kat(2, x)
+    out3 = dev.kat(3, x)
+    bad = 0
+    r3 = np.zeros(3)
+    import ctypes
+    res = ctypes.c_int32()
+    for i in range(n):
+        L.og_kat_normalize(np.ascontiguousarray(a[i]), r3)
+        bad += not np.array_equal(r3, out0[i])
+        L.og_kat_cross(np.ascontiguousarray(a[i]), np.ascontiguousarray(b[i]), r3)
+        bad += not np.array_equal(r3, out1[i])
+        e = L.og_kat_is_ee_inter(*(np.ascontiguousarray(v[i]) for v in (a, b, c, d)))
+        bad += e != int(out2[i, 0])
+        k = L.og_kat_edge_border_inter(*(np.ascontiguousarray(v[i]) for v in (a, b, c, d)), ctypes.byref(res))
+        bad += (k != int(out3[i, 0])) or (res.value != int(out3[i, 1]))
+    assert bad == 0
+
+
+def compare_grids(gg, og, levels_o):
+    assert gg["depth"] == len(levels_o)
+    lv = [[L[k] for k in ("vstart", "vnum", "bstart", "bnum", "cstart", "cnum", "rnum")] for L in levels_o]
+    assert gg["levels"].tolist() == lv
+    for key in ("v", "cells", "borders", "normals", "refs"):
+        x, y = gg[key], og[key]
+        assert x.shape == y.shape, key
+        if key in ("v", "normals"):
+            diff = np.nonzero((x.view(np.int64) != y.view(np.int64)).any(axis=1))[0]
+        else:
+            diff = np.nonzero((x != y).reshape(len(x), -1).any(axis=1))[0]
+        assert len(diff) == 0, f"{key}: {len(diff)} rows differ, first {diff[:5]}: gpu {x[diff[:2]]} oracle {y[diff[:2]]}"
+
+
+POLYS = ["Square_72.txt", "Radiometer.txt", "NA0.txt", "Mars_001.txt", "Mars_002.txt", "Mars_100.txt",
+         "Mars_300.txt", "Mars_624.txt", "Tennessee_Tennessee_00005.txt", "Tennessee_Tennessee_02999.txt"]
+
+
+@pytest.mark.parametrize("name", POLYS)
+def test_grid_and_labels_reference_polygons(dev, name):
+    say("grid+labels", name)
+    dev.load_polygon_txt(refpath(name)).build()
+    o = ol.Oracle().load_txt(refpath(name)).build()
+    compare_grids(dev.grid(), o.grid(), o.levels())
+    st = dev.stats()
+    assert st["depth"] == o.counts()["depth"]
+    assert st["gpu_space_bytes"] == o.space_bytes()
+    assert abs(st["gamma"] - o.gamma()) == 0
+    qfile = {"Square_72.txt": "Square_72_query.csv", "Radiometer.txt": "Radiometer_query.csv"}.get(name, "Mars_query.csv")
+    lon, lat = ol.load_query_csv(refpath(qfile))
+    rlon, rlat = datasets.uniform_queries(np.random.default_rng(7), 200000)
+    xyz = ol.query_lonlat_to_xyz(np.concatenate([lon, rlon]), np.concatenate([lat, rlat]))
+    g_lab = dev.query_xyz(xyz)
+    o_lab = o.query_xyz(xyz)
+    assert np.count_nonzero(g_lab != o_lab) == 0
+    assert dev.stats()["inside"] == int((o_lab == 2).sum())
+
+
+def test_star_1k_million_queries(dev):
+    say("star_1k 1M")
+    w = datasets.star_1k()
+    dev.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    compare_grids(dev.grid(), o.grid(), o.levels())
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    g_lab = dev.query_xyz(xyz)
+    o_lab = o.query_xyz(xyz)
+    assert len(g_lab) == 1_000_000
+    assert np.count_nonzero(g_lab != o_lab) == 0
+    assert (g_lab == 0).sum() == 0
+
+
+def test_fractal_grid_parity(dev):
+    say("fractal 20k")
+    w = datasets.fractal_100k(n_queries=300_000, n_outer=20_000)
+    dev.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    compare_grids(dev.grid(), o.grid(), o.levels())
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    assert np.count_nonzero(dev.query_xyz(xyz) != o.query_xyz(xyz)) == 0
+
+
+@pytest.mark.parametrize("depth", [1, 3, 8])
+def test_fixed_depth_parity(dev, depth):
+    say("fixed depth", depth)
+    g = gahhg_amd.GPISP(device=0, max_depth=depth, auto_depth=False)
+    g.load_polygon_txt(refpath("Radiometer.txt")).build()
+    o = ol.Oracle().load_txt(refpath("Radiometer.txt")).build(max_depth=depth, auto_depth=False)
+    compare_grids(g.grid(), o.grid(), o.levels())
+    lon, lat = ol.load_query_csv(refpath("Radiometer_query.csv"))
+    xyz = ol.query_lonlat_to_xyz(lon, lat)
+    assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
+    g.close()
+
+
+def test_device_resident_query_and_edge_cases(dev):
+    say("device query")
+    dev.load_polygon_txt(refpath("NA0.txt")).build()
+    lon, lat = datasets.uniform_queries(np.random.default_rng(1), 100000)
+    xyz = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    host = dev.query_xyz(xyz)
+    dx = dev.alloc(xyz.nbytes)
+    dl = dev.alloc(4 * len(host))
+    dx.upload(xyz)
+    dev.query_device(dx, len(host), dl)
+    out = dl.download(np.zeros(len(host), dtype=np.int32))
+    assert (out == host).all()
+    # empty query
+    assert dev.query_xyz(np.zeros((0, 3))).size == 0
+    dx.free()
+    dl.free()
+
+
+def test_error_paths(tmp_path):
+    say("errors")
+    g = gahhg_amd.GPISP(device=0)
+    with pytest.raises(gahhg_amd.GahhgError) as e:
+        g.query_xyz(np.zeros((3, 3)))
+    assert e.value.code == -4
+    with pytest.raises(gahhg_amd.GahhgError) as e:
+        g.build()
+    assert e.value.code == -4
+    with pytest.raises(gahhg_amd.GahhgError) as e:
+        g.load_polygon_txt(tmp_path / "missing.txt")
+    assert e.value.code == -2
+    bad = tmp_path / "bad.txt"
+    bad.write_text("POLYGON 1 0\n0 3\n0 0\n1 1\n0 0\n")
+    with pytest.raises(gahhg_amd.GahhgError) as e:
+        g.load_polygon_txt(bad)
+    assert e.value.code == -3
+    g.close()
+
+
+def test_report_format(dev, tmp_path):
+    say("report")
+    dev.load_polygon_txt(refpath("Square_72.txt")).build()
+    lon, lat = ol.load_query_csv(refpath("Square_72_query.csv"))
+    dev.query_lonlat(lon, lat)
+    p = tmp_path / "report.txt"
+    dev.write_report(p, "Square_72.txt", "Square_72_query.csv")
+    txt = p.read_text()
+    assert "Polygon file: Square_72.txt" in txt
+    assert "Number of polygon vertice: 4" in txt
+    assert "Depth of GAHHG: 1" in txt
+    assert "Number of query points: 10000" in txt
+    st = dev.stats()
+    assert f"Test result: inside {st['inside']}, outside {10000 - st['inside']}" in txt
