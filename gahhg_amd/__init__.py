@@ -40,7 +40,7 @@ class GahhgError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
-        "device", "verbose", "query_layout")]
+        "device", "verbose", "query_layout", "query_bins")]
 
 
 _STAT_FIELDS = [
@@ -197,11 +197,11 @@ class GPISP:
     """One polygon's hierarchy on one GPU (the reference's GPISP object)."""
 
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
-                 query_layout=0):
+                 query_layout=0, query_bins=256):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
                              verbose=1 if verbose else 0, block_query=block_query,
-                             query_layout=query_layout)
+                             query_layout=query_layout, query_bins=query_bins)
         h = ctypes.c_void_p()
         rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
         if rc:

@@ -40,9 +40,8 @@ void free_grid(gahhg_ctx* c) {
   for (void* p : ptrs) dfree(p);
   g = DeviceGrid();
   QueryLayout& q = c->q;
-  dfree(q.nd);
+  dfree(q.table);
   dfree(q.qc);
-  dfree(q.qe);
   dfree(q.qside);
   q = QueryLayout();
   c->built = false;
@@ -96,6 +95,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->device = 0;
   p->verbose = 0;
   p->query_layout = 0;
+  p->query_bins = 256;
   return GAHHG_OK;
 }
 
@@ -109,7 +109,8 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
   else
     gahhg_params_default(&c->prm);
   if (c->prm.max_depth < 1 || c->prm.max_depth > kMaxLevels - 1 || c->prm.block_query < 64 ||
-      c->prm.block_query > 1024 || (c->prm.block_query % 64) != 0) {
+      c->prm.block_query > 1024 || (c->prm.block_query % 64) != 0 || c->prm.query_bins < 0 ||
+      c->prm.query_bins > 4096) {
     delete c;
     return GAHHG_E_ARG;
   }

@@ -87,22 +87,22 @@ struct GridView {
   int32_t* refs;
 };
 
-// Packed, read-only layout the query kernel walks (built after
-// classification).  One 32-byte node per cell:
-//   nd[0] = bstart, nd[1] = bnum | (rnum > 0) << 3 | type << 4,
-//   nd[2] = subcell (centre child) or -1, nd[3] = first ref slot,
-//   nd[4] = rnum, nd[5..7] = unused
-// Border normals are copied per cell in border order (qn, 3 doubles each)
-// with the child-cell ids (qsub).  Boundary leaves carry their centre (qc)
-// and an edge list (qe: a, b, n per ref; 9 doubles) plus, per ref, the
-// edge-plane side of the centre (qside: bit0 = ret2 < -eps, bit1 = ret2 >
-// eps), all computed with the same operations the reference uses inline.
+// Read-only query accelerators built after classification.
+//  table: one entry per cube-map bin (6 faces x bins x bins).  For every
+//    point inside the bin's bounding cap, the reference's first-match
+//    descent provably passes through the same cell; the entry holds that
+//    cell (>= 0), -1 when even the level-0 choice is not certain, or
+//    -(2 + label) when the cell is an empty leaf (label known outright).
+//  qc: centre of every bottom-level cell (mean of its border start
+//    vertices, normalised), the value kernel_PISP recomputes per query.
+//  qside: per ref slot, the side of the owning leaf's centre relative to
+//    the edge plane (bit0: ne.c < -eps, bit1: ne.c > eps).
 struct QueryLayout {
-  int64_t ncell = 0, nref = 0, nleaf = 0;
-  int32_t* nd = nullptr;     // 8 ints per cell
-  double* qc = nullptr;      // 4 doubles per cell (centre xyz, pad)
-  double* qe = nullptr;      // 9 doubles per ref slot
-  uint8_t* qside = nullptr;  // per ref slot
+  int32_t bins = 0;
+  int32_t* table = nullptr;
+  int32_t leaf_cstart = 0, leaf_cnum = 0;
+  double* qc = nullptr;
+  uint8_t* qside = nullptr;
   int64_t bytes = 0;
 };
 

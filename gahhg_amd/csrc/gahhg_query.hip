@@ -116,11 +116,267 @@ __global__ void k_label_counts(const int32_t* __restrict__ labels, int64_t n, un
   }
 }
 
+// ---------------------------------------------------------------------------
+// Start-cell table.  Bin (f, i, j) of a cube map: face f in {+x,-x,+y,-y,
+// +z,-z}, (u, v) in [-1,1]^2 split into bins x bins squares; the bin of p is
+// found from the dominant axis with one division (any rounding of that
+// choice is covered by the cap margin below).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V3 face_point(int f, double u, double v) {
+  switch (f) {
+    case 0: return v3(1.0, u, v);
+    case 1: return v3(-1.0, u, v);
+    case 2: return v3(u, 1.0, v);
+    case 3: return v3(u, -1.0, v);
+    case 4: return v3(u, v, 1.0);
+    default: return v3(u, v, -1.0);
+  }
+}
+
+__device__ __forceinline__ int bin_of(V3 p, int B) {
+  const double ax = fabs(p.x), ay = fabs(p.y), az = fabs(p.z);
+  int f;
+  double m, a, b;
+  if (ax >= ay && ax >= az) {
+    f = p.x >= 0.0 ? 0 : 1;
+    m = ax;
+    a = p.y;
+    b = p.z;
+  } else if (ay >= az) {
+    f = p.y >= 0.0 ? 2 : 3;
+    m = ay;
+    a = p.x;
+    b = p.z;
+  } else {
+    f = p.z >= 0.0 ? 4 : 5;
+    m = az;
+    a = p.x;
+    b = p.y;
+  }
+  if (!(m > 0.0)) return -1;
+  const double inv = 1.0 / m;
+  int i = (int)floor((a * inv + 1.0) * 0.5 * B);
+  int j = (int)floor((b * inv + 1.0) * 0.5 * B);
+  i = min(max(i, 0), B - 1);
+  j = min(max(j, 0), B - 1);
+  return (f * B + i) * B + j;
+}
+
+// Conservative state of the reference's inside test (n.p >= -eps for every
+// border, in border order) over a cap (centre u, angular radius rho):
+// 0 = fails for every point, 1 = passes for every point, 2 = undecided.
+// The bounds carry a 1e-10 margin, far above the rounding of n.p and of the
+// trigonometric bound itself.
+__device__ int cell_state_cap(const GridView& g, int cid, V3 u, double rho) {
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  bool all_in = true;
+  for (int i = 0; i < bn; i++) {
+    const V3 n = load3(g.b_n + 3 * (int64_t)(bs + i));
+    const double nn = sqrt(dot(n, n));
+    const double ct = fmin(1.0, fmax(-1.0, dot(n, u) / nn));
+    const double th = acos(ct);
+    const double lo = nn * cos(fmin(th + rho, M_PI)) - 1e-10;
+    const double hi = nn * cos(fmax(th - rho, 0.0)) + 1e-10;
+    if (hi < -kEps) return 0;  // this border rejects every point of the cap
+    if (!(lo >= -kEps)) all_in = false;
+  }
+  return all_in ? 1 : 2;
+}
+
+__global__ void k_build_table(GridView g, int B, int32_t* __restrict__ table) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6LL * B * B) return;
+  const int f = (int)(t / ((int64_t)B * B));
+  const int i = (int)((t / B) % B);
+  const int j = (int)(t % B);
+  const double u0 = -1.0 + 2.0 * i / B, u1 = -1.0 + 2.0 * (i + 1) / B;
+  const double w0 = -1.0 + 2.0 * j / B, w1 = -1.0 + 2.0 * (j + 1) / B;
+  const V3 cen = unit(face_point(f, 0.5 * (u0 + u1), 0.5 * (w0 + w1)));
+  double rho = 0.0;
+  const double us[2] = {u0, u1}, ws[2] = {w0, w1};
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      const V3 cr = unit(face_point(f, us[a], ws[b]));
+      rho = fmax(rho, acos(fmin(1.0, dot(cen, cr))));
+    }
+  rho += 1e-9;
+  // first-match over level 0
+  int cur = -1;
+  for (int c = 0; c < kGrid0Cells; c++) {
+    const int s = cell_state_cap(g, c, cen, rho);
+    if (s == 0) continue;
+    if (s == 1) cur = c;
+    break;  // certain first match, or undecided: stop either way
+  }
+  if (cur < 0) {
+    table[t] = -1;
+    return;
+  }
+  // descend while the first-match child is certain
+  for (;;) {
+    const int sc = g.c_subcell[cur];
+    if (sc == -1) break;
+    int next = -1;
+    bool stop = false;
+    int s = cell_state_cap(g, sc, cen, rho);
+    if (s == 1) {
+      next = sc;
+    } else if (s == 2) {
+      stop = true;
+    } else {
+      const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+      for (int k = 0; k < bn; k++) {
+        const int tc = g.b_subcell[bs + k];
+        if (tc == -1) continue;
+        s = cell_state_cap(g, tc, cen, rho);
+        if (s == 0) continue;
+        if (s == 1) next = tc;
+        else stop = true;
+        break;
+      }
+    }
+    if (stop || next < 0) break;
+    cur = next;
+  }
+  if (g.c_subcell[cur] == -1 && g.c_rnum[cur] == 0)
+    table[t] = -(2 + g.c_type[cur]);
+  else
+    table[t] = cur;
+}
+
+// Per bottom cell with refs: its centre, and per ref the side of that
+// centre relative to the edge plane.
+__global__ void k_leaf_prep(GridView g, const double* __restrict__ edge, int cstart, int cnum,
+                            double* __restrict__ qc, uint8_t* __restrict__ qside) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  const int rn = g.c_rnum[cid];
+  if (rn <= 0) return;
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  V3 c = v3(0.0, 0.0, 0.0);
+  for (int i = 0; i < bn; i++) c = add(c, load3(g.v + 3 * (int64_t)g.b_v0[bs + i]));
+  c = unit(divk(c, bn));
+  store3(qc + 3 * (int64_t)t, c);
+  const int rs = g.c_rstart[cid];
+  for (int i = 0; i < rn; i++) {
+    const double r2 = dot(load3(edge + 9 * (int64_t)g.refs[rs + i] + 6), c);
+    qside[rs + i] = (uint8_t)((lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
+  }
+}
+
+// Table-accelerated query; labels identical to k_query_soa.
+__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const double* __restrict__ edge,
+                                                   const int32_t* __restrict__ table, int B,
+                                                   const double* __restrict__ qc, const uint8_t* __restrict__ qside,
+                                                   int leaf_cstart, const double* __restrict__ xyz, int64_t n,
+                                                   int32_t* __restrict__ labels) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const V3 p = load3(xyz + 3 * q);
+  int cur = -1;
+  if (B > 0) {
+    const int bin = bin_of(p, B);
+    const int e = bin >= 0 ? table[bin] : -1;
+    if (e <= -2) {
+      labels[q] = -e - 2;
+      return;
+    }
+    cur = e;
+  }
+  if (cur < 0) {
+    for (int i = 0; i < kGrid0Cells; i++) {
+      cur = q_inside(g, p, i);
+      if (cur != -1) break;
+    }
+    if (cur == -1) {
+      labels[q] = kUndefined;
+      return;
+    }
+  }
+  int sc;
+  while ((sc = g.c_subcell[cur]) != -1) {
+    int r = q_inside(g, p, sc);
+    if (r != -1) {
+      cur = r;
+      continue;
+    }
+    const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+    bool found = false;
+    for (int i = 0; i < bn; i++) {
+      const int tc = g.b_subcell[bs + i];
+      if (tc == -1) continue;
+      r = q_inside(g, p, tc);
+      if (r != -1) {
+        cur = r;
+        found = true;
+        break;
+      }
+    }
+    if (!found) {
+      labels[q] = kUndefined;
+      return;
+    }
+  }
+  const int rn = g.c_rnum[cur];
+  const int type = g.c_type[cur];
+  if (rn == 0) {
+    labels[q] = type;
+    return;
+  }
+  const V3 c = load3(qc + 3 * (int64_t)(cur - leaf_cstart));
+  const V3 nq = arc_normal(c, p);
+  const int rs = g.c_rstart[cur];
+  int cnt = 0;
+  for (int i = 0; i < rn; i++) {
+    const double* er = edge + 9 * (int64_t)g.refs[rs + i];
+    if (same_side(dot(nq, load3(er)), dot(nq, load3(er + 3)))) continue;
+    const double r3 = dot(load3(er + 6), p);
+    const int sd = qside[rs + i];
+    if (((sd & 1) && lt0(r3)) || ((sd & 2) && gt0(r3))) continue;
+    cnt++;
+  }
+  labels[q] = (cnt & 1) ? 3 - type : type;
+}
+
+int pack_query_layout(gahhg_ctx* c) {
+  QueryLayout& q = c->q;
+  const LevelDesc LB = c->lv[c->gnum - 1];
+  q.leaf_cstart = LB.cstart;
+  q.leaf_cnum = LB.cnum;
+  q.bins = c->prm.query_bins;
+  const int64_t R = std::max<int64_t>(LB.rnum, 1);
+  if (dalloc(c, (void**)&q.qc, std::max<int64_t>(LB.cnum, 1) * 24) || dalloc(c, (void**)&q.qside, R))
+    return GAHHG_E_NOMEM;
+  q.bytes = std::max<int64_t>(LB.cnum, 1) * 24 + R;
+  GridView g = view(c);
+  hipLaunchKernelGGL(k_leaf_prep, dim3((unsigned)std::max<int64_t>((LB.cnum + 255) / 256, 1)), dim3(256), 0,
+                     c->stream, g, (const double*)c->d_edge, LB.cstart, LB.cnum, q.qc, q.qside);
+  int rc = hip_check(c, hipGetLastError(), "leaf prep");
+  if (rc) return rc;
+  if (q.bins > 0) {
+    const int64_t nb = 6LL * q.bins * q.bins;
+    if (dalloc(c, (void**)&q.table, nb * 4)) return GAHHG_E_NOMEM;
+    q.bytes += nb * 4;
+    hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g, q.bins,
+                       q.table);
+    rc = hip_check(c, hipGetLastError(), "table build");
+  }
+  return rc;
+}
+
 int run_query(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
   if (n <= 0) return GAHHG_OK;
   const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
-  hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                     (const double*)c->d_edge, d_xyz, n, d_labels);
+  if (c->prm.query_layout == 1) {
+    hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                       (const double*)c->d_edge, d_xyz, n, d_labels);
+  } else {
+    hipLaunchKernelGGL(k_query, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                       (const double*)c->d_edge, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
+                       (const double*)c->q.qc, (const uint8_t*)c->q.qside, c->q.leaf_cstart, d_xyz, n,
+                       d_labels);
+  }
   return hip_check(c, hipGetLastError(), "query kernel");
 }
 
@@ -164,11 +420,6 @@ int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out) 
   if (n <= 0) return GAHHG_OK;
   hipLaunchKernelGGL(k_kat, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, op, d_in, n, d_out);
   return hip_check(c, hipGetLastError(), "kat kernel");
-}
-
-int pack_query_layout(gahhg_ctx* c) {
-  (void)c;
-  return GAHHG_OK;
 }
 
 }  // namespace gahhg

@@ -48,7 +48,9 @@ typedef struct {
   int32_t block_query;     /* threads per block of the query kernel           */
   int32_t device;          /* HIP device ordinal                              */
   int32_t verbose;         /* 1: print per-level progress like the reference  */
-  int32_t query_layout;    /* 0: packed query layout (default), 1: build SoA  */
+  int32_t query_layout;    /* 0: table-accelerated query (default), 1: plain
+                              descent from level 0 over the build arrays     */
+  int32_t query_bins;      /* start-cell table bins per cube face (0 = none)  */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by

@@ -199,3 +199,20 @@ def test_report_format(dev, tmp_path):
     assert "Number of query points: 10000" in txt
     st = dev.stats()
     assert f"Test result: inside {st['inside']}, outside {10000 - st['inside']}" in txt
+
+
+@pytest.mark.parametrize("layout,bins", [(1, 0), (0, 0), (0, 16), (0, 1024)])
+def test_query_paths_agree(layout, bins):
+    say("query path", layout, bins)
+    w = datasets.fractal_100k(n_queries=400_000, n_outer=30_000)
+    g = gahhg_amd.GPISP(device=0, query_layout=layout, query_bins=bins)
+    g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    # exact cube-face and bin boundaries too
+    extra = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1],
+                      [1, 1, 0], [1, -1, 1], [-1, 1, 1], [0.5, 1, 0]], dtype=np.float64)
+    extra /= np.linalg.norm(extra, axis=1)[:, None]
+    xyz = np.concatenate([xyz, extra])
+    assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
+    g.close()
