@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--builds", type=int, default=5)
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--bins", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -63,7 +65,7 @@ def main():
     Q = xyz.shape[0]
     log(f"generated N={len(w['lon']) - (len(w['ring_off']) - 1)} edges, Q={Q} in {time.time() - t0:.1f}s")
 
-    g = gahhg_amd.GPISP(device=0)
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
     builds = []
     for i in range(args.builds + 1):
@@ -114,34 +116,57 @@ def main():
     value = Q / (ms_per_step * 1e-3)
     kms = float(np.mean(kernel_ms))
 
+    # the plain path: the reference's kernel structure (linear level-0 scan,
+    # descent, per-query leaf centre and per-ref edge planes) on the same grid
+    plain = None
+    if not args.no_plain:
+        gp = gahhg_amd.GPISP(device=0, query_layout=1, query_bins=0)
+        gp.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+        dl2 = gp.alloc(4 * Q)
+        dx2 = gp.alloc(xyz.nbytes)
+        dx2.upload(xyz)
+        gp.query_device(dx2, Q, dl2)
+        pl = []
+        for _ in range(3):
+            gp.query_device(dx2, Q, dl2)
+            pl.append(gp.stats()["query_ms"])
+        lab2 = dl2.download(np.empty(Q, dtype=np.int32))
+        plain = {"queries_per_s": round(Q / (float(np.median(pl)) * 1e-3), 1), "kernel_ms": round(float(np.median(pl)), 3),
+                 "labels_equal": bool((lab2 == labels).all())}
+        log(f"plain path: {plain}")
+        dx2.free()
+        dl2.free()
+        gp.close()
+
     fq = per_query_flops(qc, ns)
     achieved_tf = fq * Q / (kms * 1e-3) / 1e12
-    algo_bytes = 28.0 * Q
-    achieved_gbs = algo_bytes / (kms * 1e-3) / 1e9
+    achieved_gbs = 28.0 * Q / (kms * 1e-3) / 1e9
     frac_fp = achieved_tf / FP64_PEAK_TFLOPS
     frac_hbm = achieved_gbs / HBM_PEAK_GBS
     if frac_fp >= frac_hbm:
         roof = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(frac_fp, 4), "traffic": None,
-                "note": "FP64 vector ALU bound (no MFMA); algorithmic flops/query "
-                        f"{fq:.1f} from oracle counters; HBM {achieved_gbs:.1f} GB/s of 28 B/query"}
+                "note": "FP64 vector ALU (no MFMA is used); algorithmic FP64 ops/query "
+                        f"{fq:.1f} from exact oracle counters; compulsory HBM {achieved_gbs:.1f} GB/s at 28 B/query"}
     else:
         roof = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(frac_hbm, 4), "traffic": None}
     roof["kernel"] = "k_query"
     roof["kernel_ms"] = round(kms, 4)
-
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "fractal_1m: one polygon, 1M-vertex fractal ring (r=40 deg, centre lat 60 deg) "
                                "+ 3 holes x 50k vertices; 100M FP64 queries (70% uniform, 30% near-edge)",
-                   "n_edges": int(st["n_edges"]), "n_queries": int(Q), "path": "plain single-polygon GPU",
+                   "n_edges": int(st["n_edges"]), "n_queries": int(Q), "path": "single-polygon GPU, start-cell table + cooperative leaf tests",
                    "parallelism": "1 GPU"},
         "build_ms": round(pre_ms, 3), "create_ms": round(create_ms, 3), "classify_ms": round(classify_ms, 3),
         "depth": int(st["depth"]), "grid_bytes": int(st["gpu_space_bytes"]),
         "parity": {"sample": int(ns), "mismatches": mism},
+        "query_bins": args.bins,
+        "plain_path": plain,
+        "speedup_vs_plain": round(value / plain["queries_per_s"], 3) if plain else None,
         "roofline": roof,
     }
     if not args.no_cpu:

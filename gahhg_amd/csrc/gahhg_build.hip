@@ -528,22 +528,36 @@ __global__ void k_scatter_refs(const int32_t* rcell, const int32_t* redge, const
 }
 
 // kernel_EC2CS_setCEInfo (:1030-1043) plus the ascending order of each run.
-__global__ void k_finish_cells(GridView g, int cstart, int cnum, const int32_t* ccnt,
-                               const int32_t* coff) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnum) return;
-  const int n = ccnt[t], s = coff[t];
-  int32_t* r = g.refs + s;
-  for (int i = 1; i < n; i++) {  // runs are short; insertion sort
-    int v = r[i], j = i - 1;
-    while (j >= 0 && r[j] > v) {
-      r[j + 1] = r[j];
-      j--;
-    }
-    r[j + 1] = v;
+// One wave per cell: each lane ranks its elements against the whole run
+// (stable by position), staged in LDS when the run fits.
+constexpr int kSortCap = 1024;
+__global__ void __launch_bounds__(256) k_finish_cells(GridView g, int cstart, int cnum, const int32_t* ccnt,
+                                                      const int32_t* coff, const int32_t* __restrict__ src) {
+  __shared__ int32_t lds[4][kSortCap];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + w;
+  int n = 0, s = 0;
+  if (t < cnum) {
+    n = ccnt[t];
+    s = coff[t];
   }
-  g.c_rnum[cstart + t] = n;
-  g.c_rstart[cstart + t] = s;
+  const bool in_lds = n <= kSortCap;
+  if (in_lds)
+    for (int k = lane; k < n; k += 64) lds[w][k] = src[s + k];
+  __syncthreads();
+  for (int k = lane; k < n; k += 64) {
+    const int v = in_lds ? lds[w][k] : src[s + k];
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+      const int u = in_lds ? lds[w][j] : src[s + j];
+      rank += (u < v) || (u == v && j < k);
+    }
+    g.refs[s + rank] = v;
+  }
+  if (t < cnum && lane == 0) {
+    g.c_rnum[cstart + t] = n;
+    g.c_rstart[cstart + t] = s;
+  }
 }
 
 __global__ void k_fill(int32_t* p, int64_t n, int32_t v) {
@@ -899,10 +913,11 @@ int build_hierarchy(gahhg_ctx* c) {
   CK(copy_i32(c, c->d_f0, c->d_o1, LB.cnum));
   hipLaunchKernelGGL(k_scatter_refs, dim3(blocks_for(R, kBlock)), dim3(kBlock), 0, c->stream,
                      (const int32_t*)c->d_rcell, (const int32_t*)c->d_redge, (const DevState*)c->d_st,
-                     LB.cstart, c->d_f0, c->g.refs);
+                     LB.cstart, c->d_f0, c->d_rflag);
   LAUNCH_CK("scatter refs");
-  hipLaunchKernelGGL(k_finish_cells, dim3(blocks_for(LB.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
-                     LB.cstart, LB.cnum, (const int32_t*)c->d_f1, (const int32_t*)c->d_o1);
+  hipLaunchKernelGGL(k_finish_cells, dim3(blocks_for(LB.cnum, 4)), dim3(256), 0, c->stream, g,
+                     LB.cstart, LB.cnum, (const int32_t*)c->d_f1, (const int32_t*)c->d_o1,
+                     (const int32_t*)c->d_rflag);
   LAUNCH_CK("finish cells");
   CK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
   CK(sync_state(c));

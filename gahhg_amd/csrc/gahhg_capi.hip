@@ -42,7 +42,10 @@ void free_grid(gahhg_ctx* c) {
   QueryLayout& q = c->q;
   dfree(q.table);
   dfree(q.qc);
-  dfree(q.qside);
+  dfree(q.hdr);
+  dfree(q.sv);
+  dfree(q.sn);
+  dfree(q.sf);
   q = QueryLayout();
   c->built = false;
   c->gnum = 0;
@@ -238,15 +241,10 @@ int gahhg_query_xyz_device(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t
   c->st.n_queries = n;
   c->st.query_unit_s = n > 0 ? ms * 1e-3 / (double)n : 0.0;
   // label census (outside the timed region)
-  unsigned long long* d_two = nullptr;
-  RCK(dalloc(c, (void**)&d_two, 16));
-  (void)hipMemsetAsync(d_two, 0, 16, c->stream);
-  int rc = count_labels(c, d_labels, n, d_two);
+  RCK(count_labels(c, d_labels, n, &c->d_st->q_in));
   unsigned long long h_two[2] = {0, 0};
-  if (!rc) rc = hip_check(c, hipMemcpyAsync(h_two, d_two, 16, hipMemcpyDeviceToHost, c->stream), "census");
-  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "census sync");
-  dfree(d_two);
-  if (rc) return rc;
+  RCK(hip_check(c, hipMemcpyAsync(h_two, &c->d_st->q_in, 16, hipMemcpyDeviceToHost, c->stream), "census"));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "census sync"));
   c->st.inside = (int64_t)h_two[0];
   c->st.unlocated = (int64_t)h_two[1];
   c->st.outside = n - c->st.inside;

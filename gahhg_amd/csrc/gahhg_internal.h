@@ -37,9 +37,11 @@ struct DevState {
   int32_t c0;       // border children
   int32_t R;        // bottom-level refs
   int32_t changed;  // classification round changed something
-  int32_t pad[2];
+  int32_t nrec;     // records of the query edge stream
+  int32_t pad;
   unsigned long long steps_mark, steps_dispatch, partner_missing, cap_hits, unlocated, bad_index;
   unsigned long long zero_ref_edges, undefined_leaves;
+  unsigned long long q_in, q_undef;  // label census of the last query
   LevelDesc lv[kMaxLevels];
 };
 
@@ -95,14 +97,23 @@ struct GridView {
 //    -(2 + label) when the cell is an empty leaf (label known outright).
 //  qc: centre of every bottom-level cell (mean of its border start
 //    vertices, normalised), the value kernel_PISP recomputes per query.
-//  qside: per ref slot, the side of the owning leaf's centre relative to
-//    the edge plane (bit0: ne.c < -eps, bit1: ne.c > eps).
+//  Edge stream of every bottom cell with refs: its refs (ascending edge id)
+//    as polylines.  Consecutive refs e, e' with end(e) == start(e') share a
+//    record, so a chain of k edges is k + 1 records.  Record r holds a
+//    vertex (sv, 3 doubles); when bit 2 of sf[r] is set, the edge from
+//    record r to record r+1 exists, sn[r] is its plane normal and bits 0/1
+//    give the side of the leaf centre (ne.c < -eps / ne.c > eps).  hdr holds
+//    (first record, record count) per bottom cell.
 struct QueryLayout {
   int32_t bins = 0;
   int32_t* table = nullptr;
   int32_t leaf_cstart = 0, leaf_cnum = 0;
   double* qc = nullptr;
-  uint8_t* qside = nullptr;
+  int32_t* hdr = nullptr;
+  double* sv = nullptr;
+  double* sn = nullptr;
+  uint8_t* sf = nullptr;
+  int64_t nrec = 0;
   int64_t bytes = 0;
 };
 

@@ -98,12 +98,14 @@ __global__ void __launch_bounds__(kQBlock) k_query_soa(GridView g, const double*
   labels[q] = (cnt & 1) ? 3 - type : type;
 }
 
-__global__ void k_label_counts(const int32_t* __restrict__ labels, int64_t n, unsigned long long* out2) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_label_counts(const int32_t* __restrict__ labels, int64_t n,
+                                                      unsigned long long* out2) {
+  __shared__ unsigned long long part[2][4];
   unsigned long long in = 0, un = 0;
-  if (i < n) {
-    in = labels[i] == kIn;
-    un = labels[i] == kUndefined;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int l = labels[i];
+    in += l == kIn;
+    un += l == kUndefined;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -111,8 +113,15 @@ __global__ void k_label_counts(const int32_t* __restrict__ labels, int64_t n, un
     un += __shfl_xor(un, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (in) atomicAdd(&out2[0], in);
-    if (un) atomicAdd(&out2[1], un);
+    part[0][threadIdx.x >> 6] = in;
+    part[1][threadIdx.x >> 6] = un;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long a = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    const unsigned long long b = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    if (a) atomicAdd(&out2[0], a);
+    if (b) atomicAdd(&out2[1], b);
   }
 }
 
@@ -244,99 +253,171 @@ __global__ void k_build_table(GridView g, int B, int32_t* __restrict__ table) {
     table[t] = cur;
 }
 
-// Per bottom cell with refs: its centre, and per ref the side of that
-// centre relative to the edge plane.
-__global__ void k_leaf_prep(GridView g, const double* __restrict__ edge, int cstart, int cnum,
-                            double* __restrict__ qc, uint8_t* __restrict__ qside) {
+__device__ __forceinline__ bool chained(const int32_t* pe, int e, int e2) { return pe[2 * e + 1] == pe[2 * e2]; }
+
+// Records of a bottom cell's edge stream: its refs plus one closing record
+// per chain.
+__global__ void k_leaf_count(GridView g, const int32_t* __restrict__ pe, int cstart, int cnum,
+                             int32_t* __restrict__ cnt) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
   const int cid = cstart + t;
   const int rn = g.c_rnum[cid];
-  if (rn <= 0) return;
+  int chains = 0;
+  if (rn > 0) {
+    const int32_t* r = g.refs + g.c_rstart[cid];
+    chains = 1;
+    for (int i = 0; i + 1 < rn; i++) chains += chained(pe, r[i], r[i + 1]) ? 0 : 1;
+  }
+  cnt[t] = rn > 0 ? rn + chains : 0;
+}
+
+// Centre and edge stream of every bottom cell with refs.
+__global__ void k_leaf_write(GridView g, const double* __restrict__ pv, const int32_t* __restrict__ pe,
+                             const double* __restrict__ edge, int cstart, int cnum, const int32_t* __restrict__ roff,
+                             double* __restrict__ qc, int32_t* __restrict__ hdr, double* __restrict__ sv,
+                             double* __restrict__ sn, uint8_t* __restrict__ sf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnum) return;
+  const int cid = cstart + t;
+  const int rn = g.c_rnum[cid];
+  int k = roff[t];
+  hdr[2 * t] = k;
+  if (rn <= 0) {
+    hdr[2 * t + 1] = 0;
+    return;
+  }
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
   V3 c = v3(0.0, 0.0, 0.0);
   for (int i = 0; i < bn; i++) c = add(c, load3(g.v + 3 * (int64_t)g.b_v0[bs + i]));
   c = unit(divk(c, bn));
   store3(qc + 3 * (int64_t)t, c);
-  const int rs = g.c_rstart[cid];
+  const int32_t* r = g.refs + g.c_rstart[cid];
+  const int k0 = k;
   for (int i = 0; i < rn; i++) {
-    const double r2 = dot(load3(edge + 9 * (int64_t)g.refs[rs + i] + 6), c);
-    qside[rs + i] = (uint8_t)((lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
+    const int e = r[i];
+    const V3 ne = load3(edge + 9 * (int64_t)e + 6);
+    const double r2 = dot(ne, c);
+    store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e]));
+    store3(sn + 3 * (int64_t)k, ne);
+    sf[k] = (uint8_t)(4 | (lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
+    k++;
+    if (i + 1 == rn || !chained(pe, e, r[i + 1])) {
+      store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e + 1]));
+      store3(sn + 3 * (int64_t)k, v3(0.0, 0.0, 0.0));
+      sf[k] = 0;
+      k++;
+    }
   }
+  hdr[2 * t + 1] = k - k0;
 }
 
 // Table-accelerated query; labels identical to k_query_soa.
-__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const double* __restrict__ edge,
-                                                   const int32_t* __restrict__ table, int B,
-                                                   const double* __restrict__ qc, const uint8_t* __restrict__ qside,
-                                                   int leaf_cstart, const double* __restrict__ xyz, int64_t n,
+// Each lane locates its own point (table start cell, then the reference's
+// first-match descent).  Points that end in a leaf with refs are then
+// resolved cooperatively, one at a time per wave: the 64 lanes split that
+// leaf's edge stream (coalesced loads), each lane tests one edge, and the
+// crossing count is the popcount of a ballot.
+__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __restrict__ table, int B,
+                                                   const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                                   const double* __restrict__ sv, const double* __restrict__ sn,
+                                                   const uint8_t* __restrict__ sf, int leaf_cstart,
+                                                   const double* __restrict__ xyz, int64_t n,
                                                    int32_t* __restrict__ labels) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const V3 p = load3(xyz + 3 * q);
-  int cur = -1;
-  if (B > 0) {
-    const int bin = bin_of(p, B);
-    const int e = bin >= 0 ? table[bin] : -1;
-    if (e <= -2) {
-      labels[q] = -e - 2;
-      return;
+  const int lane = threadIdx.x & 63;
+  V3 p = v3(0.0, 0.0, 0.0);
+  int leaf = -1;  // bottom-cell index still to resolve, or -1
+  int type = kUndefined;
+  if (q < n) {
+    p = load3(xyz + 3 * q);
+    int cur = -1;
+    bool done = false;
+    if (B > 0) {
+      const int bin = bin_of(p, B);
+      const int e = bin >= 0 ? table[bin] : -1;
+      if (e <= -2) {
+        labels[q] = -e - 2;
+        done = true;
+      }
+      cur = e;
     }
-    cur = e;
-  }
-  if (cur < 0) {
-    for (int i = 0; i < kGrid0Cells; i++) {
-      cur = q_inside(g, p, i);
-      if (cur != -1) break;
-    }
-    if (cur == -1) {
-      labels[q] = kUndefined;
-      return;
-    }
-  }
-  int sc;
-  while ((sc = g.c_subcell[cur]) != -1) {
-    int r = q_inside(g, p, sc);
-    if (r != -1) {
-      cur = r;
-      continue;
-    }
-    const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
-    bool found = false;
-    for (int i = 0; i < bn; i++) {
-      const int tc = g.b_subcell[bs + i];
-      if (tc == -1) continue;
-      r = q_inside(g, p, tc);
-      if (r != -1) {
-        cur = r;
-        found = true;
-        break;
+    if (!done && cur < 0) {
+      for (int i = 0; i < kGrid0Cells; i++) {
+        cur = q_inside(g, p, i);
+        if (cur != -1) break;
+      }
+      if (cur == -1) {
+        labels[q] = kUndefined;
+        done = true;
       }
     }
-    if (!found) {
-      labels[q] = kUndefined;
-      return;
+    if (!done) {
+      int sc;
+      while ((sc = g.c_subcell[cur]) != -1) {
+        int r = q_inside(g, p, sc);
+        if (r != -1) {
+          cur = r;
+          continue;
+        }
+        const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+        bool found = false;
+        for (int i = 0; i < bn; i++) {
+          const int tc = g.b_subcell[bs + i];
+          if (tc == -1) continue;
+          r = q_inside(g, p, tc);
+          if (r != -1) {
+            cur = r;
+            found = true;
+            break;
+          }
+        }
+        if (!found) {
+          cur = -1;
+          break;
+        }
+      }
+      if (cur == -1) {
+        labels[q] = kUndefined;
+      } else {
+        type = g.c_type[cur];
+        if (g.c_rnum[cur] == 0)
+          labels[q] = type;
+        else
+          leaf = cur - leaf_cstart;
+      }
     }
   }
-  const int rn = g.c_rnum[cur];
-  const int type = g.c_type[cur];
-  if (rn == 0) {
-    labels[q] = type;
-    return;
+  // cooperative edge tests for this wave's boundary points
+  unsigned long long pending = __ballot(leaf >= 0);
+  while (pending) {
+    const int src = __ffsll((long long)pending) - 1;
+    pending &= pending - 1;
+    const int lt = __shfl(leaf, src, 64);
+    const V3 pp = v3(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64));
+    const V3 c = load3(qc + 3 * (int64_t)lt);
+    const V3 nq = arc_normal(c, pp);
+    const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
+    int cnt = 0;
+    for (int base = 0; base < m - 1; base += 64) {
+      const int k = base + lane;
+      bool cross = false;
+      if (k < m - 1) {
+        const int rk = r0 + k;
+        const int fl = sf[rk];
+        if (fl & 4) {
+          const double ra = dot(nq, load3(sv + 3 * (int64_t)rk));
+          const double rb = dot(nq, load3(sv + 3 * (int64_t)(rk + 1)));
+          if (!same_side(ra, rb)) {
+            const double r3 = dot(load3(sn + 3 * (int64_t)rk), pp);
+            cross = !(((fl & 1) && lt0(r3)) || ((fl & 2) && gt0(r3)));
+          }
+        }
+      }
+      cnt += __popcll(__ballot(cross));
+    }
+    if (lane == src) labels[q] = (cnt & 1) ? 3 - type : type;
   }
-  const V3 c = load3(qc + 3 * (int64_t)(cur - leaf_cstart));
-  const V3 nq = arc_normal(c, p);
-  const int rs = g.c_rstart[cur];
-  int cnt = 0;
-  for (int i = 0; i < rn; i++) {
-    const double* er = edge + 9 * (int64_t)g.refs[rs + i];
-    if (same_side(dot(nq, load3(er)), dot(nq, load3(er + 3)))) continue;
-    const double r3 = dot(load3(er + 6), p);
-    const int sd = qside[rs + i];
-    if (((sd & 1) && lt0(r3)) || ((sd & 2) && gt0(r3))) continue;
-    cnt++;
-  }
-  labels[q] = (cnt & 1) ? 3 - type : type;
 }
 
 int pack_query_layout(gahhg_ctx* c) {
@@ -345,14 +426,29 @@ int pack_query_layout(gahhg_ctx* c) {
   q.leaf_cstart = LB.cstart;
   q.leaf_cnum = LB.cnum;
   q.bins = c->prm.query_bins;
-  const int64_t R = std::max<int64_t>(LB.rnum, 1);
-  if (dalloc(c, (void**)&q.qc, std::max<int64_t>(LB.cnum, 1) * 24) || dalloc(c, (void**)&q.qside, R))
-    return GAHHG_E_NOMEM;
-  q.bytes = std::max<int64_t>(LB.cnum, 1) * 24 + R;
+  const int64_t nc = std::max<int64_t>(LB.cnum, 1);
   GridView g = view(c);
-  hipLaunchKernelGGL(k_leaf_prep, dim3((unsigned)std::max<int64_t>((LB.cnum + 255) / 256, 1)), dim3(256), 0,
-                     c->stream, g, (const double*)c->d_edge, LB.cstart, LB.cnum, q.qc, q.qside);
-  int rc = hip_check(c, hipGetLastError(), "leaf prep");
+  const unsigned cb = (unsigned)((nc + 255) / 256);
+  // record counts -> offsets (flag scratch holds >= cnum entries)
+  hipLaunchKernelGGL(k_leaf_count, dim3(cb), dim3(256), 0, c->stream, g, (const int32_t*)c->d_pe, LB.cstart,
+                     LB.cnum, c->d_f0);
+  int rc = hip_check(c, hipGetLastError(), "leaf count");
+  if (rc) return rc;
+  rc = scan_i32(c, c->d_f0, c->d_o0, LB.cnum, nullptr, &c->d_st->nrec, 0);
+  if (rc) return rc;
+  rc = hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream), "readback");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  q.nrec = std::max<int32_t>(c->h_st->nrec, 1);
+  if (dalloc(c, (void**)&q.qc, nc * 24) || dalloc(c, (void**)&q.hdr, nc * 8) ||
+      dalloc(c, (void**)&q.sv, q.nrec * 24) || dalloc(c, (void**)&q.sn, q.nrec * 24) ||
+      dalloc(c, (void**)&q.sf, q.nrec))
+    return GAHHG_E_NOMEM;
+  q.bytes = nc * 32 + q.nrec * 49;
+  hipLaunchKernelGGL(k_leaf_write, dim3(cb), dim3(256), 0, c->stream, g, (const double*)c->d_pv,
+                     (const int32_t*)c->d_pe, (const double*)c->d_edge, LB.cstart, LB.cnum,
+                     (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf);
+  rc = hip_check(c, hipGetLastError(), "leaf write");
   if (rc) return rc;
   if (q.bins > 0) {
     const int64_t nb = 6LL * q.bins * q.bins;
@@ -373,17 +469,23 @@ int run_query(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
                        (const double*)c->d_edge, d_xyz, n, d_labels);
   } else {
     hipLaunchKernelGGL(k_query, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                       (const double*)c->d_edge, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
-                       (const double*)c->q.qc, (const uint8_t*)c->q.qside, c->q.leaf_cstart, d_xyz, n,
-                       d_labels);
+                       (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
+                       (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
+                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, c->q.leaf_cstart, d_xyz, n, d_labels);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
 }
 
+__global__ void k_clear_census(DevState* st) {
+  st->q_in = 0;
+  st->q_undef = 0;
+}
+
 int count_labels(gahhg_ctx* c, const int32_t* d_labels, int64_t n, unsigned long long* d_out2) {
-  if (n <= 0) return GAHHG_OK;
-  hipLaunchKernelGGL(k_label_counts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_labels, n,
-                     d_out2);
+  hipLaunchKernelGGL(k_clear_census, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  if (n <= 0) return hip_check(c, hipGetLastError(), "census");
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_label_counts, dim3((unsigned)blocks), dim3(256), 0, c->stream, d_labels, n, d_out2);
   return hip_check(c, hipGetLastError(), "label counts");
 }
 
