@@ -216,3 +216,26 @@ def test_query_paths_agree(layout, bins):
     xyz = np.concatenate([xyz, extra])
     assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
     g.close()
+
+
+def test_cli_matches_oracle(tmp_path):
+    say("cli")
+    import subprocess
+    exe = os.path.join(os.path.dirname(gahhg_amd.LIB_PATH), "gahhg")
+    rep, res = tmp_path / "report.txt", tmp_path / "result.csv"
+    r = subprocess.run([exe, "/polyfile", refpath("Radiometer.txt"), "/tpfile", refpath("Radiometer_query.csv"),
+                        "/reportfile", str(rep), "/resultfile", str(res), "/unknownflag", "x"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.array([int(x) for x in res.read_text().split()])
+    o = ol.Oracle().load_txt(refpath("Radiometer.txt")).build()
+    lon, lat = ol.load_query_csv(refpath("Radiometer_query.csv"))
+    exp = (o.query_xyz(ol.query_lonlat_to_xyz(lon, lat)) == 2).astype(int)
+    assert (got == exp).all()
+    assert "Depth of GAHHG: 4" in rep.read_text()
+    # AHHG-style fixed depth
+    r = subprocess.run([exe, "/polyfile", refpath("Radiometer.txt"), "/tpfile", refpath("Radiometer_query.csv"),
+                        "/reportfile", str(rep), "/resultfile", str(res), "/maxdepth", "6"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    assert "Depth of GAHHG: 6" in rep.read_text().split("-----\n")[-2] or "Depth of GAHHG: 6" in rep.read_text()
