@@ -70,7 +70,9 @@ EXPORTED = [
     "gahhg_query_xyz", "gahhg_query_xyz_device", "gahhg_read_query_csv", "gahhg_query_lonlat_to_xyz",
     "gahhg_get_stats", "gahhg_write_report", "gahhg_write_result_csv", "gahhg_device_alloc",
     "gahhg_device_free", "gahhg_memcpy_h2d", "gahhg_memcpy_d2h", "gahhg_grid_counts",
-    "gahhg_export_grid", "gahhg_kat_device",
+    "gahhg_export_grid", "gahhg_kat_device", "gahhg_set_polygons_lonlat", "gahhg_num_polygons",
+    "gahhg_query_batch_xyz", "gahhg_query_batch_xyz_device", "gahhg_get_polygon_stats",
+    "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid",
 ]
 
 _lib = None
@@ -117,6 +119,13 @@ def lib():
         "gahhg_grid_counts": ([P, lp], ctypes.c_int),
         "gahhg_export_grid": ([P, P, P, P, P, P, P], ctypes.c_int),
         "gahhg_kat_device": ([P, i32, dp, i64, dp], ctypes.c_int),
+        "gahhg_set_polygons_lonlat": ([P, dp, dp, lp, ip, i32], ctypes.c_int),
+        "gahhg_num_polygons": ([P, ctypes.POINTER(i32)], ctypes.c_int),
+        "gahhg_query_batch_xyz": ([P, dp, ip, i64, ip], ctypes.c_int),
+        "gahhg_query_batch_xyz_device": ([P, P, P, i64, P], ctypes.c_int),
+        "gahhg_get_polygon_stats": ([P, i32, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gahhg_polygon_grid_counts": ([P, i32, lp], ctypes.c_int),
+        "gahhg_export_polygon_grid": ([P, i32, P, P, P, P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -274,6 +283,60 @@ class GPISP:
     def write_report(self, path, poly_name="", query_name=""):
         self._check(self.L.gahhg_write_report(self.h, str(path).encode(), poly_name.encode(),
                                               query_name.encode()))
+
+    # -- batches -----------------------------------------------------------
+    def set_polygons_lonlat(self, polys):
+        """polys: list of (lon, lat, ring_off) as for set_polygon_lonlat."""
+        lon, lat, roff, proff = [], [], [0], [0]
+        base = 0
+        for plon, plat, pro in polys:
+            pro = np.asarray(pro, dtype=np.int64)
+            lon.append(np.asarray(plon, dtype=np.float64))
+            lat.append(np.asarray(plat, dtype=np.float64))
+            roff.extend((pro[1:] + base).tolist())
+            base += int(pro[-1])
+            proff.append(len(roff) - 1)
+        self._check(self.L.gahhg_set_polygons_lonlat(
+            self.h, np.ascontiguousarray(np.concatenate(lon)), np.ascontiguousarray(np.concatenate(lat)),
+            np.asarray(roff, dtype=np.int64), np.asarray(proff, dtype=np.int32), len(polys)))
+        return self
+
+    @property
+    def n_polygons(self):
+        n = ctypes.c_int32()
+        self._check(self.L.gahhg_num_polygons(self.h, ctypes.byref(n)))
+        return n.value
+
+    def query_batch_xyz(self, xyz, poly):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        poly = np.ascontiguousarray(poly, dtype=np.int32)
+        labels = np.zeros(poly.size, dtype=np.int32)
+        self._check(self.L.gahhg_query_batch_xyz(self.h, xyz, poly, poly.size, labels))
+        return labels
+
+    def query_batch_device(self, d_xyz, d_poly, n, d_labels):
+        self._check(self.L.gahhg_query_batch_xyz_device(self.h, d_xyz.ptr, d_poly.ptr, int(n), d_labels.ptr))
+
+    def polygon_stats(self, p):
+        s = Stats()
+        self._check(self.L.gahhg_get_polygon_stats(self.h, p, ctypes.byref(s)))
+        return dict((f, getattr(s, f)) for f, _ in _STAT_FIELDS)
+
+    def polygon_grid(self, p):
+        cnt = np.zeros(5, dtype=np.int64)
+        self._check(self.L.gahhg_polygon_grid_counts(self.h, p, cnt))
+        d, V, B, C, R = (int(x) for x in cnt)
+        v = np.zeros(V * 3)
+        cells = np.zeros(C * 7, dtype=np.int32)
+        borders = np.zeros(B * 6, dtype=np.int32)
+        normals = np.zeros(B * 3)
+        refs = np.zeros(max(R, 1), dtype=np.int32)
+        levels = np.zeros(7 * d, dtype=np.int32)
+        self._check(self.L.gahhg_export_polygon_grid(self.h, p, v.ctypes.data, cells.ctypes.data,
+                                                     borders.ctypes.data, normals.ctypes.data, refs.ctypes.data,
+                                                     levels.ctypes.data))
+        return dict(depth=d, v=v.reshape(-1, 3), cells=cells.reshape(-1, 7), borders=borders.reshape(-1, 6),
+                    normals=normals.reshape(-1, 3), refs=refs[:R], levels=levels.reshape(-1, 7))
 
     def grid(self):
         cnt = np.zeros(5, dtype=np.int64)

@@ -98,18 +98,39 @@ struct WalkDiag {
 
 enum { kMark = 0, kCount = 1, kWrite = 2 };
 
+// Maps a border index derived by markNECell's OVERLAP branch (computed in
+// the polygon's local numbering, :1279-1285) back to a global index, or -1
+// when the local index is outside the borders that exist at this level.
+__device__ __forceinline__ int overlap_in_index(const GridView& g, const PolyDev& P, int nbs_global, int kk,
+                                                WalkDiag& d) {
+  const int local = nbs_global - P.bs + P.lbs + kk;
+  if (local < 0 || local >= P.lbs + P.bn) {
+    d.bad_index++;
+    return -1;
+  }
+  int glob;
+  if (local >= P.lbs)
+    glob = P.bs + (local - P.lbs);
+  else if (local >= P.plbs)
+    glob = P.pbs + (local - P.plbs);
+  else {
+    d.bad_index++;
+    return -1;
+  }
+  return g.b_partner[glob];
+}
+
 // One polygon edge across the cells of one level: markNECell (:1153-1289,
 // MODE kMark), calcRefNum (:685-818, kCount), writeRef (:820-955, kWrite).
 template <int MODE>
 __device__ void walk_edge(const GridView& g, const double* __restrict__ edge, int e, int level,
-                          int ncell0, int32_t* buf, int32_t* out_cells, int nb_valid,
-                          WalkDiag& d) {
+                          const PolyDev& P, int32_t* buf, int32_t* out_cells, WalkDiag& d) {
   const double* er = edge + 9 * (int64_t)e;
   const V3 p0 = load3(er), p1 = load3(er + 3), ne = load3(er + 6);
   int cur = -1;
   if (level == 0) {
-    for (int i = 0; i < ncell0; i++) {
-      cur = inside_cell(g, p0, i);
+    for (int i = 0; i < kGrid0Cells; i++) {
+      cur = inside_cell(g, p0, P.cell0 + i);
       if (cur != -1) break;
     }
     if (cur == -1) {
@@ -206,17 +227,12 @@ __device__ void walk_edge(const GridView& g, const double* __restrict__ edge, in
       }
       const int nc = g.b_parent[pt];
       if (MODE == kMark) {
-        // markNECell derives `in` from the new cell's bstart (:1279-1285);
-        // the resulting index is kept, and treated as -1 outside the array.
+        // markNECell derives `in` from the new cell's bstart, after curcell
+        // has changed (:1279-1285); the index is kept as the reference
+        // computes it and treated as -1 outside the array.
         const int nbs = g.c_bstart[nc];
         const int kk = t > 0 ? (out - nbs + 1) % u : (out - nbs + u - 1) % u;
-        const int idx = nbs + kk;
-        if (idx < 0 || idx >= nb_valid) {
-          d.bad_index++;
-          in = -1;
-        } else {
-          in = g.b_partner[idx];
-        }
+        in = overlap_in_index(g, P, nbs, kk, d);
       } else {
         in = pt;
       }
@@ -232,20 +248,30 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
+// MODE kMark walks the edges of refining polygons at `level`; kCount and
+// kWrite walk every edge at its polygon's bottom level, starting from the
+// level above it (buf_prev) when refinement stopped by the density rule.
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_walk(GridView g, const double* __restrict__ edge, int N,
-                                                 int level, int ncell0, int32_t* buf,
-                                                 int32_t* buf_prev, const int32_t* off,
-                                                 int32_t* cnt_out, int32_t* rcell, DevState* st) {
+__global__ void __launch_bounds__(kBlock) k_walk(GridView g, const double* __restrict__ edge,
+                                                 const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd,
+                                                 int Ntot, int level, int32_t* buf, int32_t* buf_prev,
+                                                 const int32_t* off, int32_t* cnt_out, int32_t* rcell,
+                                                 DevState* st) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  // borders that exist while this level is walked (marking OVERLAP guard)
-  const int nb_valid = st->lv[level].bstart + st->lv[level].bnum;
   WalkDiag d = {0, 0, 0, 0, 0};
-  if (e < N) {
-    if (MODE == kMark) buf_prev[e] = buf[e];
-    int32_t* oc = (MODE == kWrite) ? rcell + off[e] : nullptr;
-    walk_edge<MODE>(g, edge, e, level, ncell0, buf, oc, nb_valid, d);
-    if (MODE == kCount) cnt_out[e] = (int32_t)d.steps;
+  if (e < Ntot) {
+    const PolyDev P = pd[epoly[e]];
+    if (MODE == kMark) {
+      if (P.refining) {
+        buf_prev[e] = buf[e];
+        walk_edge<MODE>(g, edge, e, level, P, buf, nullptr, d);
+      }
+    } else {
+      int32_t* b = P.use_prev ? buf_prev : buf;
+      int32_t* oc = (MODE == kWrite) ? rcell + off[e] : nullptr;
+      walk_edge<MODE>(g, edge, e, P.L, P, b, oc, d);
+      if (MODE == kCount) cnt_out[e] = (int32_t)d.steps;
+    }
   }
   unsigned long long s = wave_sum_u64(d.steps);
   unsigned long long pm = wave_sum_u64(d.partner_missing);
@@ -277,14 +303,14 @@ __global__ void k_edges(const double* __restrict__ pv, const int32_t* __restrict
   store3(r + 6, arc_normal(a, b));
 }
 
-// kernel_DivGridFast_zeroRefCount, :1127-1150
-__global__ void k_zero(GridView g, const DevState* st, int lv) {
-  const LevelDesc L = st->lv[lv];
+// kernel_DivGridFast_zeroRefCount, :1127-1150, over one level's cells of
+// every polygon (level 0 also resets the fields the reference resets there).
+__global__ void k_zero(GridView g, int lo, int n, int level0) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.cnum) return;
-  const int cid = L.cstart + t;
+  if (t >= n) return;
+  const int cid = lo + t;
   g.c_rnum[cid] = 0;
-  if (L.cstart == 0) {
+  if (level0) {
     g.c_rstart[cid] = -1;
     g.c_subcell[cid] = -1;
     g.c_upcell[cid] = -1;
@@ -294,28 +320,69 @@ __global__ void k_zero(GridView g, const DevState* st, int lv) {
   }
 }
 
-// kernel_DivGrid_calcCenterOffsets flags, :381-404
-__global__ void k_refine_flags(GridView g, const DevState* st, int lv, int32_t* f0, int32_t* f1) {
-  const LevelDesc L = st->lv[lv];
+// Kind of the border child produced for border i of refined cell P
+// (kernel_DivGrid_calcBorderSCOffsets, :473-509): 6 = hexagon shared with a
+// refined neighbour (created by the lower id), 4 = quad against an
+// unrefined or missing neighbour, 0 = none.  Refined means marked (rnum != 0)
+// at this level, which is exactly the cells createCenter gives a subcell.
+__device__ __forceinline__ int child_kind(const GridView& g, int P, int bnum, int i) {
+  if (bnum == 4 && i == 0) return 0;
+  const int pt = g.b_partner[g.c_bstart[P] + i];
+  if (pt == -1) return 4;
+  const int Q = g.b_parent[pt];
+  const bool qref = g.c_rnum[Q] != 0;
+  if (!qref) return 4;
+  return P < Q ? 6 : 0;
+}
+
+// Plan of one level (kernel_DivGrid_calcCenterOffsets :381-404 and the
+// border-child counts): per cell, new corners (= bnum when refined), the
+// centre child, border children and their borders.
+__global__ void k_plan(GridView g, int lo, int n, int32_t* fa, int32_t* fb, int32_t* fc, int32_t* fd) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.cnum) return;
-  const int cid = L.cstart + t;
+  if (t >= n) return;
+  const int cid = lo + t;
   const bool r = g.c_rnum[cid] != 0;
-  f0[t] = r ? g.c_bnum[cid] : 0;
-  f1[t] = r ? 1 : 0;
+  const int bn = g.c_bnum[cid];
+  int nc = 0, nb = 0;
+  if (r)
+    for (int i = 0; i < bn; i++) {
+      const int k = child_kind(g, cid, bn, i);
+      nc += k ? 1 : 0;
+      nb += k;
+    }
+  fa[t] = r ? bn : 0;
+  fb[t] = r ? 1 : 0;
+  fc[t] = nc;
+  fd[t] = nb;
+}
+
+// Per-polygon totals of the plan from the exclusive scans.
+__global__ void k_plan_totals(PolyDev* pd, int npoly, int lo, const int32_t* fa, const int32_t* fb,
+                              const int32_t* fc, const int32_t* fd, const int32_t* oa, const int32_t* ob,
+                              const int32_t* oc, const int32_t* od) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoly) return;
+  PolyDev& P = pd[p];
+  if (!P.refining || P.cn <= 0) return;
+  const int f = P.cs - lo, l = f + P.cn - 1;
+  P.nv = oa[l] + fa[l] - oa[f];
+  P.U = ob[l] + fb[l] - ob[f];
+  P.nbc = oc[l] + fc[l] - oc[f];
+  P.nbb = od[l] + fd[l] - od[f];
 }
 
 // kernel_DivGrid_createCenter, :406-470
-__global__ void k_create_center(GridView g, const DevState* st, int lv, const int32_t* o0,
-                                const int32_t* o1) {
-  const LevelDesc L = st->lv[lv];
+__global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oa,
+                                const int32_t* ob) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.cnum) return;
-  const int pid = L.cstart + t;
+  if (t >= n) return;
+  const int pid = lo + t;
   if (g.c_rnum[pid] == 0) return;
-  const int vbase = L.vstart + L.vnum;
-  const int bbase = L.bstart + L.bnum;
-  const int cbase = L.cstart + L.cnum;
+  const int p = g.c_poly[pid];
+  const PolyDev& P = pd[p];
+  if (!P.refining) return;
+  const int f = P.cs - lo;
   const int bs = g.c_bstart[pid], bn = g.c_bnum[pid];
   V3 center;
   if (bn == 4) {
@@ -326,9 +393,9 @@ __global__ void k_create_center(GridView g, const DevState* st, int lv, const in
     center = divk(center, bn);
   }
   center = unit(center);
-  const int vo = vbase + o0[t];
-  const int bo = bbase + o0[t];
-  const int child = cbase + o1[t];
+  const int vo = P.nvb + (oa[t] - oa[f]);
+  const int bo = P.nbb0 + (oa[t] - oa[f]);
+  const int child = P.ncb + (ob[t] - ob[f]);
   double* vw = const_cast<double*>(g.v);
   for (int i = 0; i < bn; i++) store3(vw + 3 * (int64_t)(vo + i), unit(mul(add(vtx(g, g.b_v0[bs + i]), center), 0.5)));
   for (int i = 0; i < bn; i++) {
@@ -348,83 +415,73 @@ __global__ void k_create_center(GridView g, const DevState* st, int lv, const in
   g.c_upcell[child] = pid;
   g.c_subcell[child] = -1;
   g.c_type[child] = kUndefined;
+  g.c_poly[child] = p;
   g.c_subcell[pid] = child;
 }
 
-// Which border child (if any) centre-child border b produces:
-// 6 = hexagon shared with the refined neighbour (created by the lower id),
-// 4 = quad against an unrefined or missing neighbour, 0 = none.
-// kernel_DivGrid_calcBorderSCOffsets, :473-509
-__device__ __forceinline__ int border_child_kind(const GridView& g, int b, int* upedge_out) {
-  const int P = g.c_upcell[g.b_parent[b]];
-  const int id = g.b_id[b];
-  const int ue = g.c_bstart[P] + id;
-  *upedge_out = ue;
-  if (g.c_bnum[P] == 4 && id == 0) return 0;
-  const int pt = g.b_partner[ue];
-  if (pt != -1 && g.c_subcell[g.b_parent[pt]] != -1 && P < g.b_parent[pt]) return 6;
-  if (pt == -1 || g.c_subcell[g.b_parent[pt]] == -1) return 4;
-  return 0;
-}
-
-// t indexes the centre-child borders of level lv+1: [bbase, bbase + nv)
-__global__ void k_bsc_flags(GridView g, int bbase, int nv, int32_t* f0, int32_t* f1) {
+// kernel_DivGrid_createBorderSC, :512-607: the border children of one
+// refined cell, in border order.
+__global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oc,
+                                  const int32_t* od) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nv) return;
-  int ue;
-  const int k = border_child_kind(g, bbase + t, &ue);
-  f0[t] = k;
-  f1[t] = k ? 1 : 0;
-}
-
-// kernel_DivGrid_createBorderSC, :512-607
-__global__ void k_bsc_create(GridView g, int bbase, int nv, int cbase, int U, const int32_t* o0,
-                             const int32_t* o1) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nv) return;
-  const int b = bbase + t;
-  int ue;
-  const int kind = border_child_kind(g, b, &ue);
-  if (!kind) return;
-  const int eo = bbase + nv + o0[t];
-  const int co = cbase + U + o1[t];
-  int vv[6];
-  if (kind == 6) {
-    const int pt = g.b_partner[ue];
-    const int psub = g.c_subcell[g.b_parent[pt]];
-    const int pse = g.c_bstart[psub] + g.b_id[pt];
-    vv[0] = g.b_v0[ue];
-    vv[1] = g.b_v1[pse];
-    vv[2] = g.b_v0[pse];
-    vv[3] = g.b_v1[ue];
-    vv[4] = g.b_v1[b];
-    vv[5] = g.b_v0[b];
-  } else {
-    vv[0] = g.b_v0[ue];
-    vv[1] = g.b_v1[ue];
-    vv[2] = g.b_v1[b];
-    vv[3] = g.b_v0[b];
+  if (t >= n) return;
+  const int P = lo + t;
+  if (g.c_rnum[P] == 0) return;
+  const int p = g.c_poly[P];
+  const PolyDev& D = pd[p];
+  if (!D.refining) return;
+  const int f = D.cs - lo;
+  int co = D.ncb + D.U + (oc[t] - oc[f]);
+  int eo = D.nbb0 + D.nv + (od[t] - od[f]);
+  const int bnP = g.c_bnum[P];
+  const int pbs = g.c_bstart[P];
+  const int cbs = g.c_bstart[g.c_subcell[P]];  // centre child's borders
+  for (int i = 0; i < bnP; i++) {
+    const int kind = child_kind(g, P, bnP, i);
+    if (!kind) continue;
+    const int ue = pbs + i;
+    const int b = cbs + i;
+    int vv[6];
+    if (kind == 6) {
+      const int pt = g.b_partner[ue];
+      const int psub = g.c_subcell[g.b_parent[pt]];
+      const int pse = g.c_bstart[psub] + g.b_id[pt];
+      vv[0] = g.b_v0[ue];
+      vv[1] = g.b_v1[pse];
+      vv[2] = g.b_v0[pse];
+      vv[3] = g.b_v1[ue];
+      vv[4] = g.b_v1[b];
+      vv[5] = g.b_v0[b];
+    } else {
+      vv[0] = g.b_v0[ue];
+      vv[1] = g.b_v1[ue];
+      vv[2] = g.b_v1[b];
+      vv[3] = g.b_v0[b];
+    }
+    for (int k = 0; k < kind; k++) {
+      const int nb = eo + k;
+      const int a = vv[k], z = vv[(k + 1) % kind];
+      g.b_id[nb] = k;
+      g.b_v0[nb] = a;
+      g.b_v1[nb] = z;
+      g.b_parent[nb] = co;
+      g.b_partner[nb] = -1;
+      g.b_subcell[nb] = -1;
+      store3(g.b_n + 3 * (int64_t)nb, arc_normal(vtx(g, a), vtx(g, z)));
+    }
+    g.c_bnum[co] = kind;
+    g.c_bstart[co] = eo;
+    g.c_rstart[co] = -1;
+    g.c_rnum[co] = -1;
+    g.c_subcell[co] = -1;
+    g.c_upcell[co] = P;
+    g.c_type[co] = kUndefined;
+    g.c_poly[co] = p;
+    g.b_subcell[ue] = co;
+    if (kind == 6) g.b_subcell[g.b_partner[ue]] = co;
+    co += 1;
+    eo += kind;
   }
-  for (int i = 0; i < kind; i++) {
-    const int nb = eo + i;
-    const int a = vv[i], z = vv[(i + 1) % kind];
-    g.b_id[nb] = i;
-    g.b_v0[nb] = a;
-    g.b_v1[nb] = z;
-    g.b_parent[nb] = co;
-    g.b_partner[nb] = -1;
-    g.b_subcell[nb] = -1;
-    store3(g.b_n + 3 * (int64_t)nb, arc_normal(vtx(g, a), vtx(g, z)));
-  }
-  g.c_bnum[co] = kind;
-  g.c_bstart[co] = eo;
-  g.c_rstart[co] = -1;
-  g.c_rnum[co] = -1;
-  g.c_subcell[co] = -1;
-  g.c_upcell[co] = g.b_parent[ue];
-  g.c_type[co] = kUndefined;
-  g.b_subcell[ue] = co;
-  if (kind == 6) g.b_subcell[g.b_partner[ue]] = co;
 }
 
 // device_setBorderPartner, :42-58: link `edge` with the border of `cell`
@@ -449,11 +506,14 @@ __device__ __forceinline__ void link_or_clear(const GridView& g, int te, int edg
     g.b_partner[edge] = -1;
 }
 
-// kernel_DivGrid_setSCPartner, :610-682 (border children [start, start+c0))
-__global__ void k_sc_partner(GridView g, int start, const DevState* st) {
+// kernel_DivGrid_setSCPartner, :610-682, over the new level's cells (the
+// centre children are skipped).
+__global__ void k_sc_partner(GridView g, const PolyDev* __restrict__ pd, int lo, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= st->c0) return;
-  const int X = start + t;
+  if (t >= n) return;
+  const int X = lo + t;
+  const PolyDev& D = pd[g.c_poly[X]];
+  if (X - D.ncb < D.U) return;
   const int Pu = g.c_upcell[X];
   const int ub = g.c_bnum[Pu], ubs = g.c_bstart[Pu];
   int upedge = -1;
@@ -489,57 +549,73 @@ __global__ void k_sc_partner(GridView g, int start, const DevState* st) {
   }
 }
 
-// Level ranges of level lv+1 once its border children exist.
-__global__ void k_level_next(DevState* st, int lv, int vbase, int nv, int bbase, int cbase, int U) {
-  LevelDesc n;
-  n.vstart = vbase;
-  n.vnum = nv;
-  n.bstart = bbase;
-  n.bnum = nv + st->e0;
-  n.cstart = cbase;
-  n.cnum = U + st->c0;
-  n.rnum = 0;
-  st->lv[lv + 1] = n;
-  st->lv[lv].rnum = U;
-}
-
 // --- dispatch (:1700-1784) ---
-// head flags at the first ref of every edge except edge 0 (:971-981); an
-// edge without refs shares its slot with the next edge.
-__global__ void k_edge_heads(const int32_t* off, int N, const DevState* st, int32_t* flag) {
+// Refs of each polygon: [roff, roff + R) in the edge-major ref array.
+__global__ void k_poly_refs(PolyDev* pd, int npoly, const int32_t* off, const int32_t* cnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoly) return;
+  PolyDev& P = pd[p];
+  const int last = P.e0 + P.N - 1;
+  P.roff = off[P.e0];
+  P.R = off[last] + cnt[last] - P.roff;
+}
+
+// head flags at the first ref of every edge but each polygon's first
+// (:971-981); an edge without refs shares its slot with the next edge.
+__global__ void k_edge_heads(const int32_t* off, const int32_t* epoly, const PolyDev* __restrict__ pd, int Ntot,
+                             int32_t* flag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < 1 || e >= N) return;
+  if (e >= Ntot) return;
+  const PolyDev& P = pd[epoly[e]];
+  if (e == P.e0) return;
   const int o = off[e];
-  if (o < st->R) flag[o] = 1;
+  if (o < P.roff + P.R) flag[o] = 1;
 }
 
-__global__ void k_count_cells(const int32_t* rcell, const DevState* st, int cstart, int32_t* ccnt) {
+// ref -> global edge id from the inclusive scan of head flags
+// (kernel_EC2CS_adjustEFlag, :984-994), counted from the polygon's first ref.
+__global__ void k_ref_edges(const int32_t* rcell, const int32_t* incl, const int32_t* cpoly,
+                            const PolyDev* __restrict__ pd, const DevState* st, int32_t* redge) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st->R) return;
-  atomicAdd(&ccnt[rcell[i] - cstart], 1);
+  const PolyDev& P = pd[cpoly[rcell[i]]];
+  const int base = P.roff > 0 ? incl[P.roff - 1] : 0;
+  redge[i] = P.e0 + (incl[i] - base);
 }
 
-__global__ void k_scatter_refs(const int32_t* rcell, const int32_t* redge, const DevState* st,
-                               int cstart, int32_t* fill, int32_t* refs) {
+__global__ void k_count_cells(const int32_t* rcell, const DevState* st, int32_t* ccnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st->R) return;
-  const int pos = atomicAdd(&fill[rcell[i] - cstart], 1);
+  atomicAdd(&ccnt[rcell[i]], 1);
+}
+
+__global__ void k_scatter_refs(const int32_t* rcell, const int32_t* redge, const DevState* st, int32_t* fill,
+                               int32_t* refs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st->R) return;
+  const int pos = atomicAdd(&fill[rcell[i]], 1);
   refs[pos] = redge[i];
 }
 
-// kernel_EC2CS_setCEInfo (:1030-1043) plus the ascending order of each run.
-// One wave per cell: each lane ranks its elements against the whole run
-// (stable by position), staged in LDS when the run fits.
+// kernel_EC2CS_setCEInfo (:1030-1043) for bottom cells, plus the ascending
+// order of each run.  One wave per cell: each lane ranks its elements
+// against the whole run (stable by position), staged in LDS when it fits.
 constexpr int kSortCap = 1024;
-__global__ void __launch_bounds__(256) k_finish_cells(GridView g, int cstart, int cnum, const int32_t* ccnt,
-                                                      const int32_t* coff, const int32_t* __restrict__ src) {
+__global__ void __launch_bounds__(256) k_finish_cells(GridView g, const PolyDev* __restrict__ pd, int ncell,
+                                                      const int32_t* ccnt, const int32_t* coff,
+                                                      const int32_t* __restrict__ src) {
   __shared__ int32_t lds[4][kSortCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + w;
   int n = 0, s = 0;
-  if (t < cnum) {
-    n = ccnt[t];
-    s = coff[t];
+  bool bottom = false;
+  if (t < ncell) {
+    const PolyDev& P = pd[g.c_poly[t]];
+    bottom = t >= P.bcs && t < P.bcs + P.bcn;
+    if (bottom) {
+      n = ccnt[t];
+      s = coff[t];
+    }
   }
   const bool in_lds = n <= kSortCap;
   if (in_lds)
@@ -554,9 +630,9 @@ __global__ void __launch_bounds__(256) k_finish_cells(GridView g, int cstart, in
     }
     g.refs[s + rank] = v;
   }
-  if (t < cnum && lane == 0) {
-    g.c_rnum[cstart + t] = n;
-    g.c_rstart[cstart + t] = s;
+  if (bottom && lane == 0) {
+    g.c_rnum[t] = n;
+    g.c_rstart[t] = s;
   }
 }
 
@@ -632,6 +708,7 @@ GridView view(gahhg_ctx* c) {
   v.c_upcell = c->g.c_upcell;
   v.c_subcell = c->g.c_subcell;
   v.c_type = c->g.c_type;
+  v.c_poly = c->g.c_poly;
   v.refs = c->g.refs;
   return v;
 }
@@ -696,7 +773,7 @@ static int ensure_c(gahhg_ctx* c, int64_t used, int64_t need) {
   if (grow(c, &g.c_bstart, used, need, nc) || grow(c, &g.c_bnum, used, need, nc) ||
       grow(c, &g.c_rstart, used, need, nc) || grow(c, &g.c_rnum, used, need, nc) ||
       grow(c, &g.c_upcell, used, need, nc) || grow(c, &g.c_subcell, used, need, nc) ||
-      grow(c, &g.c_type, used, need, nc))
+      grow(c, &g.c_type, used, need, nc) || grow(c, &g.c_poly, used, need, nc))
     return GAHHG_E_NOMEM;
   g.ccap = nc;
   return GAHHG_OK;
@@ -706,14 +783,13 @@ static int ensure_flags(gahhg_ctx* c, int64_t need) {
   if (need <= c->fcap) return GAHHG_OK;
   CK_SYNC(c);
   int64_t nc = next_cap(c->fcap, need);
-  dfree(c->d_f0);
-  dfree(c->d_f1);
-  dfree(c->d_o0);
-  dfree(c->d_o1);
-  c->d_f0 = c->d_f1 = c->d_o0 = c->d_o1 = nullptr;
-  if (dalloc(c, (void**)&c->d_f0, nc * 4) || dalloc(c, (void**)&c->d_f1, nc * 4) ||
-      dalloc(c, (void**)&c->d_o0, nc * 4) || dalloc(c, (void**)&c->d_o1, nc * 4))
-    return GAHHG_E_NOMEM;
+  int32_t** bufs[8] = {&c->d_f0, &c->d_f1, &c->d_o0, &c->d_o1, &c->d_f2, &c->d_f3, &c->d_o2, &c->d_o3};
+  for (int i = 0; i < 8; i++) {
+    dfree(*bufs[i]);
+    *bufs[i] = nullptr;
+  }
+  for (int i = 0; i < 8; i++)
+    if (dalloc(c, (void**)bufs[i], nc * 4)) return GAHHG_E_NOMEM;
   c->fcap = nc;
   return GAHHG_OK;
 }
@@ -741,187 +817,289 @@ static int sync_state(gahhg_ctx* c) {
   return hip_check(c, s, "state readback");
 }
 
-#define CK(x)                   \
-  do {                          \
-    int _rc = (x);              \
+static int pd_download(gahhg_ctx* c) {
+  hipError_t s = hipMemcpyAsync(c->pd.data(), c->d_pd, c->pd.size() * sizeof(PolyDev), hipMemcpyDeviceToHost,
+                                c->stream);
+  if (s == hipSuccess) s = hipStreamSynchronize(c->stream);
+  return hip_check(c, s, "polygon state readback");
+}
+
+static int pd_upload(gahhg_ctx* c) {
+  return hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                     c->stream),
+                   "polygon state upload");
+}
+
+#define CK(x)                        \
+  do {                               \
+    int _rc = (x);                   \
     if (_rc != GAHHG_OK) return _rc; \
   } while (0)
 #define LAUNCH_CK(name) CK(hip_check(c, hipGetLastError(), name))
 
-// GPISP::createAHHGInit + createAHHG (:1947-2015) + dispatchSELast.
+// GPISP::createAHHGInit + createAHHG (:1947-2015) + dispatchSELast
+// (:1700-1784) for every polygon of the context at once.  Level l of every
+// polygon still refining is processed by the same launches; one host
+// synchronisation per level applies the density rule per polygon and sizes
+// the next level.
 int build_hierarchy(gahhg_ctx* c) {
-  const int N = c->poly.N;
+  const int np = (int)c->polys.size();
+  const int64_t Nt = c->n_edges;
   const int maxd = c->prm.max_depth;
   const bool autod = c->prm.auto_depth != 0;
 
-  // level 0 grid
+  // per-edge records (p0, p1, plane normal) for every walk and leaf test
+  hipLaunchKernelGGL(k_edges, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const double*)c->d_pv,
+                     (const int32_t*)c->d_pe, (int)Nt, c->d_edge);
+  LAUNCH_CK("edge records");
+
+  // level 0: one copy of the 42-cell grid per polygon
   std::vector<double> v0(3 * kGrid0Verts), bn0(3 * kGrid0Borders);
   std::vector<int32_t> bi0(6 * kGrid0Borders), ci0(7 * kGrid0Cells);
   build_grid0(v0.data(), bi0.data(), bn0.data(), ci0.data());
-
-  // capacities (createAHHGInit :1949-1953)
-  CK(ensure_v(c, 0, std::max(N, 160)));
-  CK(ensure_b(c, 0, std::max<int64_t>(4LL * N, 480)));
-  CK(ensure_c(c, 0, std::max(N, 84)));
-  CK(ensure_flags(c, std::max<int64_t>(4LL * N, 480)));
-
-  {  // upload grid-0 (copyGrid0HostToDevice, :1532-1555) as SoA
-    std::vector<int32_t> col(kGrid0Borders);
-    auto up_b = [&](int32_t* dst, int k) {
-      for (int i = 0; i < kGrid0Borders; i++) col[i] = bi0[6 * i + k];
-      return hipMemcpy(dst, col.data(), kGrid0Borders * 4, hipMemcpyHostToDevice);
-    };
-    std::vector<int32_t> colc(kGrid0Cells);
-    auto up_c = [&](int32_t* dst, int k) {
-      for (int i = 0; i < kGrid0Cells; i++) colc[i] = ci0[7 * i + k];
-      return hipMemcpy(dst, colc.data(), kGrid0Cells * 4, hipMemcpyHostToDevice);
-    };
-    hipError_t s = hipMemcpy(c->g.v, v0.data(), v0.size() * 8, hipMemcpyHostToDevice);
-    if (s == hipSuccess) s = hipMemcpy(c->g.b_n, bn0.data(), bn0.size() * 8, hipMemcpyHostToDevice);
+  const int64_t V0 = (int64_t)kGrid0Verts * np, B0 = (int64_t)kGrid0Borders * np, C0 = (int64_t)kGrid0Cells * np;
+  // capacities (createAHHGInit :1949-1953, summed over the batch)
+  CK(ensure_v(c, 0, std::max<int64_t>(Nt, 160) + V0));
+  CK(ensure_b(c, 0, std::max<int64_t>(4 * Nt, 480) + B0));
+  CK(ensure_c(c, 0, std::max<int64_t>(Nt, 84) + C0));
+  CK(ensure_flags(c, std::max<int64_t>(4 * Nt, 480) + C0));
+  {
+    std::vector<double> hv(3 * V0), hn(3 * B0);
+    std::vector<int32_t> cols[7], bcols[6];
+    for (auto& x : cols) x.resize(C0);
+    for (auto& x : bcols) x.resize(B0);
+    std::vector<int32_t> cpoly(C0);
+    for (int p = 0; p < np; p++) {
+      const int vb = kGrid0Verts * p, bb = kGrid0Borders * p, cb = kGrid0Cells * p;
+      std::copy(v0.begin(), v0.end(), hv.begin() + 3 * (int64_t)vb);
+      std::copy(bn0.begin(), bn0.end(), hn.begin() + 3 * (int64_t)bb);
+      for (int i = 0; i < kGrid0Borders; i++) {
+        bcols[0][bb + i] = bi0[6 * i] + cb;                                   // parent
+        bcols[1][bb + i] = bi0[6 * i + 1] < 0 ? -1 : bi0[6 * i + 1] + bb;     // partner
+        bcols[2][bb + i] = bi0[6 * i + 2] + vb;                               // v0
+        bcols[3][bb + i] = bi0[6 * i + 3] + vb;                               // v1
+        bcols[4][bb + i] = bi0[6 * i + 4];                                    // subcell (-1)
+        bcols[5][bb + i] = bi0[6 * i + 5];                                    // id
+      }
+      for (int i = 0; i < kGrid0Cells; i++) {
+        cols[0][cb + i] = ci0[7 * i] + bb;
+        for (int k = 1; k < 7; k++) cols[k][cb + i] = ci0[7 * i + k];
+        cpoly[cb + i] = p;
+      }
+    }
+    hipError_t s = hipMemcpy(c->g.v, hv.data(), hv.size() * 8, hipMemcpyHostToDevice);
+    if (s == hipSuccess) s = hipMemcpy(c->g.b_n, hn.data(), hn.size() * 8, hipMemcpyHostToDevice);
     int32_t* bd[6] = {c->g.b_parent, c->g.b_partner, c->g.b_v0, c->g.b_v1, c->g.b_subcell, c->g.b_id};
-    for (int k = 0; k < 6 && s == hipSuccess; k++) s = up_b(bd[k], k);
+    for (int k = 0; k < 6 && s == hipSuccess; k++) s = hipMemcpy(bd[k], bcols[k].data(), B0 * 4, hipMemcpyHostToDevice);
     int32_t* cd[7] = {c->g.c_bstart, c->g.c_bnum, c->g.c_rstart, c->g.c_rnum, c->g.c_upcell, c->g.c_subcell, c->g.c_type};
-    for (int k = 0; k < 7 && s == hipSuccess; k++) s = up_c(cd[k], k);
+    for (int k = 0; k < 7 && s == hipSuccess; k++) s = hipMemcpy(cd[k], cols[k].data(), C0 * 4, hipMemcpyHostToDevice);
+    if (s == hipSuccess) s = hipMemcpy(c->g.c_poly, cpoly.data(), C0 * 4, hipMemcpyHostToDevice);
     CK(hip_check(c, s, "grid0 upload"));
   }
 
-  // per-edge records (p0, p1, plane normal) for every walk and leaf test
-  hipLaunchKernelGGL(k_edges, dim3(blocks_for(N, kBlock)), dim3(kBlock), 0, c->stream,
-                     (const double*)c->d_pv, (const int32_t*)c->d_pe, N, c->d_edge);
-  LAUNCH_CK("edge records");
-
-  memset(c->lv, 0, sizeof(c->lv));
-  c->lv[0] = LevelDesc{0, kGrid0Verts, 0, kGrid0Borders, 0, kGrid0Cells, 0};
-  memset(c->h_st, 0, sizeof(DevState));
-  c->h_st->lv[0] = c->lv[0];
-  CK(hip_check(c, hipMemcpy(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice), "state init"));
-  CK(fill_i32(c, c->d_loc_cur, N, 0));
-  CK(fill_i32(c, c->d_loc_prev, N, 0));
-
-  CK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
-
-  c->gnum = maxd;
-  c->gamma = 0.f;
-  bool early = false;
-  GridView g = view(c);
-  for (int lv = 0; lv < c->gnum - 1; lv++) {
-    if (c->prm.verbose) printf("Subdivide level %d on GPU.\n", lv);
-    // upper bound of this level's cell count (exact for level 0)
-    int64_t cbound = c->lv[lv].cnum;
-    // cell flags of this level, centre-border flags (<= 6 per cell) and the
-    // next level's cell flags (<= 7 per cell) share these buffers
-    CK(ensure_flags(c, 7 * cbound + 1));
-    hipLaunchKernelGGL(k_zero, dim3(blocks_for(cbound, kBlock)), dim3(kBlock), 0, c->stream, g, c->d_st, lv);
-    LAUNCH_CK("zero");
-    hipLaunchKernelGGL(k_walk<kMark>, dim3(blocks_for(N, kBlock)), dim3(kBlock), 0, c->stream, g,
-                       (const double*)c->d_edge, N, lv, kGrid0Cells, c->d_loc_cur, c->d_loc_prev,
-                       (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, c->d_st);
-    LAUNCH_CK("mark walk");
-    hipLaunchKernelGGL(k_refine_flags, dim3(blocks_for(cbound, kBlock)), dim3(kBlock), 0, c->stream,
-                       g, (const DevState*)c->d_st, lv, c->d_f0, c->d_f1);
-    LAUNCH_CK("refine flags");
-    const int32_t* dn = &c->d_st->lv[lv].cnum;
-    CK(scan_i32(c, c->d_f0, c->d_o0, cbound, dn, &c->d_st->nv, 0));
-    CK(scan_i32(c, c->d_f1, c->d_o1, cbound, dn, &c->d_st->U, 0));
-    CK(sync_state(c));
-    c->lv[lv] = c->h_st->lv[lv];
-    const int U = c->h_st->U;
-    const int64_t nv = c->h_st->nv;
-    if (autod) {
-      // auto-depth rule, :1597-1604
-      float k = U * 1.0 / N;
-      c->gamma = k;
-      if (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01)) {
-        early = true;
-        c->gnum = lv + 1;
-        break;
-      }
-    }
-    const LevelDesc L = c->lv[lv];
-    const int64_t vbase = (int64_t)L.vstart + L.vnum;
-    const int64_t bbase = (int64_t)L.bstart + L.bnum;
-    const int64_t cbase = (int64_t)L.cstart + L.cnum;
-    if (vbase + nv > INT32_MAX || bbase + 7 * nv > INT32_MAX || cbase + U + nv > INT32_MAX)
-      return set_err(c, GAHHG_E_LIMIT, "build: level %d exceeds int32 indexing", lv + 1);
-    CK(ensure_v(c, vbase, vbase + nv));
-    CK(ensure_b(c, bbase, bbase + 7 * nv));
-    CK(ensure_c(c, cbase, cbase + U + nv));
-    g = view(c);
-    hipLaunchKernelGGL(k_create_center, dim3(blocks_for(L.cnum, kBlock)), dim3(kBlock), 0, c->stream,
-                       g, (const DevState*)c->d_st, lv, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
-    LAUNCH_CK("create center");
-    hipLaunchKernelGGL(k_bsc_flags, dim3(blocks_for(nv, kBlock)), dim3(kBlock), 0, c->stream, g,
-                       (int)bbase, (int)nv, c->d_f0, c->d_f1);
-    LAUNCH_CK("border child flags");
-    CK(scan_i32(c, c->d_f0, c->d_o0, nv, nullptr, &c->d_st->e0, 0));
-    CK(scan_i32(c, c->d_f1, c->d_o1, nv, nullptr, &c->d_st->c0, 0));
-    hipLaunchKernelGGL(k_bsc_create, dim3(blocks_for(nv, kBlock)), dim3(kBlock), 0, c->stream, g,
-                       (int)bbase, (int)nv, (int)cbase, U, (const int32_t*)c->d_o0,
-                       (const int32_t*)c->d_o1);
-    LAUNCH_CK("border child create");
-    hipLaunchKernelGGL(k_sc_partner, dim3(blocks_for(nv, kBlock)), dim3(kBlock), 0, c->stream, g,
-                       (int)(cbase + U), (const DevState*)c->d_st);
-    LAUNCH_CK("border child partners");
-    hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, c->stream, c->d_st, lv, (int)vbase, (int)nv,
-                       (int)bbase, (int)cbase, U);
-    LAUNCH_CK("level table");
-    // host-side upper bound for the next level until the next readback
-    c->lv[lv + 1] = LevelDesc{(int)vbase, (int)nv, (int)bbase, (int)(7 * nv), (int)cbase, (int)(U + nv), 0};
+  // polygon records
+  c->pd.assign(np, PolyDev());
+  c->plv.assign(np, std::vector<LevelDesc>());
+  c->pgamma.assign(np, 0.f);
+  int64_t e0 = 0;
+  for (int p = 0; p < np; p++) {
+    PolyDev& P = c->pd[p];
+    memset(&P, 0, sizeof(P));
+    P.e0 = (int32_t)e0;
+    P.N = c->polys[p].N;
+    P.cell0 = kGrid0Cells * p;
+    P.refining = maxd > 1 ? 1 : 0;
+    P.L = 0;
+    P.cs = kGrid0Cells * p;
+    P.cn = kGrid0Cells;
+    P.bs = kGrid0Borders * p;
+    P.bn = kGrid0Borders;
+    P.lbs = 0;
+    P.tbl = -1;
+    c->plv[p].push_back(LevelDesc{kGrid0Verts * p, kGrid0Verts, kGrid0Borders * p, kGrid0Borders, kGrid0Cells * p,
+                                  kGrid0Cells, 0});
+    e0 += P.N;
   }
-  if (maxd == 1) c->gamma = (float)(42 * 1.0 / N);
+  c->glv.clear();
+  c->glv.push_back(LevelDesc{0, (int32_t)V0, 0, (int32_t)B0, 0, (int32_t)C0, 0});
+  c->tot_v = V0;
+  c->tot_b = B0;
+  c->tot_c = C0;
+  CK(pd_upload(c));
+  memset(c->h_st, 0, sizeof(DevState));
+  CK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state init"));
+  CK(fill_i32(c, c->d_loc_cur, Nt, 0));
+  CK(fill_i32(c, c->d_loc_prev, Nt, 0));
 
-  // dispatchSELast at level L = gnum - 1
-  const int Lb = c->gnum - 1;
-  int32_t* par = (early && c->gnum != 1) ? c->d_loc_prev : c->d_loc_cur;
-  CK(sync_state(c));
-  for (int l = 0; l < c->gnum; l++) c->lv[l] = c->h_st->lv[l];
-  const LevelDesc LB = c->lv[Lb];
-  if (c->ecap < N) {
+  GridView g = view(c);
+  int lv = 0;
+  for (;; lv++) {
+    const LevelDesc GL = c->glv[lv];
+    bool any = false;
+    for (const PolyDev& P : c->pd) any |= P.refining != 0;
+    if (!any) break;
+    if (lv == maxd - 1) {
+      // the last level is created but never marked (createAHHG :1983)
+      for (PolyDev& P : c->pd)
+        if (P.refining) {
+          P.refining = 0;
+          P.L = lv;
+          P.use_prev = 0;
+        }
+      break;
+    }
+    if (c->prm.verbose) printf("Subdivide level %d on GPU.\n", lv);
+    CK(ensure_flags(c, (int64_t)GL.cnum + 1));
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
+                       lv == 0 ? 1 : 0);
+    LAUNCH_CK("zero");
+    hipLaunchKernelGGL(k_walk<kMark>, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                       (const double*)c->d_edge, (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, lv,
+                       c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                       c->d_st);
+    LAUNCH_CK("mark walk");
+    hipLaunchKernelGGL(k_plan, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
+                       c->d_f0, c->d_f1, c->d_f2, c->d_f3);
+    LAUNCH_CK("plan");
+    CK(scan_i32(c, c->d_f0, c->d_o0, GL.cnum, nullptr, nullptr, 0));
+    CK(scan_i32(c, c->d_f1, c->d_o1, GL.cnum, nullptr, nullptr, 0));
+    CK(scan_i32(c, c->d_f2, c->d_o2, GL.cnum, nullptr, nullptr, 0));
+    CK(scan_i32(c, c->d_f3, c->d_o3, GL.cnum, nullptr, nullptr, 0));
+    hipLaunchKernelGGL(k_plan_totals, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
+                       GL.cstart, (const int32_t*)c->d_f0, (const int32_t*)c->d_f1, (const int32_t*)c->d_f2,
+                       (const int32_t*)c->d_f3, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1,
+                       (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
+    LAUNCH_CK("plan totals");
+    CK(pd_download(c));
+    // density rule per polygon (:1597-1604) and the next level's layout
+    int64_t nv_all = 0, nb_all = 0, nc_all = 0;
+    for (int p = 0; p < np; p++) {
+      PolyDev& P = c->pd[p];
+      if (!P.refining) continue;
+      if (autod) {
+        float k = P.U * 1.0 / P.N;
+        c->pgamma[p] = k;
+        if (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01)) {
+          P.refining = 0;
+          P.L = lv;
+          P.use_prev = lv != 0 ? 1 : 0;
+          continue;
+        }
+      }
+      P.nvb = (int32_t)(c->tot_v + nv_all);
+      P.nbb0 = (int32_t)(c->tot_b + nb_all);
+      P.ncb = (int32_t)(c->tot_c + nc_all);
+      nv_all += P.nv;
+      nb_all += (int64_t)P.nv + P.nbb;
+      nc_all += (int64_t)P.U + P.nbc;
+    }
+    if (nc_all == 0) break;
+    if (c->tot_v + nv_all > INT32_MAX || c->tot_b + nb_all > INT32_MAX || c->tot_c + nc_all > INT32_MAX)
+      return set_err(c, GAHHG_E_LIMIT, "build: level %d exceeds int32 indexing", lv + 1);
+    CK(ensure_v(c, c->tot_v, c->tot_v + nv_all));
+    CK(ensure_b(c, c->tot_b, c->tot_b + nb_all));
+    CK(ensure_c(c, c->tot_c, c->tot_c + nc_all));
+    CK(pd_upload(c));
+    g = view(c);
+    hipLaunchKernelGGL(k_create_center, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
+                       (const PolyDev*)c->d_pd, GL.cstart, GL.cnum, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
+    LAUNCH_CK("create center");
+    hipLaunchKernelGGL(k_border_children, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
+                       (const PolyDev*)c->d_pd, GL.cstart, GL.cnum, (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
+    LAUNCH_CK("border children");
+    const LevelDesc NL{(int32_t)c->tot_v, (int32_t)nv_all, (int32_t)c->tot_b, (int32_t)nb_all, (int32_t)c->tot_c,
+                       (int32_t)nc_all, 0};
+    hipLaunchKernelGGL(k_sc_partner, dim3(blocks_for(NL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
+                       (const PolyDev*)c->d_pd, NL.cstart, NL.cnum);
+    LAUNCH_CK("border child partners");
+    c->glv.push_back(NL);
+    c->tot_v += nv_all;
+    c->tot_b += nb_all;
+    c->tot_c += nc_all;
+    // advance every refining polygon to its new level
+    for (int p = 0; p < np; p++) {
+      PolyDev& P = c->pd[p];
+      if (!P.refining) continue;
+      c->plv[p].back().rnum = P.U;
+      c->plv[p].push_back(LevelDesc{P.nvb, P.nv, P.nbb0, P.nv + P.nbb, P.ncb, P.U + P.nbc, 0});
+      P.pbs = P.bs;
+      P.plbs = P.lbs;
+      P.lbs = P.lbs + P.bn;
+      P.cs = P.ncb;
+      P.cn = P.U + P.nbc;
+      P.bs = P.nbb0;
+      P.bn = P.nv + P.nbb;
+    }
+    CK(pd_upload(c));
+  }
+  if (maxd == 1)
+    for (int p = 0; p < np; p++) c->pgamma[p] = (float)(42 * 1.0 / c->pd[p].N);
+  c->gnum = (int32_t)c->glv.size();
+
+  // bottom segments
+  for (int p = 0; p < np; p++) {
+    PolyDev& P = c->pd[p];
+    const LevelDesc& B = c->plv[p][P.L];
+    P.bcs = B.cstart;
+    P.bcn = B.cnum;
+    P.refining = 0;
+  }
+  CK(pd_upload(c));
+
+  // dispatchSELast for every polygon at its bottom level
+  if (c->ecap < Nt) {
     dfree(c->d_ecnt);
     c->d_ecnt = nullptr;
-    CK(dalloc(c, (void**)&c->d_ecnt, (int64_t)N * 4));
-    c->ecap = N;
+    CK(dalloc(c, (void**)&c->d_ecnt, Nt * 4));
+    c->ecap = Nt;
   }
-  CK(ensure_flags(c, std::max<int64_t>(N, LB.cnum) + 1));
+  CK(ensure_flags(c, std::max<int64_t>(Nt, c->tot_c) + 1));
   g = view(c);
-  hipLaunchKernelGGL(k_walk<kCount>, dim3(blocks_for(N, kBlock)), dim3(kBlock), 0, c->stream, g,
-                     (const double*)c->d_edge, N, Lb, kGrid0Cells, par, (int32_t*)nullptr,
-                     (const int32_t*)nullptr, c->d_ecnt, (int32_t*)nullptr, c->d_st);
+  hipLaunchKernelGGL(k_walk<kCount>, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                     (const double*)c->d_edge, (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, 0,
+                     c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr, c->d_ecnt, (int32_t*)nullptr, c->d_st);
   LAUNCH_CK("count walk");
-  CK(scan_i32(c, c->d_ecnt, c->d_o0, N, nullptr, &c->d_st->R, 0));
+  CK(scan_i32(c, c->d_ecnt, c->d_o0, Nt, nullptr, &c->d_st->R, 0));
+  hipLaunchKernelGGL(k_poly_refs, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
+                     (const int32_t*)c->d_o0, (const int32_t*)c->d_ecnt);
+  LAUNCH_CK("polygon refs");
   CK(sync_state(c));
+  CK(pd_download(c));
   const int64_t R = c->h_st->R;
   if (R < 0) return set_err(c, GAHHG_E_LIMIT, "dispatch: ref count overflow");
+  c->tot_r = R;
   CK(ensure_refs(c, R + 1));
   g = view(c);
-  hipLaunchKernelGGL(k_walk<kWrite>, dim3(blocks_for(N, kBlock)), dim3(kBlock), 0, c->stream, g,
-                     (const double*)c->d_edge, N, Lb, kGrid0Cells, par, (int32_t*)nullptr,
-                     (const int32_t*)c->d_o0, (int32_t*)nullptr, c->d_rcell, c->d_st);
+  hipLaunchKernelGGL(k_walk<kWrite>, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                     (const double*)c->d_edge, (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, 0,
+                     c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0, (int32_t*)nullptr, c->d_rcell, c->d_st);
   LAUNCH_CK("write walk");
-  // ref -> edge id: inclusive scan of head flags
   CK(fill_i32(c, c->d_rflag, R, 0));
-  hipLaunchKernelGGL(k_edge_heads, dim3(blocks_for(N, kBlock)), dim3(kBlock), 0, c->stream,
-                     (const int32_t*)c->d_o0, N, (const DevState*)c->d_st, c->d_rflag);
+  hipLaunchKernelGGL(k_edge_heads, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_o0,
+                     (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, c->d_rflag);
   LAUNCH_CK("edge heads");
-  CK(scan_i32(c, c->d_rflag, c->d_redge, R, nullptr, nullptr, 1));
+  CK(scan_i32(c, c->d_rflag, c->d_o1, R, nullptr, nullptr, 1));
+  hipLaunchKernelGGL(k_ref_edges, dim3(blocks_for(R, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
+                     (const int32_t*)c->d_o1, (const int32_t*)c->g.c_poly, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, c->d_redge);
+  LAUNCH_CK("ref edges");
   // counting sort into cell-major order
-  CK(fill_i32(c, c->d_f1, LB.cnum, 0));
+  const int64_t C = c->tot_c;
+  CK(fill_i32(c, c->d_f1, C, 0));
   hipLaunchKernelGGL(k_count_cells, dim3(blocks_for(R, kBlock)), dim3(kBlock), 0, c->stream,
-                     (const int32_t*)c->d_rcell, (const DevState*)c->d_st, LB.cstart, c->d_f1);
+                     (const int32_t*)c->d_rcell, (const DevState*)c->d_st, c->d_f1);
   LAUNCH_CK("count cells");
-  CK(scan_i32(c, c->d_f1, c->d_o1, LB.cnum, nullptr, nullptr, 0));
-  CK(copy_i32(c, c->d_f0, c->d_o1, LB.cnum));
+  CK(scan_i32(c, c->d_f1, c->d_o1, C, nullptr, nullptr, 0));
+  CK(copy_i32(c, c->d_f0, c->d_o1, C));
   hipLaunchKernelGGL(k_scatter_refs, dim3(blocks_for(R, kBlock)), dim3(kBlock), 0, c->stream,
-                     (const int32_t*)c->d_rcell, (const int32_t*)c->d_redge, (const DevState*)c->d_st,
-                     LB.cstart, c->d_f0, c->d_rflag);
+                     (const int32_t*)c->d_rcell, (const int32_t*)c->d_redge, (const DevState*)c->d_st, c->d_f0,
+                     c->d_rflag);
   LAUNCH_CK("scatter refs");
-  hipLaunchKernelGGL(k_finish_cells, dim3(blocks_for(LB.cnum, 4)), dim3(256), 0, c->stream, g,
-                     LB.cstart, LB.cnum, (const int32_t*)c->d_f1, (const int32_t*)c->d_o1,
-                     (const int32_t*)c->d_rflag);
+  hipLaunchKernelGGL(k_finish_cells, dim3(blocks_for(C, 4)), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (int)C, (const int32_t*)c->d_f1, (const int32_t*)c->d_o1, (const int32_t*)c->d_rflag);
   LAUNCH_CK("finish cells");
-  CK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
+  for (int p = 0; p < np; p++) c->plv[p][c->pd[p].L].rnum = c->pd[p].R;
   CK(sync_state(c));
-  c->lv[Lb].rnum = (int32_t)R;
   return GAHHG_OK;
 }
 

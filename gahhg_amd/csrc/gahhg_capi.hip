@@ -35,8 +35,8 @@ int select_device(gahhg_ctx* c) { return hip_check(c, hipSetDevice(c->device), "
 
 void free_grid(gahhg_ctx* c) {
   DeviceGrid& g = c->g;
-  void* ptrs[] = {g.v, g.b_parent, g.b_partner, g.b_v0, g.b_v1, g.b_subcell, g.b_id, g.b_n,
-                  g.c_bstart, g.c_bnum, g.c_rstart, g.c_rnum, g.c_upcell, g.c_subcell, g.c_type, g.refs};
+  void* ptrs[] = {g.v, g.b_parent, g.b_partner, g.b_v0, g.b_v1, g.b_subcell, g.b_id, g.b_n, g.c_bstart,
+                  g.c_bnum, g.c_rstart, g.c_rnum, g.c_upcell, g.c_subcell, g.c_type, g.c_poly, g.refs};
   for (void* p : ptrs) dfree(p);
   g = DeviceGrid();
   QueryLayout& q = c->q;
@@ -52,34 +52,170 @@ void free_grid(gahhg_ctx* c) {
 }
 
 void free_scratch(gahhg_ctx* c) {
-  void* ptrs[] = {c->d_f0, c->d_f1, c->d_o0, c->d_o1, c->d_scan, c->d_ecnt, c->d_eoff,
-                  c->d_rcell, c->d_rflag, c->d_redge, c->d_loc_cur, c->d_loc_prev};
+  void* ptrs[] = {c->d_f0, c->d_f1, c->d_o0, c->d_o1, c->d_f2, c->d_f3, c->d_o2, c->d_o3, c->d_scan,
+                  c->d_ecnt, c->d_rcell, c->d_rflag, c->d_redge, c->d_loc_cur, c->d_loc_prev};
   for (void* p : ptrs) dfree(p);
   c->d_f0 = c->d_f1 = c->d_o0 = c->d_o1 = c->d_scan = c->d_ecnt = nullptr;
-  c->d_eoff = nullptr;
+  c->d_f2 = c->d_f3 = c->d_o2 = c->d_o3 = nullptr;
   c->d_rcell = c->d_rflag = c->d_redge = c->d_loc_cur = c->d_loc_prev = nullptr;
   c->fcap = c->scap = c->ecap = c->r2cap = 0;
 }
 
-int upload_polygon(gahhg_ctx* c) {
+int upload_polygons(gahhg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_grid(c);
   free_scratch(c);
-  dfree(c->d_pv);
-  dfree(c->d_pe);
-  dfree(c->d_edge);
+  void* ptrs[] = {c->d_pv, c->d_pe, c->d_edge, c->d_epoly, c->d_pd};
+  for (void* p : ptrs) dfree(p);
   c->d_pv = nullptr;
   c->d_pe = nullptr;
   c->d_edge = nullptr;
-  const int64_t N = c->poly.N;
+  c->d_epoly = nullptr;
+  c->d_pd = nullptr;
+  c->have_poly = false;
+  int64_t N = 0;
+  for (const Polygon& p : c->polys) N += p.N;
+  if (N > INT32_MAX / 2) return set_err(c, GAHHG_E_LIMIT, "batch: too many edges");
+  c->n_edges = N;
+  std::vector<double> pv((size_t)(3 * N));
+  std::vector<int32_t> pe((size_t)(2 * N)), ep((size_t)N);
+  int64_t off = 0;
+  for (size_t k = 0; k < c->polys.size(); k++) {
+    const Polygon& p = c->polys[k];
+    std::copy(p.pv.begin(), p.pv.end(), pv.begin() + 3 * off);
+    for (int64_t i = 0; i < 2LL * p.N; i++) pe[2 * off + i] = (int32_t)(p.pe[i] + off);
+    for (int64_t i = 0; i < p.N; i++) ep[off + i] = (int32_t)k;
+    off += p.N;
+  }
   RCK(dalloc(c, (void**)&c->d_pv, N * 24));
   RCK(dalloc(c, (void**)&c->d_pe, N * 8));
+  RCK(dalloc(c, (void**)&c->d_epoly, N * 4));
   RCK(dalloc(c, (void**)&c->d_edge, N * 72));
   RCK(dalloc(c, (void**)&c->d_loc_cur, N * 4));
   RCK(dalloc(c, (void**)&c->d_loc_prev, N * 4));
-  RCK(hip_check(c, hipMemcpy(c->d_pv, c->poly.pv.data(), N * 24, hipMemcpyHostToDevice), "polygon H2D"));
-  RCK(hip_check(c, hipMemcpy(c->d_pe, c->poly.pe.data(), N * 8, hipMemcpyHostToDevice), "polygon H2D"));
+  RCK(dalloc(c, (void**)&c->d_pd, (int64_t)c->polys.size() * (int64_t)sizeof(PolyDev)));
+  RCK(hip_check(c, hipMemcpy(c->d_pv, pv.data(), N * 24, hipMemcpyHostToDevice), "polygon H2D"));
+  RCK(hip_check(c, hipMemcpy(c->d_pe, pe.data(), N * 8, hipMemcpyHostToDevice), "polygon H2D"));
+  RCK(hip_check(c, hipMemcpy(c->d_epoly, ep.data(), N * 4, hipMemcpyHostToDevice), "polygon H2D"));
   c->have_poly = true;
+  return GAHHG_OK;
+}
+
+// Per-polygon statistics in the reference's terms (GPISP::getStat).
+void poly_stats(gahhg_ctx* c, int p, gahhg_stats* o) {
+  memset(o, 0, sizeof(*o));
+  const std::vector<LevelDesc>& L = c->plv[p];
+  const int depth = c->pd[p].L + 1;
+  for (int l = 0; l < depth; l++) {
+    o->vnum += L[l].vnum;
+    o->bnum += L[l].bnum;
+    o->cnum += L[l].cnum;
+  }
+  o->rnum = c->pd[p].R;
+  o->depth = depth;
+  o->gamma = c->pgamma[p];
+  o->n_edges = c->polys[p].N;
+  o->vspace = 24 * o->vnum;
+  o->bspace = 48 * o->bnum;
+  o->cspace = 28 * o->cnum;
+  o->rspace = 4 * o->rnum;
+  o->gpu_space_bytes = o->vspace + o->bspace + o->cspace + o->rspace;
+  // sizeof(GAHHG) + gnum * sizeof(GAHHGGrid) + sizeof(GPISPParam) +
+  // sizeof(GPISPStat) on the reference's 64-bit build (144 + 28 gnum + 552
+  // + 96), GPISPAHHG.cu:2100.
+  o->cpu_space_bytes = 792 + 28LL * depth;
+  o->orientation_flipped = c->polys[p].flips;
+}
+
+// Segment map from global to one polygon's local indices.
+struct SegMap {
+  std::vector<int64_t> g0, g1, l0;
+  int32_t map(int32_t x) const {
+    if (x < 0) return x;
+    for (size_t k = 0; k < g0.size(); k++)
+      if (x >= g0[k] && x < g1[k]) return (int32_t)(l0[k] + (x - g0[k]));
+    return INT32_MIN;  // outside this polygon (never expected)
+  }
+};
+
+// Export polygon p's hierarchy in its local (single-polygon reference)
+// numbering.
+int export_local(gahhg_ctx* c, int p, int64_t* counts, double* v, int32_t* cells, int32_t* borders,
+                 double* normals, int32_t* refs, int32_t* levels) {
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  gahhg_stats ps;
+  poly_stats(c, p, &ps);
+  const int depth = ps.depth;
+  if (counts) {
+    counts[0] = depth;
+    counts[1] = ps.vnum;
+    counts[2] = ps.bnum;
+    counts[3] = ps.cnum;
+    counts[4] = ps.rnum;
+  }
+  if (!v && !cells && !borders && !normals && !refs && !levels) return GAHHG_OK;
+  const std::vector<LevelDesc>& L = c->plv[p];
+  SegMap mv, mb, mc;
+  int64_t lv = 0, lb = 0, lc = 0;
+  for (int l = 0; l < depth; l++) {
+    mv.g0.push_back(L[l].vstart), mv.g1.push_back(L[l].vstart + L[l].vnum), mv.l0.push_back(lv);
+    mb.g0.push_back(L[l].bstart), mb.g1.push_back(L[l].bstart + L[l].bnum), mb.l0.push_back(lb);
+    mc.g0.push_back(L[l].cstart), mc.g1.push_back(L[l].cstart + L[l].cnum), mc.l0.push_back(lc);
+    if (levels) {
+      int32_t* o = levels + 7 * l;
+      o[0] = (int32_t)lv, o[1] = L[l].vnum, o[2] = (int32_t)lb, o[3] = L[l].bnum, o[4] = (int32_t)lc,
+      o[5] = L[l].cnum, o[6] = L[l].rnum;
+    }
+    lv += L[l].vnum, lb += L[l].bnum, lc += L[l].cnum;
+  }
+  auto d2h = [&](void* h, const void* d, int64_t bytes) {
+    return hip_check(c, hipMemcpy(h, d, (size_t)bytes, hipMemcpyDeviceToHost), "export");
+  };
+  const PolyDev& P = c->pd[p];
+  // this polygon's bottom cells are contiguous, so their refs form one run
+  // of the cell-major ref array, starting at the first bottom cell's offset
+  int32_t rbase = 0;
+  RCK(d2h(&rbase, c->g.c_rstart + P.bcs, 4));
+  for (int l = 0; l < depth; l++) {
+    const LevelDesc& D = L[l];
+    const int64_t V = D.vnum, B = D.bnum, C = D.cnum;
+    if (v) RCK(d2h(v + 3 * mv.l0[l], c->g.v + 3 * (int64_t)D.vstart, V * 24));
+    if (normals) RCK(d2h(normals + 3 * mb.l0[l], c->g.b_n + 3 * (int64_t)D.bstart, B * 24));
+    std::vector<int32_t> col;
+    if (cells) {
+      col.resize((size_t)std::max<int64_t>(C, 1));
+      int32_t* cs[7] = {c->g.c_bstart, c->g.c_bnum, c->g.c_rstart, c->g.c_rnum, c->g.c_upcell, c->g.c_subcell,
+                        c->g.c_type};
+      for (int k = 0; k < 7; k++) {
+        RCK(d2h(col.data(), cs[k] + D.cstart, C * 4));
+        for (int64_t i = 0; i < C; i++) {
+          int32_t x = col[i];
+          if (k == 0) x = mb.map(x);
+          else if (k == 2 && x >= 0) x -= rbase;
+          else if (k == 4 || k == 5) x = mc.map(x);
+          cells[7 * (mc.l0[l] + i) + k] = x;
+        }
+      }
+    }
+    if (borders) {
+      col.resize((size_t)std::max<int64_t>(B, 1));
+      int32_t* bs[6] = {c->g.b_parent, c->g.b_partner, c->g.b_v0, c->g.b_v1, c->g.b_subcell, c->g.b_id};
+      for (int k = 0; k < 6; k++) {
+        RCK(d2h(col.data(), bs[k] + D.bstart, B * 4));
+        for (int64_t i = 0; i < B; i++) {
+          int32_t x = col[i];
+          if (k == 0 || k == 4) x = mc.map(x);
+          else if (k == 1) x = mb.map(x);
+          else if (k == 2 || k == 3) x = mv.map(x);
+          borders[6 * (mb.l0[l] + i) + k] = x;
+        }
+      }
+    }
+  }
+  if (refs && P.R > 0) {
+    RCK(d2h(refs, c->g.refs + rbase, (int64_t)P.R * 4));
+    for (int64_t i = 0; i < P.R; i++) refs[i] -= P.e0;
+  }
   return GAHHG_OK;
 }
 
@@ -165,8 +301,9 @@ int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
   Polygon p;
   int rc = load_polygon_txt(path, &p, &c->err);
   if (rc) return rc;
-  c->poly = std::move(p);
-  return upload_polygon(c);
+  c->polys.clear();
+  c->polys.push_back(std::move(p));
+  return upload_polygons(c);
 }
 
 int gahhg_set_polygon_lonlat(gahhg_ctx* c, const double* lon, const double* lat, const int64_t* ring_off,
@@ -176,8 +313,25 @@ int gahhg_set_polygon_lonlat(gahhg_ctx* c, const double* lon, const double* lat,
   Polygon p;
   int rc = polygon_from_lonlat(lon, lat, ring_off, n_rings, &p, &c->err);
   if (rc) return rc;
-  c->poly = std::move(p);
-  return upload_polygon(c);
+  c->polys.clear();
+  c->polys.push_back(std::move(p));
+  return upload_polygons(c);
+}
+
+int gahhg_set_polygons_lonlat(gahhg_ctx* c, const double* lon, const double* lat, const int64_t* ring_off,
+                              const int32_t* poly_ring_off, int32_t n_polys) {
+  RCK(ctx_ok(c));
+  if (n_polys < 1 || !poly_ring_off || !ring_off) return set_err(c, GAHHG_E_ARG, "batch: need polygons");
+  RCK(select_device(c));
+  std::vector<Polygon> ps((size_t)n_polys);
+  for (int k = 0; k < n_polys; k++) {
+    const int32_t r0 = poly_ring_off[k], r1 = poly_ring_off[k + 1];
+    if (r1 <= r0) return set_err(c, GAHHG_E_ARG, "batch: polygon %d has no ring", k);
+    int rc = polygon_from_lonlat(lon, lat, ring_off + r0, r1 - r0, &ps[k], &c->err);
+    if (rc) return rc;
+  }
+  c->polys = std::move(ps);
+  return upload_polygons(c);
 }
 
 int gahhg_build(gahhg_ctx* c) {
@@ -194,45 +348,46 @@ int gahhg_build(gahhg_ctx* c) {
   RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
   const double t2 = now_ms();
   c->built = true;
+  if (c->polys.size() == 1) {
+    poly_stats(c, 0, &c->st);
+  } else {
+    gahhg_stats tot, ps;
+    memset(&tot, 0, sizeof(tot));
+    for (int p = 0; p < (int)c->polys.size(); p++) {
+      poly_stats(c, p, &ps);
+      tot.vnum += ps.vnum, tot.bnum += ps.bnum, tot.cnum += ps.cnum, tot.rnum += ps.rnum;
+      tot.vspace += ps.vspace, tot.bspace += ps.bspace, tot.cspace += ps.cspace, tot.rspace += ps.rspace;
+      tot.gpu_space_bytes += ps.gpu_space_bytes;
+      tot.cpu_space_bytes += ps.cpu_space_bytes;
+      tot.depth = std::max(tot.depth, ps.depth);
+    }
+    tot.n_edges = c->n_edges;
+    c->st = tot;
+  }
   c->st.create_ms = t1 - t0;
   c->st.classify_ms = t2 - t1;
   c->st.pre_ms = t2 - t0;
-  c->st.depth = c->gnum;
-  c->st.gamma = c->gamma;
-  c->st.n_edges = c->poly.N;
-  const LevelDesc& L = c->lv[c->gnum - 1];
-  c->st.vnum = (int64_t)L.vstart + L.vnum;
-  c->st.bnum = (int64_t)L.bstart + L.bnum;
-  c->st.cnum = (int64_t)L.cstart + L.cnum;
-  c->st.rnum = L.rnum;
-  c->st.vspace = 24 * c->st.vnum;
-  c->st.bspace = 48 * c->st.bnum;
-  c->st.cspace = 28 * c->st.cnum;
-  c->st.rspace = 4 * c->st.rnum;
-  c->st.gpu_space_bytes = c->st.vspace + c->st.bspace + c->st.cspace + c->st.rspace;
-  // sizeof(GAHHG) + gnum * sizeof(GAHHGGrid) + sizeof(GPISPParam) +
-  // sizeof(GPISPStat) on the reference's 64-bit build (144 + 28 gnum + 552
-  // + 96), GPISPAHHG.cu:2100.
-  c->st.cpu_space_bytes = 792 + 28LL * c->gnum;
-  c->st.orientation_flipped = c->poly.flips;
   const DevState& s = *c->h_st;
   c->st.walk_steps_mark = (int64_t)s.steps_mark;
   c->st.walk_steps_dispatch = (int64_t)s.steps_dispatch;
   c->st.walk_partner_missing = (int64_t)s.partner_missing;
   c->st.walk_step_cap_hits = (int64_t)s.cap_hits;
   c->st.walk_unlocated = (int64_t)s.unlocated;
-  c->st.mark_passes = c->gnum - 1 + ((c->gnum < c->prm.max_depth || c->prm.max_depth == 1) ? 1 : 0);
-  if (c->prm.max_depth == 1) c->st.mark_passes = 0;
+  c->st.undefined_leaves_snapshot = c->und_snap;
+  c->st.undefined_leaves_final = c->und_final;
   return GAHHG_OK;
 }
 
-int gahhg_query_xyz_device(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
+int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n,
+                                  int32_t* d_labels) {
   RCK(ctx_ok(c));
   if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
   if (n < 0 || (n > 0 && (!d_xyz || !d_labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  if (!d_poly && c->polys.size() != 1)
+    return set_err(c, GAHHG_E_ARG, "query: polygon ids required for a batch of %d", (int)c->polys.size());
   RCK(select_device(c));
   RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
-  RCK(run_query(c, d_xyz, n, d_labels));
+  RCK(run_query(c, d_xyz, d_poly, n, d_labels));
   RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
   RCK(hip_check(c, hipEventSynchronize(c->ev[1]), "query sync"));
   float ms = 0.f;
@@ -249,6 +404,33 @@ int gahhg_query_xyz_device(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t
   c->st.unlocated = (int64_t)h_two[1];
   c->st.outside = n - c->st.inside;
   return GAHHG_OK;
+}
+
+int gahhg_query_xyz_device(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
+  return gahhg_query_batch_xyz_device(c, d_xyz, nullptr, n, d_labels);
+}
+
+int gahhg_query_batch_xyz(gahhg_ctx* c, const double* xyz, const int32_t* poly, int64_t n, int32_t* labels) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
+  if (n < 0 || (n > 0 && (!xyz || !labels || !poly))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  if (n == 0) return GAHHG_OK;
+  for (int64_t i = 0; i < n; i++)
+    if (poly[i] < 0 || poly[i] >= (int32_t)c->polys.size()) return set_err(c, GAHHG_E_ARG, "query: bad polygon id");
+  RCK(select_device(c));
+  double* d_xyz = nullptr;
+  int32_t *d_lab = nullptr, *d_poly = nullptr;
+  RCK(dalloc(c, (void**)&d_xyz, n * 24));
+  int rc = dalloc(c, (void**)&d_lab, n * 4);
+  if (!rc) rc = dalloc(c, (void**)&d_poly, n * 4);
+  if (!rc) rc = hip_check(c, hipMemcpy(d_xyz, xyz, n * 24, hipMemcpyHostToDevice), "query H2D");
+  if (!rc) rc = hip_check(c, hipMemcpy(d_poly, poly, n * 4, hipMemcpyHostToDevice), "query H2D");
+  if (!rc) rc = gahhg_query_batch_xyz_device(c, d_xyz, d_poly, n, d_lab);
+  if (!rc) rc = hip_check(c, hipMemcpy(labels, d_lab, n * 4, hipMemcpyDeviceToHost), "labels D2H");
+  dfree(d_xyz);
+  dfree(d_lab);
+  dfree(d_poly);
+  return rc;
 }
 
 int gahhg_query_xyz(gahhg_ctx* c, const double* xyz, int64_t n, int32_t* labels) {
@@ -305,7 +487,7 @@ int gahhg_get_stats(gahhg_ctx* c, gahhg_stats* out) {
   int64_t bytes = 0;
   bytes += c->g.vcap * 24 + c->g.bcap * (6 * 4 + 24) + c->g.ccap * 7 * 4 + c->g.rcap * 4;
   bytes += c->q.bytes;
-  bytes += (int64_t)c->poly.N * (24 + 8 + 72 + 8);
+  bytes += c->n_edges * (24 + 8 + 72 + 12);
   out->device_bytes = bytes;
   return GAHHG_OK;
 }
@@ -320,7 +502,7 @@ int gahhg_write_report(gahhg_ctx* c, const char* path, const char* poly_name, co
   fprintf(fp, "\n----------------------------------------%d/%d/%d %d:%d:%d-----------------------------------------\n",
           lt.tm_year + 1900, lt.tm_mon + 1, lt.tm_mday, lt.tm_hour, lt.tm_min, lt.tm_sec);
   fprintf(fp, "\nPolygon file: %s\n", poly_name ? poly_name : "");
-  fprintf(fp, "Number of polygon vertice: %d\n", c->poly.N);
+  fprintf(fp, "Number of polygon vertice: %lld\n", (long long)c->n_edges);
   fprintf(fp, "Query point file: %s\n", query_name ? query_name : "");
   fprintf(fp, "Number of query points: %lld\n", (long long)c->st.n_queries);
   fprintf(fp, "Depth of GAHHG: %d\n", c->st.depth);
@@ -379,50 +561,44 @@ int gahhg_memcpy_d2h(gahhg_ctx* c, void* h, const void* d, int64_t bytes) {
 }
 
 int gahhg_grid_counts(gahhg_ctx* c, int64_t* counts) {
+  return gahhg_polygon_grid_counts(c, 0, counts);
+}
+
+int gahhg_export_grid(gahhg_ctx* c, double* v, int32_t* cells, int32_t* borders, double* normals, int32_t* refs,
+                      int32_t* levels) {
+  return gahhg_export_polygon_grid(c, 0, v, cells, borders, normals, refs, levels);
+}
+
+int gahhg_num_polygons(gahhg_ctx* c, int32_t* n) {
   RCK(ctx_ok(c));
-  if (!counts) return GAHHG_E_ARG;
-  if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
-  counts[0] = c->gnum;
-  counts[1] = c->st.vnum;
-  counts[2] = c->st.bnum;
-  counts[3] = c->st.cnum;
-  counts[4] = c->st.rnum;
+  if (!n) return GAHHG_E_ARG;
+  *n = (int32_t)c->polys.size();
   return GAHHG_OK;
 }
 
-int gahhg_export_grid(gahhg_ctx* c, double* v, int32_t* cells, int32_t* borders, double* normals,
-                      int32_t* refs, int32_t* levels) {
+int gahhg_polygon_grid_counts(gahhg_ctx* c, int32_t p, int64_t* counts) {
+  RCK(ctx_ok(c));
+  if (!counts) return GAHHG_E_ARG;
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
+  if (p < 0 || p >= (int32_t)c->polys.size()) return set_err(c, GAHHG_E_ARG, "bad polygon index");
+  return export_local(c, p, counts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int gahhg_export_polygon_grid(gahhg_ctx* c, int32_t p, double* v, int32_t* cells, int32_t* borders,
+                              double* normals, int32_t* refs, int32_t* levels) {
   RCK(ctx_ok(c));
   if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
+  if (p < 0 || p >= (int32_t)c->polys.size()) return set_err(c, GAHHG_E_ARG, "bad polygon index");
   RCK(select_device(c));
-  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
-  const int64_t V = c->st.vnum, B = c->st.bnum, C = c->st.cnum, R = c->st.rnum;
-  auto d2h = [&](void* h, const void* d, int64_t bytes) {
-    return hip_check(c, hipMemcpy(h, d, (size_t)bytes, hipMemcpyDeviceToHost), "export");
-  };
-  if (v) RCK(d2h(v, c->g.v, V * 24));
-  if (normals) RCK(d2h(normals, c->g.b_n, B * 24));
-  if (refs && R > 0) RCK(d2h(refs, c->g.refs, R * 4));
-  std::vector<int32_t> col;
-  if (cells) {
-    col.resize((size_t)std::max<int64_t>(C, 1));
-    int32_t* cs[7] = {c->g.c_bstart, c->g.c_bnum, c->g.c_rstart, c->g.c_rnum, c->g.c_upcell, c->g.c_subcell,
-                      c->g.c_type};
-    for (int k = 0; k < 7; k++) {
-      RCK(d2h(col.data(), cs[k], C * 4));
-      for (int64_t i = 0; i < C; i++) cells[7 * i + k] = col[i];
-    }
-  }
-  if (borders) {
-    col.resize((size_t)std::max<int64_t>(B, 1));
-    int32_t* bs[6] = {c->g.b_parent, c->g.b_partner, c->g.b_v0, c->g.b_v1, c->g.b_subcell, c->g.b_id};
-    for (int k = 0; k < 6; k++) {
-      RCK(d2h(col.data(), bs[k], B * 4));
-      for (int64_t i = 0; i < B; i++) borders[6 * i + k] = col[i];
-    }
-  }
-  if (levels)
-    for (int l = 0; l < c->gnum; l++) memcpy(levels + 7 * l, &c->lv[l], 7 * sizeof(int32_t));
+  return export_local(c, p, nullptr, v, cells, borders, normals, refs, levels);
+}
+
+int gahhg_get_polygon_stats(gahhg_ctx* c, int32_t p, gahhg_stats* out) {
+  RCK(ctx_ok(c));
+  if (!out) return GAHHG_E_ARG;
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "no grid");
+  if (p < 0 || p >= (int32_t)c->polys.size()) return set_err(c, GAHHG_E_ARG, "bad polygon index");
+  poly_stats(c, p, out);
   return GAHHG_OK;
 }
 

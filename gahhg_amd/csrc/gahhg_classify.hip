@@ -129,11 +129,19 @@ __device__ int parity_label(const GridView& g, const double* __restrict__ edge, 
   return (cnt % 2 == 0) ? from_type : 3 - from_type;
 }
 
-__global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const double* pv, const int32_t* pe,
-                                                const double* edge, int cstart, int cnum) {
+// Cells of one level whose polygon bottoms out there (bottom) or deeper
+// (upper); every polygon of the batch follows its own reference sequence.
+__device__ __forceinline__ int cell_phase(const GridView& g, const PolyDev* pd, int cid, int lv) {
+  const int L = pd[g.c_poly[cid]].L;
+  return L == lv ? 0 : (L > lv ? 1 : 2);
+}
+
+__global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __restrict__ pd, const double* pv,
+                                                const int32_t* pe, const double* edge, int cstart, int cnum, int lv) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
   const int cid = cstart + t;
+  if (cell_phase(g, pd, cid, lv) != 0) return;
   if (g.c_rnum[cid] != 0) g.c_type[cid] = cp_prop(g, pv, pe, edge, cid, mean_center(g, cid));
 }
 
@@ -144,11 +152,13 @@ __global__ void k_snap(const int32_t* type, int cstart, int cnum, int32_t* snap)
 
 // setBottomDECell: every border whose neighbour was labelled in the
 // snapshot decides in turn; the last one wins, as in the reference loop.
-__global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const double* edge, int cstart,
-                                                       int cnum, const int32_t* snap) {
+__global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev* __restrict__ pd,
+                                                       const double* edge, int cstart, int cnum, int lv,
+                                                       const int32_t* snap) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum || snap[t] != kUndefined) return;
   const int cid = cstart + t;
+  if (cell_phase(g, pd, cid, lv) != 0) return;
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
   int lab = kUndefined;
   bool set = false;
@@ -188,10 +198,12 @@ __global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum
 
 // setDECell: an empty unlabelled cell above the bottom takes its label from
 // the leaf reached through the first refined neighbour.
-__global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const double* edge, int cstart, int cnum) {
+__global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const PolyDev* __restrict__ pd,
+                                                      const double* edge, int cstart, int cnum, int lv) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
   const int cid = cstart + t;
+  if (cell_phase(g, pd, cid, lv) != 1) return;
   if (!(g.c_rnum[cid] == 0 && g.c_type[cid] == kUndefined)) return;
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
   int pe_ = -1, pc = -1;
@@ -212,21 +224,24 @@ __global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const double* 
     g.c_type[cid] = parity_label(g, edge, target, pe_, g.c_type[target]);
 }
 
-// setTopECell on the 42 level-0 cells, repeated to a fixpoint in one block.
-__global__ void k_top(GridView g, int cnum) {
+// setTopECell on one polygon's 42 level-0 cells (block = polygon),
+// repeated to a fixpoint.
+__global__ void k_top(GridView g, const PolyDev* __restrict__ pd) {
   __shared__ int snap[64];
   __shared__ int changed;
+  const int base = pd[blockIdx.x].cell0;
+  const int cnum = kGrid0Cells;
   const int t = threadIdx.x;
   for (;;) {
-    if (t < cnum) snap[t] = g.c_type[t];
+    if (t < cnum) snap[t] = g.c_type[base + t];
     if (t == 0) changed = 0;
     __syncthreads();
-    if (t < cnum && g.c_rnum[t] == 0 && snap[t] == kUndefined) {
-      const int bs = g.c_bstart[t], bn = g.c_bnum[t];
+    if (t < cnum && g.c_rnum[base + t] == 0 && snap[t] == kUndefined) {
+      const int bs = g.c_bstart[base + t], bn = g.c_bnum[base + t];
       for (int i = 0; i < bn; i++) {
         const int pe_ = g.b_partner[bs + i];
-        if (pe_ != -1 && snap[g.b_parent[pe_]] != kUndefined) {
-          g.c_type[t] = snap[g.b_parent[pe_]];
+        if (pe_ != -1 && snap[g.b_parent[pe_] - base] != kUndefined) {
+          g.c_type[base + t] = snap[g.b_parent[pe_] - base];
           changed = 1;
           break;
         }
@@ -261,7 +276,7 @@ __global__ void k_clear_undef(DevState* st) { st->undefined_leaves = 0; }
 // means the last round changed nothing.  Returns the number of leaves still
 // unlabelled after the first round via *after_first.
 static int ie_fixpoint(gahhg_ctx* c, int lv, int32_t* snap, unsigned long long* after_first) {
-  const LevelDesc L = c->lv[lv];
+  const LevelDesc L = c->glv[lv];
   GridView g = view(c);
   bool first = true;
   for (int batch = 0; batch < 1 << 20; batch++) {
@@ -290,46 +305,42 @@ static int ie_fixpoint(gahhg_ctx* c, int lv, int32_t* snap, unsigned long long* 
 }
 
 int classify_hierarchy(gahhg_ctx* c) {
-  const int Lb = c->gnum - 1;
-  const LevelDesc LB = c->lv[Lb];
+  const int np = (int)c->polys.size();
+  int maxL = 0;
+  for (const PolyDev& P : c->pd) maxL = std::max(maxL, (int)P.L);
   GridView g = view(c);
-  int64_t maxc = 0;
-  for (int l = 0; l < c->gnum; l++) maxc = std::max<int64_t>(maxc, c->lv[l].cnum);
-  // snapshot buffer: the flag scratch holds at least max(cnum) entries
   int32_t* snap = c->d_f0;
-  if (c->fcap < maxc) return set_err(c, GAHHG_E_STATE, "classify: scratch too small");
+  for (int lv = 0; lv <= maxL; lv++)
+    if (c->fcap < c->glv[lv].cnum) return set_err(c, GAHHG_E_STATE, "classify: scratch too small");
   unsigned long long und_snap = 0, und1 = 0;
-  if (c->prm.verbose) printf("Classify level %d cells.\n", Lb);
-  hipLaunchKernelGGL(k_ne, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream, g, (const double*)c->d_pv,
-                     (const int32_t*)c->d_pe, (const double*)c->d_edge, LB.cstart, LB.cnum);
-  hipLaunchKernelGGL(k_snap, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream,
-                     (const int32_t*)c->g.c_type, LB.cstart, LB.cnum, snap);
-  hipLaunchKernelGGL(k_de_bottom, dim3(cblocks(LB.cnum)), dim3(kCBlock), 0, c->stream, g,
-                     (const double*)c->d_edge, LB.cstart, LB.cnum, (const int32_t*)snap);
-  CCK(hip_check(c, hipGetLastError(), "classify bottom"));
-  CCK(ie_fixpoint(c, Lb, snap, &und1));
-  und_snap += und1;
-  for (int lv = Lb - 1; lv >= 0; lv--) {
+  const PolyDev* pd = c->d_pd;
+  for (int lv = maxL; lv >= 0; lv--) {
     if (c->prm.verbose) printf("Classify level %d cells.\n", lv);
-    const LevelDesc L = c->lv[lv];
-    hipLaunchKernelGGL(k_de_upper, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g,
-                       (const double*)c->d_edge, L.cstart, L.cnum);
-    CCK(hip_check(c, hipGetLastError(), "classify upper"));
+    const LevelDesc L = c->glv[lv];
+    // bottom polygons: setBottomNECell, then setBottomDECell on a snapshot
+    hipLaunchKernelGGL(k_ne, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_pv,
+                       (const int32_t*)c->d_pe, (const double*)c->d_edge, L.cstart, L.cnum, lv);
+    hipLaunchKernelGGL(k_snap, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, (const int32_t*)c->g.c_type,
+                       L.cstart, L.cnum, snap);
+    hipLaunchKernelGGL(k_de_bottom, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd,
+                       (const double*)c->d_edge, L.cstart, L.cnum, lv, (const int32_t*)snap);
+    // deeper polygons: setDECell
+    hipLaunchKernelGGL(k_de_upper, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd,
+                       (const double*)c->d_edge, L.cstart, L.cnum, lv);
+    CCK(hip_check(c, hipGetLastError(), "classify level"));
     CCK(ie_fixpoint(c, lv, snap, &und1));
     und_snap += und1;
   }
-  hipLaunchKernelGGL(k_top, dim3(1), dim3(64), 0, c->stream, g, c->lv[0].cnum);
+  hipLaunchKernelGGL(k_top, dim3(np), dim3(64), 0, c->stream, g, pd);
   hipLaunchKernelGGL(k_clear_undef, dim3(1), dim3(1), 0, c->stream, c->d_st);
-  for (int l = 0; l < c->gnum; l++)
-    hipLaunchKernelGGL(k_count_undef, dim3(cblocks(c->lv[l].cnum)), dim3(kCBlock), 0, c->stream, g,
-                       c->lv[l].cstart, c->lv[l].cnum, c->d_st);
+  hipLaunchKernelGGL(k_count_undef, dim3(cblocks(c->tot_c)), dim3(kCBlock), 0, c->stream, g, 0, (int)c->tot_c,
+                     c->d_st);
   CCK(hip_check(c, hipGetLastError(), "classify top"));
-  CCK(hip_check(c, hipEventRecord(c->ev[2], c->stream), "event"));
   CCK(hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream),
                 "readback"));
   CCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
-  c->st.undefined_leaves_snapshot = (int64_t)und_snap;
-  c->st.undefined_leaves_final = (int64_t)c->h_st->undefined_leaves;
+  c->und_snap = (int64_t)und_snap;
+  c->und_final = (int64_t)c->h_st->undefined_leaves;
   return GAHHG_OK;
 }
 

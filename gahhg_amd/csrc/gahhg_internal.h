@@ -29,6 +29,29 @@ struct LevelDesc {
   int32_t vstart, vnum, bstart, bnum, cstart, cnum, rnum;
 };
 
+// One polygon of a batch, shared by host and device.  Cell, border and
+// vertex indices are global (all polygons share the arrays); each polygon
+// owns one contiguous segment per level, so its local numbering -- the
+// single-polygon numbering of the reference -- is the global index minus
+// the segment start plus the local start of that level.
+struct PolyDev {
+  int32_t e0, N;        // edges [e0, e0 + N)
+  int32_t cell0;        // first level-0 cell (42 per polygon)
+  int32_t refining;     // 1 while this polygon is refined at the current level
+  int32_t L;            // bottom level once refinement stopped
+  int32_t use_prev;     // dispatch locates edges from the level above the bottom
+  int32_t roff, R;      // refs of the bottom cells
+  int32_t bcs, bcn;     // bottom-level cell segment
+  int32_t cs, cn;       // current level: cell segment
+  int32_t bs, bn;       // current level: border segment
+  int32_t lbs;          // current level: local start of its borders
+  int32_t pbs, plbs;    // previous level: global / local border start
+  int32_t U, nv, nbc, nbb;  // refined cells, their corners, border children, their borders
+  int32_t nvb, nbb0, ncb;   // next level: first vertex, border, cell
+  int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
+  int32_t pad;
+};
+
 // Device-side counters and the level table, read back by one copy per level.
 struct DevState {
   int32_t U;        // crossed cells at the level being refined
@@ -65,6 +88,7 @@ struct DeviceGrid {
   int32_t* c_upcell = nullptr;
   int32_t* c_subcell = nullptr;
   int32_t* c_type = nullptr;
+  int32_t* c_poly = nullptr;  // polygon of each cell
   int64_t rcap = 0;
   int32_t* refs = nullptr;
 };
@@ -86,6 +110,7 @@ struct GridView {
   int32_t* c_upcell;
   int32_t* c_subcell;
   int32_t* c_type;
+  int32_t* c_poly;
   int32_t* refs;
 };
 
@@ -107,7 +132,6 @@ struct GridView {
 struct QueryLayout {
   int32_t bins = 0;
   int32_t* table = nullptr;
-  int32_t leaf_cstart = 0, leaf_cnum = 0;
   double* qc = nullptr;
   int32_t* hdr = nullptr;
   double* sv = nullptr;
@@ -142,11 +166,19 @@ struct gahhg_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
-  gahhg::Polygon poly;
+  std::vector<gahhg::Polygon> polys;  // a batch; a single polygon is a batch of one
+  int64_t n_edges = 0;                // all polygons
   bool have_poly = false;
-  double* d_pv = nullptr;
-  int32_t* d_pe = nullptr;
-  double* d_edge = nullptr;  // 9 per edge: a, b, arc_normal(a, b)
+  double* d_pv = nullptr;             // vertices of all polygons
+  int32_t* d_pe = nullptr;            // edge endpoints (global vertex indices)
+  int32_t* d_epoly = nullptr;         // polygon of each edge
+  double* d_edge = nullptr;           // 9 per edge: a, b, arc_normal(a, b)
+  gahhg::PolyDev* d_pd = nullptr;
+  std::vector<gahhg::PolyDev> pd;
+  std::vector<std::vector<gahhg::LevelDesc>> plv;  // per polygon: global segments per level
+  std::vector<float> pgamma;                        // per polygon density ratio
+  std::vector<gahhg::LevelDesc> glv;                // per level: global ranges over all polygons
+  int64_t tot_v = 0, tot_b = 0, tot_c = 0, tot_r = 0;
 
   gahhg::DeviceGrid g;
   gahhg::LevelDesc lv[gahhg::kMaxLevels];
@@ -160,15 +192,16 @@ struct gahhg_ctx {
   // scratch
   int64_t fcap = 0;
   int32_t *d_f0 = nullptr, *d_f1 = nullptr, *d_o0 = nullptr, *d_o1 = nullptr;
+  int32_t *d_f2 = nullptr, *d_f3 = nullptr, *d_o2 = nullptr, *d_o3 = nullptr;
   int64_t scap = 0;
   int32_t* d_scan = nullptr;
   int64_t ecap = 0;
   int32_t* d_ecnt = nullptr;  // per edge
-  int64_t* d_eoff = nullptr;  // unused placeholder for >2^31 refs
   int64_t r2cap = 0;
   int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr;
 
   gahhg::QueryLayout q;
+  int64_t und_snap = 0, und_final = 0;
   bool built = false;
   gahhg_stats st;
   double last_query_ms = 0.0;
@@ -189,7 +222,7 @@ int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, con
 int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
-int run_query(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels);
+int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 
 GridView view(gahhg_ctx* c);

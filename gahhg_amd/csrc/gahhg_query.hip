@@ -40,15 +40,18 @@ __device__ __forceinline__ bool ee_cross(V3 nq, V3 ne, V3 a, V3 b, V3 c, V3 p) {
 }
 
 // Straightforward query over the build layout.
-__global__ void __launch_bounds__(kQBlock) k_query_soa(GridView g, const double* __restrict__ edge,
-                                                       const double* __restrict__ xyz, int64_t n,
+__global__ void __launch_bounds__(kQBlock) k_query_soa(GridView g, const PolyDev* __restrict__ pd,
+                                                       const double* __restrict__ edge,
+                                                       const double* __restrict__ xyz,
+                                                       const int32_t* __restrict__ qpoly, int64_t n,
                                                        int32_t* __restrict__ labels) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const V3 p = load3(xyz + 3 * q);
+  const int cell0 = pd[qpoly ? qpoly[q] : 0].cell0;
   int cur = -1;
   for (int i = 0; i < kGrid0Cells; i++) {
-    cur = q_inside(g, p, i);
+    cur = q_inside(g, p, cell0 + i);
     if (cur != -1) break;
   }
   if (cur == -1) {
@@ -192,9 +195,15 @@ __device__ int cell_state_cap(const GridView& g, int cid, V3 u, double rho) {
   return all_in ? 1 : 2;
 }
 
-__global__ void k_build_table(GridView g, int B, int32_t* __restrict__ table) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 6LL * B * B) return;
+__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly, int B,
+                              int32_t* __restrict__ table_all) {
+  const int64_t per = 6LL * B * B;
+  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= per * npoly) return;
+  const int poly = (int)(tt / per);
+  const int64_t t = tt - per * poly;
+  int32_t* table = table_all + per * poly;
+  const int cell0 = pd[poly].cell0;
   const int f = (int)(t / ((int64_t)B * B));
   const int i = (int)((t / B) % B);
   const int j = (int)(t % B);
@@ -212,9 +221,9 @@ __global__ void k_build_table(GridView g, int B, int32_t* __restrict__ table) {
   // first-match over level 0
   int cur = -1;
   for (int c = 0; c < kGrid0Cells; c++) {
-    const int s = cell_state_cap(g, c, cen, rho);
+    const int s = cell_state_cap(g, cell0 + c, cen, rho);
     if (s == 0) continue;
-    if (s == 1) cur = c;
+    if (s == 1) cur = cell0 + c;
     break;  // certain first match, or undecided: stop either way
   }
   if (cur < 0) {
@@ -257,12 +266,17 @@ __device__ __forceinline__ bool chained(const int32_t* pe, int e, int e2) { retu
 
 // Records of a bottom cell's edge stream: its refs plus one closing record
 // per chain.
-__global__ void k_leaf_count(GridView g, const int32_t* __restrict__ pe, int cstart, int cnum,
-                             int32_t* __restrict__ cnt) {
+__device__ __forceinline__ bool is_bottom(const GridView& g, const PolyDev* pd, int cid) {
+  const PolyDev& P = pd[g.c_poly[cid]];
+  return cid >= P.bcs && cid < P.bcs + P.bcn;
+}
+
+__global__ void k_leaf_count(GridView g, const PolyDev* __restrict__ pd, const int32_t* __restrict__ pe,
+                             int cnum, int32_t* __restrict__ cnt) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
-  const int cid = cstart + t;
-  const int rn = g.c_rnum[cid];
+  const int cid = t;
+  const int rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
   int chains = 0;
   if (rn > 0) {
     const int32_t* r = g.refs + g.c_rstart[cid];
@@ -273,14 +287,14 @@ __global__ void k_leaf_count(GridView g, const int32_t* __restrict__ pe, int cst
 }
 
 // Centre and edge stream of every bottom cell with refs.
-__global__ void k_leaf_write(GridView g, const double* __restrict__ pv, const int32_t* __restrict__ pe,
-                             const double* __restrict__ edge, int cstart, int cnum, const int32_t* __restrict__ roff,
-                             double* __restrict__ qc, int32_t* __restrict__ hdr, double* __restrict__ sv,
-                             double* __restrict__ sn, uint8_t* __restrict__ sf) {
+__global__ void k_leaf_write(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ pv,
+                             const int32_t* __restrict__ pe, const double* __restrict__ edge, int cnum,
+                             const int32_t* __restrict__ roff, double* __restrict__ qc, int32_t* __restrict__ hdr,
+                             double* __restrict__ sv, double* __restrict__ sn, uint8_t* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
-  const int cid = cstart + t;
-  const int rn = g.c_rnum[cid];
+  const int cid = t;
+  const int rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
   int k = roff[t];
   hdr[2 * t] = k;
   if (rn <= 0) {
@@ -318,11 +332,12 @@ __global__ void k_leaf_write(GridView g, const double* __restrict__ pv, const in
 // resolved cooperatively, one at a time per wave: the 64 lanes split that
 // leaf's edge stream (coalesced loads), each lane tests one edge, and the
 // crossing count is the popcount of a ballot.
-__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __restrict__ table, int B,
+__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
+                                                   const int32_t* __restrict__ table, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
                                                    const double* __restrict__ sv, const double* __restrict__ sn,
-                                                   const uint8_t* __restrict__ sf, int leaf_cstart,
-                                                   const double* __restrict__ xyz, int64_t n,
+                                                   const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
+                                                   const int32_t* __restrict__ qpoly, int64_t n,
                                                    int32_t* __restrict__ labels) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -331,11 +346,12 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __
   int type = kUndefined;
   if (q < n) {
     p = load3(xyz + 3 * q);
+    const PolyDev& P = pd[qpoly ? qpoly[q] : 0];
     int cur = -1;
     bool done = false;
     if (B > 0) {
       const int bin = bin_of(p, B);
-      const int e = bin >= 0 ? table[bin] : -1;
+      const int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
       if (e <= -2) {
         labels[q] = -e - 2;
         done = true;
@@ -344,7 +360,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __
     }
     if (!done && cur < 0) {
       for (int i = 0; i < kGrid0Cells; i++) {
-        cur = q_inside(g, p, i);
+        cur = q_inside(g, p, P.cell0 + i);
         if (cur != -1) break;
       }
       if (cur == -1) {
@@ -384,7 +400,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __
         if (g.c_rnum[cur] == 0)
           labels[q] = type;
         else
-          leaf = cur - leaf_cstart;
+          leaf = cur;
       }
     }
   }
@@ -422,19 +438,19 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const int32_t* __
 
 int pack_query_layout(gahhg_ctx* c) {
   QueryLayout& q = c->q;
-  const LevelDesc LB = c->lv[c->gnum - 1];
-  q.leaf_cstart = LB.cstart;
-  q.leaf_cnum = LB.cnum;
-  q.bins = c->prm.query_bins;
-  const int64_t nc = std::max<int64_t>(LB.cnum, 1);
+  const int np = (int)c->polys.size();
+  // start tables: the requested resolution for one polygon; batches keep
+  // each polygon's table to at most 128 bins per face side (393 KB)
+  q.bins = np == 1 ? c->prm.query_bins : std::min(c->prm.query_bins, 128);
+  const int64_t nc = std::max<int64_t>(c->tot_c, 1);
   GridView g = view(c);
   const unsigned cb = (unsigned)((nc + 255) / 256);
-  // record counts -> offsets (flag scratch holds >= cnum entries)
-  hipLaunchKernelGGL(k_leaf_count, dim3(cb), dim3(256), 0, c->stream, g, (const int32_t*)c->d_pe, LB.cstart,
-                     LB.cnum, c->d_f0);
+  // record counts -> offsets (flag scratch holds >= tot_c entries)
+  hipLaunchKernelGGL(k_leaf_count, dim3(cb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0);
   int rc = hip_check(c, hipGetLastError(), "leaf count");
   if (rc) return rc;
-  rc = scan_i32(c, c->d_f0, c->d_o0, LB.cnum, nullptr, &c->d_st->nrec, 0);
+  rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
   if (rc) return rc;
   rc = hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream), "readback");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
@@ -445,33 +461,40 @@ int pack_query_layout(gahhg_ctx* c) {
       dalloc(c, (void**)&q.sf, q.nrec))
     return GAHHG_E_NOMEM;
   q.bytes = nc * 32 + q.nrec * 49;
-  hipLaunchKernelGGL(k_leaf_write, dim3(cb), dim3(256), 0, c->stream, g, (const double*)c->d_pv,
-                     (const int32_t*)c->d_pe, (const double*)c->d_edge, LB.cstart, LB.cnum,
+  hipLaunchKernelGGL(k_leaf_write, dim3(cb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const double*)c->d_pv, (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
                      (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf);
   rc = hip_check(c, hipGetLastError(), "leaf write");
   if (rc) return rc;
   if (q.bins > 0) {
-    const int64_t nb = 6LL * q.bins * q.bins;
+    const int64_t per = 6LL * q.bins * q.bins;
+    const int64_t nb = per * np;
     if (dalloc(c, (void**)&q.table, nb * 4)) return GAHHG_E_NOMEM;
     q.bytes += nb * 4;
-    hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g, q.bins,
-                       q.table);
+    for (int p = 0; p < np; p++) c->pd[p].tbl = (int32_t)(per * p);
+    if (nb > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
+    rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                     c->stream),
+                   "polygon upload");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
+                       (const PolyDev*)c->d_pd, np, q.bins, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
   }
   return rc;
 }
 
-int run_query(gahhg_ctx* c, const double* d_xyz, int64_t n, int32_t* d_labels) {
+int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {
   if (n <= 0) return GAHHG_OK;
   const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
   if (c->prm.query_layout == 1) {
     hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                       (const double*)c->d_edge, d_xyz, n, d_labels);
+                       (const PolyDev*)c->d_pd, (const double*)c->d_edge, d_xyz, d_poly, n, d_labels);
   } else {
     hipLaunchKernelGGL(k_query, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                       (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
+                       (const PolyDev*)c->d_pd, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
                        (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
-                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, c->q.leaf_cstart, d_xyz, n, d_labels);
+                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
 }

@@ -119,6 +119,24 @@ int gahhg_query_xyz(gahhg_ctx* ctx, const double* xyz, int64_t n, int32_t* label
  * and waits for it.  This is GPISP::test's timed region. */
 int gahhg_query_xyz_device(gahhg_ctx* ctx, const double* d_xyz, int64_t n, int32_t* d_labels);
 
+/* Batches (no counterpart in the reference, which handles one polygon per
+ * process): n_polys polygons whose rings are ring_off[poly_ring_off[k] ..
+ * poly_ring_off[k+1]] (each ring as in gahhg_set_polygon_lonlat).  All are
+ * built by gahhg_build with shared launches; each polygon's hierarchy and
+ * labels equal what a single-polygon context builds for it. */
+int gahhg_set_polygons_lonlat(gahhg_ctx* ctx, const double* lon, const double* lat, const int64_t* ring_off,
+                              const int32_t* poly_ring_off, int32_t n_polys);
+int gahhg_num_polygons(gahhg_ctx* ctx, int32_t* n);
+/* Queries against a batch: poly[i] selects the polygon of point i. */
+int gahhg_query_batch_xyz(gahhg_ctx* ctx, const double* xyz, const int32_t* poly, int64_t n, int32_t* labels);
+int gahhg_query_batch_xyz_device(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n,
+                                 int32_t* d_labels);
+/* Per-polygon statistics and grid export in that polygon's own numbering. */
+int gahhg_get_polygon_stats(gahhg_ctx* ctx, int32_t p, gahhg_stats* out);
+int gahhg_polygon_grid_counts(gahhg_ctx* ctx, int32_t p, int64_t* counts5);
+int gahhg_export_polygon_grid(gahhg_ctx* ctx, int32_t p, double* v, int32_t* cells, int32_t* borders,
+                              double* normals, int32_t* refs, int32_t* levels);
+
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */
 int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);

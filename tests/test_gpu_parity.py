@@ -239,3 +239,48 @@ def test_cli_matches_oracle(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
     assert "Depth of GAHHG: 6" in rep.read_text().split("-----\n")[-2] or "Depth of GAHHG: 6" in rep.read_text()
+
+
+def test_batch_matches_single_polygon_builds():
+    say("batch of reference + synthetic polygons")
+    from gahhg_amd import datasets as ds
+    items = []
+    # reference polygons re-read as lon/lat rings
+    for name in ["Square_72.txt", "Radiometer.txt", "NA0.txt", "Mars_300.txt", "Mars_001.txt", "Tennessee_Tennessee_00500.txt"]:
+        lon, lat, roff = [], [], [0]
+        with open(refpath(name)) as f:
+            holes = int(f.readline().split()[2])
+            for r in range(holes + 1):
+                cnt = int(f.readline().split()[1])
+                for _ in range(cnt):
+                    a, b = f.readline().split()[:2]
+                    lon.append(float(a))
+                    lat.append(float(b))
+                roff.append(len(lon))
+        items.append((np.array(lon), np.array(lat), np.array(roff)))
+    w = ds.star_1k(n_queries=10)
+    items.append((w["lon"], w["lat"], w["ring_off"]))
+    w2 = ds.fractal_100k(n_queries=10, n_outer=8000)
+    items.append((w2["lon"], w2["lat"], w2["ring_off"]))
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygons_lonlat(items).build()
+    assert g.n_polygons == len(items)
+    rng = np.random.default_rng(3)
+    nq = 400_000
+    qlon, qlat = ds.uniform_queries(rng, nq)
+    xyz = ol.query_lonlat_to_xyz(qlon, qlat)
+    poly = rng.integers(0, len(items), nq).astype(np.int32)
+    got = g.query_batch_xyz(xyz, poly)
+    for k, (lon, lat, roff) in enumerate(items):
+        o = ol.Oracle().set_lonlat(lon, lat, roff).build()
+        compare_grids(g.polygon_grid(k), o.grid(), o.levels())
+        ps = g.polygon_stats(k)
+        assert ps["depth"] == o.counts()["depth"] and ps["gpu_space_bytes"] == o.space_bytes()
+        sel = poly == k
+        assert np.count_nonzero(got[sel] != o.query_xyz(xyz[sel])) == 0, k
+    # the plain query path on the same batch
+    gp = gahhg_amd.GPISP(device=0, query_layout=1, query_bins=0)
+    gp.set_polygons_lonlat(items).build()
+    assert (gp.query_batch_xyz(xyz, poly) == got).all()
+    g.close()
+    gp.close()
