@@ -801,10 +801,12 @@ static int ensure_refs(gahhg_ctx* c, int64_t R) {
   dfree(c->d_rcell);
   dfree(c->d_rflag);
   dfree(c->d_redge);
+  dfree(c->d_rscan);
   dfree(c->g.refs);
-  c->d_rcell = c->d_rflag = c->d_redge = c->g.refs = nullptr;
+  c->d_rcell = c->d_rflag = c->d_redge = c->d_rscan = c->g.refs = nullptr;
   if (dalloc(c, (void**)&c->d_rcell, nc * 4) || dalloc(c, (void**)&c->d_rflag, nc * 4) ||
-      dalloc(c, (void**)&c->d_redge, nc * 4) || dalloc(c, (void**)&c->g.refs, nc * 4))
+      dalloc(c, (void**)&c->d_redge, nc * 4) || dalloc(c, (void**)&c->d_rscan, nc * 4) ||
+      dalloc(c, (void**)&c->g.refs, nc * 4))
     return GAHHG_E_NOMEM;
   c->r2cap = nc;
   c->g.rcap = nc;
@@ -1078,9 +1080,10 @@ int build_hierarchy(gahhg_ctx* c) {
   hipLaunchKernelGGL(k_edge_heads, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_o0,
                      (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, c->d_rflag);
   LAUNCH_CK("edge heads");
-  CK(scan_i32(c, c->d_rflag, c->d_o1, R, nullptr, nullptr, 1));
+  // R refs: every buffer indexed by ref slot holds at least R + 1 entries
+  CK(scan_i32(c, c->d_rflag, c->d_rscan, R, nullptr, nullptr, 1));
   hipLaunchKernelGGL(k_ref_edges, dim3(blocks_for(R, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
-                     (const int32_t*)c->d_o1, (const int32_t*)c->g.c_poly, (const PolyDev*)c->d_pd,
+                     (const int32_t*)c->d_rscan, (const int32_t*)c->g.c_poly, (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, c->d_redge);
   LAUNCH_CK("ref edges");
   // counting sort into cell-major order

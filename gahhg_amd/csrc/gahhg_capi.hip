@@ -53,11 +53,11 @@ void free_grid(gahhg_ctx* c) {
 
 void free_scratch(gahhg_ctx* c) {
   void* ptrs[] = {c->d_f0, c->d_f1, c->d_o0, c->d_o1, c->d_f2, c->d_f3, c->d_o2, c->d_o3, c->d_scan,
-                  c->d_ecnt, c->d_rcell, c->d_rflag, c->d_redge, c->d_loc_cur, c->d_loc_prev};
+                  c->d_ecnt, c->d_rcell, c->d_rflag, c->d_redge, c->d_rscan, c->d_loc_cur, c->d_loc_prev};
   for (void* p : ptrs) dfree(p);
   c->d_f0 = c->d_f1 = c->d_o0 = c->d_o1 = c->d_scan = c->d_ecnt = nullptr;
   c->d_f2 = c->d_f3 = c->d_o2 = c->d_o3 = nullptr;
-  c->d_rcell = c->d_rflag = c->d_redge = c->d_loc_cur = c->d_loc_prev = nullptr;
+  c->d_rcell = c->d_rflag = c->d_redge = c->d_rscan = c->d_loc_cur = c->d_loc_prev = nullptr;
   c->fcap = c->scap = c->ecap = c->r2cap = 0;
 }
 

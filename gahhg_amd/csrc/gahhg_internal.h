@@ -198,7 +198,7 @@ struct gahhg_ctx {
   int64_t ecap = 0;
   int32_t* d_ecnt = nullptr;  // per edge
   int64_t r2cap = 0;
-  int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr;
+  int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr, *d_rscan = nullptr;
 
   gahhg::QueryLayout q;
   int64_t und_snap = 0, und_final = 0;

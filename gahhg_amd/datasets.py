@@ -179,4 +179,65 @@ def fractal_1m(n_queries=100_000_000, seed=3, n_outer=1_000_000, n_hole=50_000):
     return dict(name="fractal_1m", lon=lon, lat=lat, ring_off=off, qlon=q[0], qlat=q[1])
 
 
-WORKLOADS = {"star_1k": star_1k, "fractal_100k": fractal_100k, "fractal_1m": fractal_1m}
+def cap_queries(rng, lon0, lat0, radius_deg, n):
+    """n points uniform in the spherical cap of the given radius."""
+    ct = rng.uniform(np.cos(radius_deg * DEG), 1.0, n)
+    az = rng.uniform(0.0, 2.0 * np.pi, n)
+    return destination(lon0, lat0, az, np.arccos(ct))
+
+
+def batch_polygons(n_poly=256, seed=4, n_min=16, n_max=200_000):
+    """n_poly polygons: vertex counts log-uniform in [n_min, n_max], centres
+    uniform on the sphere, radii U(1, 25) deg, 0-4 star holes of
+    max(16, n/10) vertices at 0.4 R.  Rings of >= 1024 vertices are fractal
+    (64-gon base), smaller ones star-shaped."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    for k in range(n_poly):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        lon0 = float(rng.uniform(-180.0, 180.0))
+        lat0 = float(np.degrees(np.arcsin(rng.uniform(-1.0, 1.0))))
+        R = float(rng.uniform(1.0, 25.0))
+        if n >= 1024:
+            outer = fractal_ring(rng, lon0, lat0, n, R)
+        else:
+            outer = star_ring(rng, lon0, lat0, n, R)
+        h = int(rng.integers(0, 5))
+        holes = []
+        for j in range(h):
+            hl, ha = destination(lon0, lat0, 2.0 * np.pi * j / max(h, 1), 0.4 * R * DEG)
+            holes.append(star_ring(rng, float(hl), float(ha), max(16, n // 10), 0.12 * R))
+        lon, lat, off = assemble([outer] + holes)
+        polys.append(dict(lon=lon, lat=lat, ring_off=off, centre=(lon0, lat0), radius=R))
+    return polys
+
+
+def batch_256(n_queries=64_000_000, seed=4, n_poly=256):
+    """The batch plus queries: Zipf(1.1) polygon choice over a random rank
+    order, each point uniform in its polygon's cap of radius 1.3 R."""
+    polys = batch_polygons(n_poly, seed)
+    rng = np.random.default_rng(seed + 1000)
+    rank = rng.permutation(n_poly)
+    w = 1.0 / np.arange(1, n_poly + 1) ** 1.1
+    w /= w.sum()
+    cdf = np.cumsum(w)
+    cdf[-1] = 1.0
+    pid = rank[np.searchsorted(cdf, rng.random(n_queries), side="right")].astype(np.int32)
+    qlon = np.empty(n_queries)
+    qlat = np.empty(n_queries)
+    order = np.argsort(pid, kind="stable")
+    counts = np.bincount(pid, minlength=n_poly)
+    start = 0
+    for k in range(n_poly):
+        cnt = counts[k]
+        if cnt:
+            c = polys[k]
+            lo, la = cap_queries(rng, c["centre"][0], c["centre"][1], 1.3 * c["radius"], cnt)
+            idx = order[start:start + cnt]
+            qlon[idx] = lo
+            qlat[idx] = la
+        start += cnt
+    return dict(name="batch_256", polys=polys, qlon=qlon, qlat=qlat, qpoly=pid)
+
+
+WORKLOADS = {"star_1k": star_1k, "fractal_100k": fractal_100k, "fractal_1m": fractal_1m, "batch_256": batch_256}

@@ -284,3 +284,21 @@ def test_batch_matches_single_polygon_builds():
     assert (gp.query_batch_xyz(xyz, poly) == got).all()
     g.close()
     gp.close()
+
+
+def test_refs_beyond_flag_scratch():
+    """Regression: a polygon whose ref count R exceeds the per-cell/per-edge
+    scratch (dense star hole, R ~ 4.9 N) must build and match the oracle."""
+    say("polygon 68 of the 256-polygon batch, alone (R > 4 N)")
+    from gahhg_amd import datasets as ds
+    p = ds.batch_polygons(n_poly=69)[68]
+    o = ol.Oracle().set_lonlat(p["lon"], p["lat"], p["ring_off"]).build()
+    assert o.counts()["R"] > 4 * o.n_edges
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygon_lonlat(p["lon"], p["lat"], p["ring_off"]).build()
+    compare_grids(g.grid(), o.grid(), o.levels())
+    rng = np.random.default_rng(2)
+    lon, lat = ds.cap_queries(rng, p["centre"][0], p["centre"][1], 1.3 * p["radius"], 300_000)
+    xyz = ol.query_lonlat_to_xyz(lon, lat)
+    assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
+    g.close()
