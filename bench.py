@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bins", type=int, default=1024)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-queries", type=int, default=64_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -174,10 +176,93 @@ def main():
                                "kind": "port",
                                "sample": f"CPU oracle, same polygon, {ns} of the {Q} queries on {nthreads} threads; "
                                          f"oracle build {cpu_build_s * 1e3:.0f} ms"}
-    print(json.dumps(out), flush=True)
     d_xyz.free()
     d_lab.free()
     g.close()
+    del xyz, labels
+    if not args.no_batch:
+        out["batch_256"] = bench_batch(args, oracle_lib)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args, oracle_lib):
+    """256 polygons (log-uniform 16..200k vertices), 64M Zipf-skewed queries:
+    one batched build + one query launch, against building and querying each
+    polygon on its own (the reference runs one polygon per process)."""
+    import gahhg_amd
+    from gahhg_amd import datasets
+
+    t0 = time.time()
+    w = datasets.batch_256(n_queries=args.batch_queries)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in w["polys"]]
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    qp = w["qpoly"]
+    Q = len(qp)
+    n_edges = int(sum(len(p[0]) - (len(p[2]) - 1) for p in items))
+    log(f"batch_256: {len(items)} polygons, {n_edges} edges, {Q} queries generated in {time.time() - t0:.1f}s")
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
+    g.set_polygons_lonlat(items)
+    bt = []
+    for i in range(4):
+        g.build()
+        if i:
+            bt.append(g.stats()["pre_ms"])
+    build_ms = float(np.median(bt))
+    dx = g.alloc(xyz.nbytes)
+    dp = g.alloc(4 * Q)
+    dl = g.alloc(4 * Q)
+    dx.upload(xyz)
+    dp.upload(qp)
+    g.query_batch_device(dx, dp, Q, dl)
+    ks = []
+    tq = time.perf_counter()
+    for _ in range(max(3, args.steps // 2)):
+        g.query_batch_device(dx, dp, Q, dl)
+        ks.append(g.stats()["query_ms"])
+    wall_ms = (time.perf_counter() - tq) * 1e3 / len(ks)
+    lab = dl.download(np.empty(Q, dtype=np.int32))
+    dx.free()
+    dp.free()
+    dl.free()
+    g.close()
+    # per-polygon runs: build each polygon alone, query its points (grouped)
+    order = np.argsort(qp, kind="stable")
+    cnt = np.bincount(qp, minlength=len(items))
+    gp = gahhg_amd.GPISP(device=0, query_layout=1, query_bins=0)
+    pb = pq = 0.0
+    start = 0
+    mism = 0
+    for k, it in enumerate(items):
+        gp.set_polygon_lonlat(*it).build()
+        pb += gp.stats()["pre_ms"]
+        idx = order[start:start + cnt[k]]
+        start += cnt[k]
+        if len(idx):
+            l2 = gp.query_xyz(xyz[idx])
+            pq += gp.stats()["query_ms"]
+            mism += int(np.count_nonzero(l2 != lab[idx]))
+    gp.close()
+    # oracle check on a sample of polygons' points
+    rng = np.random.default_rng(1)
+    omism = 0
+    checked = 0
+    for k in rng.choice(len(items), 24, replace=False):
+        idx = order[int(cnt[:k].sum()):int(cnt[:k + 1].sum())][:20000]
+        if not len(idx):
+            continue
+        o = oracle_lib.Oracle().set_lonlat(*items[k]).build()
+        omism += int(np.count_nonzero(o.query_xyz(np.ascontiguousarray(xyz[idx])) != lab[idx]))
+        checked += len(idx)
+    res = {"n_polygons": len(items), "n_edges": n_edges, "n_queries": Q,
+           "batched": {"build_ms": round(build_ms, 3), "query_kernel_ms": round(float(np.median(ks)), 3),
+                       "query_wall_ms": round(wall_ms, 3), "queries_per_s": round(Q / (float(np.median(ks)) * 1e-3), 1)},
+           "per_polygon_plain": {"build_ms_sum": round(pb, 3), "query_kernel_ms_sum": round(pq, 3),
+                                 "queries_per_s": round(Q / (pq * 1e-3), 1) if pq else None},
+           "build_ratio": round(build_ms / pb, 4) if pb else None,
+           "query_speedup": round(pq / float(np.median(ks)), 3) if pq else None,
+           "labels_equal_to_per_polygon": mism == 0, "oracle_sample": checked, "oracle_mismatches": omism}
+    log(f"batch_256: {res}")
+    return res
 
 
 if __name__ == "__main__":

@@ -416,6 +416,7 @@ __global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int 
   g.c_subcell[child] = -1;
   g.c_type[child] = kUndefined;
   g.c_poly[child] = p;
+  g.c_root[child] = g.c_root[pid];
   g.c_subcell[pid] = child;
 }
 
@@ -477,6 +478,7 @@ __global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, in
     g.c_upcell[co] = P;
     g.c_type[co] = kUndefined;
     g.c_poly[co] = p;
+    g.c_root[co] = g.c_root[P];
     g.b_subcell[ue] = co;
     if (kind == 6) g.b_subcell[g.b_partner[ue]] = co;
     co += 1;
@@ -709,6 +711,7 @@ GridView view(gahhg_ctx* c) {
   v.c_subcell = c->g.c_subcell;
   v.c_type = c->g.c_type;
   v.c_poly = c->g.c_poly;
+  v.c_root = c->g.c_root;
   v.refs = c->g.refs;
   return v;
 }
@@ -773,7 +776,8 @@ static int ensure_c(gahhg_ctx* c, int64_t used, int64_t need) {
   if (grow(c, &g.c_bstart, used, need, nc) || grow(c, &g.c_bnum, used, need, nc) ||
       grow(c, &g.c_rstart, used, need, nc) || grow(c, &g.c_rnum, used, need, nc) ||
       grow(c, &g.c_upcell, used, need, nc) || grow(c, &g.c_subcell, used, need, nc) ||
-      grow(c, &g.c_type, used, need, nc) || grow(c, &g.c_poly, used, need, nc))
+      grow(c, &g.c_type, used, need, nc) || grow(c, &g.c_poly, used, need, nc) ||
+      grow(c, &g.c_root, used, need, nc))
     return GAHHG_E_NOMEM;
   g.ccap = nc;
   return GAHHG_OK;
@@ -870,7 +874,7 @@ int build_hierarchy(gahhg_ctx* c) {
     std::vector<int32_t> cols[7], bcols[6];
     for (auto& x : cols) x.resize(C0);
     for (auto& x : bcols) x.resize(B0);
-    std::vector<int32_t> cpoly(C0);
+    std::vector<int32_t> cpoly(C0), croot(C0);
     for (int p = 0; p < np; p++) {
       const int vb = kGrid0Verts * p, bb = kGrid0Borders * p, cb = kGrid0Cells * p;
       std::copy(v0.begin(), v0.end(), hv.begin() + 3 * (int64_t)vb);
@@ -887,6 +891,7 @@ int build_hierarchy(gahhg_ctx* c) {
         cols[0][cb + i] = ci0[7 * i] + bb;
         for (int k = 1; k < 7; k++) cols[k][cb + i] = ci0[7 * i + k];
         cpoly[cb + i] = p;
+        croot[cb + i] = i;
       }
     }
     hipError_t s = hipMemcpy(c->g.v, hv.data(), hv.size() * 8, hipMemcpyHostToDevice);
@@ -896,6 +901,7 @@ int build_hierarchy(gahhg_ctx* c) {
     int32_t* cd[7] = {c->g.c_bstart, c->g.c_bnum, c->g.c_rstart, c->g.c_rnum, c->g.c_upcell, c->g.c_subcell, c->g.c_type};
     for (int k = 0; k < 7 && s == hipSuccess; k++) s = hipMemcpy(cd[k], cols[k].data(), C0 * 4, hipMemcpyHostToDevice);
     if (s == hipSuccess) s = hipMemcpy(c->g.c_poly, cpoly.data(), C0 * 4, hipMemcpyHostToDevice);
+    if (s == hipSuccess) s = hipMemcpy(c->g.c_root, croot.data(), C0 * 4, hipMemcpyHostToDevice);
     CK(hip_check(c, s, "grid0 upload"));
   }
 
@@ -918,6 +924,8 @@ int build_hierarchy(gahhg_ctx* c) {
     P.bn = kGrid0Borders;
     P.lbs = 0;
     P.tbl = -1;
+    P.depth = p < (int)c->depth_override.size() ? c->depth_override[p] : 0;
+    if (P.depth == 1) P.refining = 0;  // level 0 is the bottom; nothing is marked
     c->plv[p].push_back(LevelDesc{kGrid0Verts * p, kGrid0Verts, kGrid0Borders * p, kGrid0Borders, kGrid0Cells * p,
                                   kGrid0Cells, 0});
     e0 += P.N;
@@ -938,6 +946,16 @@ int build_hierarchy(gahhg_ctx* c) {
   for (;; lv++) {
     const LevelDesc GL = c->glv[lv];
     bool any = false;
+    for (const PolyDev& P : c->pd) any |= P.refining != 0;
+    // polygons with a fixed depth stop before marking their last level
+    // (the level loop of createAHHG with autoDepth off, :1983-1992)
+    for (PolyDev& P : c->pd)
+      if (P.refining && P.depth > 0 && lv == P.depth - 1) {
+        P.refining = 0;
+        P.L = lv;
+        P.use_prev = 0;
+      }
+    any = false;
     for (const PolyDev& P : c->pd) any |= P.refining != 0;
     if (!any) break;
     if (lv == maxd - 1) {
@@ -978,7 +996,7 @@ int build_hierarchy(gahhg_ctx* c) {
     for (int p = 0; p < np; p++) {
       PolyDev& P = c->pd[p];
       if (!P.refining) continue;
-      if (autod) {
+      if (autod && P.depth == 0) {
         float k = P.U * 1.0 / P.N;
         c->pgamma[p] = k;
         if (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01)) {
@@ -1034,8 +1052,8 @@ int build_hierarchy(gahhg_ctx* c) {
     }
     CK(pd_upload(c));
   }
-  if (maxd == 1)
-    for (int p = 0; p < np; p++) c->pgamma[p] = (float)(42 * 1.0 / c->pd[p].N);
+  for (int p = 0; p < np; p++)
+    if ((c->pd[p].depth == 0 && maxd == 1) || c->pd[p].depth == 1) c->pgamma[p] = (float)(42 * 1.0 / c->pd[p].N);
   c->gnum = (int32_t)c->glv.size();
 
   // bottom segments

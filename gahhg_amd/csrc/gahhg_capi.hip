@@ -36,7 +36,7 @@ int select_device(gahhg_ctx* c) { return hip_check(c, hipSetDevice(c->device), "
 void free_grid(gahhg_ctx* c) {
   DeviceGrid& g = c->g;
   void* ptrs[] = {g.v, g.b_parent, g.b_partner, g.b_v0, g.b_v1, g.b_subcell, g.b_id, g.b_n, g.c_bstart,
-                  g.c_bnum, g.c_rstart, g.c_rnum, g.c_upcell, g.c_subcell, g.c_type, g.c_poly, g.refs};
+                  g.c_bnum, g.c_rstart, g.c_rnum, g.c_upcell, g.c_subcell, g.c_type, g.c_poly, g.c_root, g.refs};
   for (void* p : ptrs) dfree(p);
   g = DeviceGrid();
   QueryLayout& q = c->q;

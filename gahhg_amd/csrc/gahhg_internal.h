@@ -49,7 +49,7 @@ struct PolyDev {
   int32_t U, nv, nbc, nbb;  // refined cells, their corners, border children, their borders
   int32_t nvb, nbb0, ncb;   // next level: first vertex, border, cell
   int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
-  int32_t pad;
+  int32_t depth;        // fixed depth for this polygon (0 = density rule)
 };
 
 // Device-side counters and the level table, read back by one copy per level.
@@ -89,6 +89,7 @@ struct DeviceGrid {
   int32_t* c_subcell = nullptr;
   int32_t* c_type = nullptr;
   int32_t* c_poly = nullptr;  // polygon of each cell
+  int32_t* c_root = nullptr;  // level-0 ancestor (0..41) of each cell
   int64_t rcap = 0;
   int32_t* refs = nullptr;
 };
@@ -111,6 +112,7 @@ struct GridView {
   int32_t* c_subcell;
   int32_t* c_type;
   int32_t* c_poly;
+  int32_t* c_root;
   int32_t* refs;
 };
 
@@ -177,6 +179,7 @@ struct gahhg_ctx {
   std::vector<gahhg::PolyDev> pd;
   std::vector<std::vector<gahhg::LevelDesc>> plv;  // per polygon: global segments per level
   std::vector<float> pgamma;                        // per polygon density ratio
+  std::vector<int32_t> depth_override;              // per polygon fixed depth (0 = rule)
   std::vector<gahhg::LevelDesc> glv;                // per level: global ranges over all polygons
   int64_t tot_v = 0, tot_b = 0, tot_c = 0, tot_r = 0;
 
