@@ -72,8 +72,15 @@ EXPORTED = [
     "gahhg_device_free", "gahhg_memcpy_h2d", "gahhg_memcpy_d2h", "gahhg_grid_counts",
     "gahhg_export_grid", "gahhg_kat_device", "gahhg_set_polygons_lonlat", "gahhg_num_polygons",
     "gahhg_query_batch_xyz", "gahhg_query_batch_xyz_device", "gahhg_get_polygon_stats",
-    "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid",
+    "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
+    "gahhg_trace_query_device", "gahhg_build_trace",
 ]
+
+# Query trace record fields (6 int32 per point) and build trace layout.
+TRACE_FIELDS = ("leaf", "root", "level", "start_level", "plane_tests", "records")
+TRACE_LEVELS = 32
+TOP_CELLS = 42
+FIXED_POINT = 2.0 ** 40
 
 _lib = None
 
@@ -126,6 +133,9 @@ def lib():
         "gahhg_get_polygon_stats": ([P, i32, ctypes.POINTER(Stats)], ctypes.c_int),
         "gahhg_polygon_grid_counts": ([P, i32, lp], ctypes.c_int),
         "gahhg_export_polygon_grid": ([P, i32, P, P, P, P, P, P], ctypes.c_int),
+        "gahhg_set_polygon_depths": ([P, ip, i32], ctypes.c_int),
+        "gahhg_trace_query_device": ([P, P, P, i64, P, P], ctypes.c_int),
+        "gahhg_build_trace": ([P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -316,6 +326,51 @@ class GPISP:
 
     def query_batch_device(self, d_xyz, d_poly, n, d_labels):
         self._check(self.L.gahhg_query_batch_xyz_device(self.h, d_xyz.ptr, d_poly.ptr, int(n), d_labels.ptr))
+
+    def set_polygon_depths(self, depths):
+        """Fixed depth per polygon (0 = the context's rule); call after
+        setting the polygons."""
+        d = np.ascontiguousarray(depths, dtype=np.int32)
+        self._check(self.L.gahhg_set_polygon_depths(self.h, d, d.size))
+        return self
+
+    def trace_device(self, d_xyz, d_poly, n, d_labels, d_trace):
+        """Traced query: labels as query_batch_device plus 6 int32 per point
+        (TRACE_FIELDS) in d_trace."""
+        self._check(self.L.gahhg_trace_query_device(self.h, d_xyz.ptr, d_poly.ptr if d_poly is not None else None,
+                                                    int(n), d_labels.ptr, d_trace.ptr))
+
+    def trace_xyz(self, xyz, poly=None):
+        """Host wrapper of trace_device: returns (labels, trace[n, 6])."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        dx, dl, dt = self.alloc(max(xyz.nbytes, 8)), self.alloc(max(4 * n, 4)), self.alloc(max(24 * n, 4))
+        dp = None
+        try:
+            dx.upload(xyz)
+            if poly is not None:
+                dp = self.alloc(max(4 * n, 4))
+                dp.upload(np.ascontiguousarray(poly, dtype=np.int32))
+            self.trace_device(dx, dp, n, dl, dt)
+            lab = dl.download(np.empty(n, dtype=np.int32))
+            tr = dt.download(np.empty((n, 6), dtype=np.int32))
+        finally:
+            for b in (dx, dl, dt, dp):
+                if b is not None:
+                    b.free()
+        return lab, tr
+
+    def build_trace(self):
+        """Build trace of the last build: cells[p, level, top cell, 4] =
+        (cells, refined cells, bottom cells with refs, refs) and
+        edges[p, top cell, 3] = (edges, sum arc length rad, sum turning rad)."""
+        np_ = self.n_polygons
+        cells = np.zeros((np_, TRACE_LEVELS, TOP_CELLS, 4), dtype=np.int32)
+        edges = np.zeros((np_, TOP_CELLS, 3), dtype=np.uint64)
+        self._check(self.L.gahhg_build_trace(self.h, cells.ctypes.data, edges.ctypes.data))
+        e = edges.astype(np.float64)
+        e[..., 1:] /= FIXED_POINT
+        return cells, e
 
     def polygon_stats(self, p):
         s = Stats()

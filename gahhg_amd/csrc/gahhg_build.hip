@@ -949,12 +949,15 @@ int build_hierarchy(gahhg_ctx* c) {
     for (const PolyDev& P : c->pd) any |= P.refining != 0;
     // polygons with a fixed depth stop before marking their last level
     // (the level loop of createAHHG with autoDepth off, :1983-1992)
+    bool stopped = false;
     for (PolyDev& P : c->pd)
       if (P.refining && P.depth > 0 && lv == P.depth - 1) {
         P.refining = 0;
         P.L = lv;
         P.use_prev = 0;
+        stopped = true;
       }
+    if (stopped) CK(pd_upload(c));  // the level's kernels read the device copy
     any = false;
     for (const PolyDev& P : c->pd) any |= P.refining != 0;
     if (!any) break;

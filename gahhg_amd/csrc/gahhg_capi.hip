@@ -63,6 +63,7 @@ void free_scratch(gahhg_ctx* c) {
 
 int upload_polygons(gahhg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
+  c->depth_override.clear();  // fixed depths belong to the previous polygons
   free_grid(c);
   free_scratch(c);
   void* ptrs[] = {c->d_pv, c->d_pe, c->d_edge, c->d_epoly, c->d_pd};
@@ -403,6 +404,36 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   c->st.inside = (int64_t)h_two[0];
   c->st.unlocated = (int64_t)h_two[1];
   c->st.outside = n - c->st.inside;
+  return GAHHG_OK;
+}
+
+int gahhg_trace_query_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n,
+                             int32_t* d_labels, int32_t* d_trace) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "trace before build");
+  if (n < 0 || (n > 0 && (!d_xyz || !d_labels || !d_trace))) return set_err(c, GAHHG_E_ARG, "trace: bad arguments");
+  if (!d_poly && c->polys.size() != 1)
+    return set_err(c, GAHHG_E_ARG, "trace: polygon ids required for a batch of %d", (int)c->polys.size());
+  RCK(select_device(c));
+  RCK(run_query_trace(c, d_xyz, d_poly, n, d_labels, d_trace));
+  return hip_check(c, hipStreamSynchronize(c->stream), "trace sync");
+}
+
+int gahhg_build_trace(gahhg_ctx* c, int32_t* cells, uint64_t* edges) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "build trace before build");
+  RCK(select_device(c));
+  return build_trace(c, cells, edges);
+}
+
+int gahhg_set_polygon_depths(gahhg_ctx* c, const int32_t* depths, int32_t n) {
+  RCK(ctx_ok(c));
+  if (n < 0 || (n > 0 && !depths)) return set_err(c, GAHHG_E_ARG, "depths: bad arguments");
+  for (int32_t i = 0; i < n; i++)
+    if (depths[i] < 0 || depths[i] > kMaxLevels - 1)
+      return set_err(c, GAHHG_E_ARG, "depths: polygon %d depth %d outside 0..%d", i, depths[i], kMaxLevels - 1);
+  c->depth_override.assign(depths, depths + n);
+  c->built = false;
   return GAHHG_OK;
 }
 

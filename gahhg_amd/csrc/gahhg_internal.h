@@ -52,6 +52,13 @@ struct PolyDev {
   int32_t depth;        // fixed depth for this polygon (0 = density rule)
 };
 
+// First global cell of every level (cstart[n] = total cells), passed to
+// kernels by value to find a cell's level.
+struct LevelTab {
+  int32_t n;
+  int32_t cstart[kMaxLevels + 1];
+};
+
 // Device-side counters and the level table, read back by one copy per level.
 struct DevState {
   int32_t U;        // crossed cells at the level being refined
@@ -227,6 +234,10 @@ int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
+int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                    int32_t* d_trace);
+LevelTab level_tab(const gahhg_ctx* c);
+int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges);
 
 GridView view(gahhg_ctx* c);
 int set_err(gahhg_ctx* c, int code, const char* fmt, ...);

@@ -8,6 +8,7 @@
 // (device_isEEInter, :351-378).  A point that reaches no leaf gets 0 (the
 // reference leaves that output unwritten).
 #include <float.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -28,6 +29,60 @@ __device__ __forceinline__ int q_inside(const GridView& g, V3 p, int cid) {
     if (r < -kEps) return -1;
   }
   return cid;
+}
+
+// q_inside that also counts the border planes it evaluates.
+__device__ __forceinline__ int q_inside_n(const GridView& g, V3 p, int cid, int& tests) {
+  const int bs = g.c_bstart[cid];
+  const int bn = g.c_bnum[cid];
+  for (int i = 0; i < bn; i++) {
+    const double* n = g.b_n + 3 * (int64_t)(bs + i);
+    double r = n[0] * p.x + n[1] * p.y + n[2] * p.z;
+    if (r < -kEps) {
+      tests += i + 1;
+      return -1;
+    }
+  }
+  tests += bn;
+  return cid;
+}
+
+// Plain first-match descent from level 0 to the leaf holding p (-1 = none).
+__device__ int locate_leaf(const GridView& g, V3 p, int cell0) {
+  int cur = -1;
+  for (int i = 0; i < kGrid0Cells; i++) {
+    cur = q_inside(g, p, cell0 + i);
+    if (cur != -1) break;
+  }
+  if (cur == -1) return -1;
+  int sc;
+  while ((sc = g.c_subcell[cur]) != -1) {
+    int r = q_inside(g, p, sc);
+    if (r != -1) {
+      cur = r;
+      continue;
+    }
+    const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+    bool found = false;
+    for (int i = 0; i < bn; i++) {
+      const int tc = g.b_subcell[bs + i];
+      if (tc == -1) continue;
+      r = q_inside(g, p, tc);
+      if (r != -1) {
+        cur = r;
+        found = true;
+        break;
+      }
+    }
+    if (!found) return -1;
+  }
+  return cur;
+}
+
+__device__ __forceinline__ int level_of(const LevelTab& lt, int cid) {
+  int l = 0;
+  while (l + 1 < lt.n && cid >= lt.cstart[l + 1]) l++;
+  return l;
 }
 
 // Crossing test of the arc c -> p against polygon edge (a, b) with plane
@@ -332,18 +387,25 @@ __global__ void k_leaf_write(GridView g, const PolyDev* __restrict__ pd, const d
 // resolved cooperatively, one at a time per wave: the 64 lanes split that
 // leaf's edge stream (coalesced loads), each lane tests one edge, and the
 // crossing count is the popcount of a ballot.
+// TRACE adds one record of 6 int32 per point: {leaf cell (-1 = none),
+// its top-level cell, its level, level of the table start cell (-1 = level-0
+// scan, -2 = resolved by the table), border planes evaluated, leaf edge
+// records scanned}; labels are unchanged.
+template <bool TRACE>
 __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
                                                    const int32_t* __restrict__ table, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
                                                    const double* __restrict__ sv, const double* __restrict__ sn,
                                                    const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
                                                    const int32_t* __restrict__ qpoly, int64_t n,
-                                                   int32_t* __restrict__ labels) {
+                                                   int32_t* __restrict__ labels, LevelTab lt,
+                                                   int32_t* __restrict__ trace) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   V3 p = v3(0.0, 0.0, 0.0);
   int leaf = -1;  // bottom-cell index still to resolve, or -1
   int type = kUndefined;
+  int tests = 0, start_level = -1;
   if (q < n) {
     p = load3(xyz + 3 * q);
     const PolyDev& P = pd[qpoly ? qpoly[q] : 0];
@@ -357,10 +419,11 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
         done = true;
       }
       cur = e;
+      if (TRACE) start_level = e <= -2 ? -2 : (e >= 0 ? level_of(lt, e) : -1);
     }
     if (!done && cur < 0) {
       for (int i = 0; i < kGrid0Cells; i++) {
-        cur = q_inside(g, p, P.cell0 + i);
+        cur = TRACE ? q_inside_n(g, p, P.cell0 + i, tests) : q_inside(g, p, P.cell0 + i);
         if (cur != -1) break;
       }
       if (cur == -1) {
@@ -371,7 +434,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
     if (!done) {
       int sc;
       while ((sc = g.c_subcell[cur]) != -1) {
-        int r = q_inside(g, p, sc);
+        int r = TRACE ? q_inside_n(g, p, sc, tests) : q_inside(g, p, sc);
         if (r != -1) {
           cur = r;
           continue;
@@ -381,7 +444,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
         for (int i = 0; i < bn; i++) {
           const int tc = g.b_subcell[bs + i];
           if (tc == -1) continue;
-          r = q_inside(g, p, tc);
+          r = TRACE ? q_inside_n(g, p, tc, tests) : q_inside(g, p, tc);
           if (r != -1) {
             cur = r;
             found = true;
@@ -402,6 +465,16 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
         else
           leaf = cur;
       }
+    }
+    if (TRACE) {
+      const int lc = locate_leaf(g, p, P.cell0);
+      int32_t* tr = trace + 6 * q;
+      tr[0] = lc;
+      tr[1] = lc >= 0 ? g.c_root[lc] : -1;
+      tr[2] = lc >= 0 ? level_of(lt, lc) : -1;
+      tr[3] = start_level;
+      tr[4] = tests;
+      tr[5] = leaf >= 0 ? hdr[2 * leaf + 1] : 0;
     }
   }
   // cooperative edge tests for this wave's boundary points
@@ -491,12 +564,35 @@ int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t 
     hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                        (const PolyDev*)c->d_pd, (const double*)c->d_edge, d_xyz, d_poly, n, d_labels);
   } else {
-    hipLaunchKernelGGL(k_query, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+    hipLaunchKernelGGL(k_query<false>, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                        (const PolyDev*)c->d_pd, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
                        (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
-                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels);
+                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels,
+                       level_tab(c), (int32_t*)nullptr);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
+}
+
+LevelTab level_tab(const gahhg_ctx* c) {
+  LevelTab lt;
+  memset(&lt, 0, sizeof(lt));
+  lt.n = (int32_t)std::min<size_t>(c->glv.size(), kMaxLevels);
+  for (int l = 0; l < lt.n; l++) lt.cstart[l] = c->glv[l].cstart;
+  lt.cstart[lt.n] = (int32_t)c->tot_c;
+  return lt;
+}
+
+int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                    int32_t* d_trace) {
+  if (n <= 0) return GAHHG_OK;
+  if (!c->q.hdr) return set_err(c, GAHHG_E_STATE, "trace: the accelerated query layout is not built");
+  const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
+  hipLaunchKernelGGL(k_query<true>, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                     (const PolyDev*)c->d_pd, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
+                     (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
+                     (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels, level_tab(c),
+                     d_trace);
+  return hip_check(c, hipGetLastError(), "traced query kernel");
 }
 
 __global__ void k_clear_census(DevState* st) {

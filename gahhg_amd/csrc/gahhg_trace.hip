@@ -1,0 +1,116 @@
+// gahhg_trace.hip -- build trace: per polygon, per top-level cell (the 42
+// cells of level 0) and per level, the cell counts that describe how the
+// hierarchy grew there, plus per top-level cell aggregates of the polygon's
+// edges.  No counterpart in the reference; these are the inputs of the
+// learned refinement model and of the query-cost traces.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gahhg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gahhg {
+
+namespace {
+
+__device__ __forceinline__ int level_of_cell(const LevelTab& lt, int cid) {
+  int l = 0;
+  while (l + 1 < lt.n && cid >= lt.cstart[l + 1]) l++;
+  return l;
+}
+
+// out[((p * kMaxLevels + l) * 42 + r) * 4 + k], k = {cells, refined cells,
+// bottom cells with refs, refs of those cells}.  LDS-free: one atomic per
+// counted quantity (a trace, not a hot path).
+__global__ void k_cell_trace(GridView g, const PolyDev* __restrict__ pd, LevelTab lt, int64_t C,
+                             int32_t* __restrict__ out) {
+  const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cid >= C) return;
+  const int p = g.c_poly[cid];
+  const int l = level_of_cell(lt, (int)cid);
+  const int r = g.c_root[cid];
+  int32_t* o = out + (((int64_t)p * kMaxLevels + l) * kGrid0Cells + r) * 4;
+  atomicAdd(o + 0, 1);
+  if (g.c_subcell[cid] != -1) atomicAdd(o + 1, 1);
+  const PolyDev& P = pd[p];
+  if (cid >= P.bcs && cid < P.bcs + P.bcn && g.c_rnum[cid] > 0) {
+    atomicAdd(o + 2, 1);
+    atomicAdd(o + 3, g.c_rnum[cid]);
+  }
+}
+
+// Edge aggregates per top-level cell of the edge's first endpoint (first
+// matching level-0 cell): out[(p * 42 + r) * 3 + k] = {edges, sum of arc
+// lengths, sum of turning angles to the next edge of the ring}, the sums in
+// fixed point (2^-40 rad) so that they do not depend on summation order.
+constexpr double kFix = 1099511627776.0;  // 2^40
+
+__global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ edge,
+                             const int32_t* __restrict__ epoly, const int32_t* __restrict__ pe, int64_t Nt,
+                             unsigned long long* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Nt) return;
+  const int p = epoly[e];
+  const PolyDev& P = pd[p];
+  const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
+  int root = -1;
+  for (int i = 0; i < kGrid0Cells && root < 0; i++) {
+    const int cid = P.cell0 + i;
+    const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+    bool in = true;
+    for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), p0));
+    if (in) root = i;
+  }
+  if (root < 0) return;
+  const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
+  double turn = 0.0;
+  if (e + 1 < Nt && epoly[e + 1] == p && pe[2 * e + 1] == pe[2 * (e + 1)]) {
+    const V3 n2 = load3(edge + 9 * (e + 1) + 6);
+    turn = acos(fmin(1.0, fmax(-1.0, dot(ne, n2))));
+  }
+  unsigned long long* o = out + ((int64_t)p * kGrid0Cells + root) * 3;
+  atomicAdd(o + 0, 1ULL);
+  atomicAdd(o + 1, (unsigned long long)(len * kFix));
+  atomicAdd(o + 2, (unsigned long long)(turn * kFix));
+}
+
+}  // namespace
+
+int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges) {
+  const int np = (int)c->polys.size();
+  const int64_t ncell = (int64_t)np * kMaxLevels * kGrid0Cells * 4;
+  const int64_t nedge = (int64_t)np * kGrid0Cells * 3;
+  int32_t* d_cells = nullptr;
+  unsigned long long* d_edges = nullptr;
+  int rc = dalloc(c, (void**)&d_cells, ncell * 4);
+  if (!rc) rc = dalloc(c, (void**)&d_edges, nedge * 8);
+  if (!rc) rc = fill_i32(c, d_cells, ncell, 0);
+  if (!rc) rc = fill_i32(c, (int32_t*)d_edges, nedge * 2, 0);
+  if (!rc) {
+    GridView g = view(c);
+    const int64_t C = c->tot_c;
+    hipLaunchKernelGGL(k_cell_trace, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, c->stream, g,
+                       (const PolyDev*)c->d_pd, level_tab(c), C, d_cells);
+    rc = hip_check(c, hipGetLastError(), "cell trace");
+    const int64_t Nt = c->n_edges;
+    if (!rc && Nt > 0) {
+      hipLaunchKernelGGL(k_edge_trace, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, c->stream, g,
+                         (const PolyDev*)c->d_pd, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                         (const int32_t*)c->d_pe, Nt, d_edges);
+      rc = hip_check(c, hipGetLastError(), "edge trace");
+    }
+  }
+  if (!rc && h_cells)
+    rc = hip_check(c, hipMemcpyAsync(h_cells, d_cells, ncell * 4, hipMemcpyDeviceToHost, c->stream), "trace D2H");
+  if (!rc && h_edges)
+    rc = hip_check(c, hipMemcpyAsync(h_edges, d_edges, nedge * 8, hipMemcpyDeviceToHost, c->stream), "trace D2H");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "trace sync");
+  dfree(d_cells);
+  dfree(d_edges);
+  return rc;
+}
+
+}  // namespace gahhg

@@ -137,6 +137,31 @@ int gahhg_polygon_grid_counts(gahhg_ctx* ctx, int32_t p, int64_t* counts5);
 int gahhg_export_polygon_grid(gahhg_ctx* ctx, int32_t p, double* v, int32_t* cells, int32_t* borders,
                               double* normals, int32_t* refs, int32_t* levels);
 
+/* Fixed depth per polygon: AHHG.exe's /maxdepth mode (test/testAHHG.bat:6,
+ * GPISPParam.maxDepth with autoDepth off, source/GPISPAHHG.cu:1976-2015)
+ * chosen per polygon of a batch.  depths[k] = levels for polygon k (capped
+ * at max_depth), 0 = the context's own rule; polygons k >= n use the rule.
+ * Call after setting the polygons (setting polygons clears it); takes
+ * effect at the next gahhg_build. */
+int gahhg_set_polygon_depths(gahhg_ctx* ctx, const int32_t* depths, int32_t n);
+
+/* Trace recorder (no counterpart in the reference).  Query trace: the
+ * accelerated query of gahhg_query_batch_xyz_device (same labels) that also
+ * writes 6 int32 per point to d_trace: {leaf cell of the first-match
+ * descent (global cell id, -1 = none), its top-level cell 0..41, its level,
+ * level of the start cell taken from the table (-1 = level-0 scan, -2 =
+ * resolved by the table alone), border planes evaluated, leaf edge records
+ * scanned}.  d_poly may be NULL for a single polygon. */
+int gahhg_trace_query_device(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n,
+                             int32_t* d_labels, int32_t* d_trace);
+/* Build trace of the last build.  cells[((p*32 + l)*42 + r)*4 + k] for
+ * polygon p, level l (0..31), top-level cell r: k = {cells, refined cells,
+ * bottom cells with refs, refs}.  edges[(p*42 + r)*3 + k] for edges whose
+ * first endpoint lies in top-level cell r: k = {edges, sum of arc lengths,
+ * sum of turning angles to the next edge of the ring}, sums in units of
+ * 2^-40 rad.  Either pointer may be NULL. */
+int gahhg_build_trace(gahhg_ctx* ctx, int32_t* cells, uint64_t* edges);
+
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */
 int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);

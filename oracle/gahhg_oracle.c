@@ -1228,27 +1228,35 @@ int og_build(og_ctx* c, int32_t max_depth, int32_t auto_depth) {
 /* ---------------------------------------------------------------------- */
 /* Query: kernel_PISP, source/GPISPAHHG.cu:1422-1504.                      */
 /* ---------------------------------------------------------------------- */
-static int32_t query_one(const og_ctx* c, v3 p, og_qcount* q) {
-    int64_t tests = 0;
+/* First-match descent to the leaf holding p (-1 = no leaf), counting the
+ * border planes evaluated. */
+static int locate_leaf(const og_ctx* c, v3 p, int64_t* tests) {
     int cur = -1;
     for (int i = 0; i < 42; i++) {
-        cur = inside_cell(c, p, i, &tests);
+        cur = inside_cell(c, p, i, tests);
         if (cur != -1) break;
     }
-    if (cur == -1) { q->plane_tests += tests; q->unlocated++; return 0; }
+    if (cur == -1) return -1;
     while (c->c[cur].subcell != -1) {
         int found = 0;
-        int ret = inside_cell(c, p, c->c[cur].subcell, &tests);
+        int ret = inside_cell(c, p, c->c[cur].subcell, tests);
         if (ret != -1) { cur = ret; continue; }
         const og_cell* cc = &c->c[cur];
         for (int i = 0; i < cc->bnum; i++) {
             int tc = c->b[cc->bstart + i].subcell;
             if (tc == -1) continue;
-            ret = inside_cell(c, p, tc, &tests);
+            ret = inside_cell(c, p, tc, tests);
             if (ret != -1) { cur = ret; found = 1; break; }
         }
-        if (!found) { q->plane_tests += tests; q->unlocated++; return 0; }
+        if (!found) return -1;
     }
+    return cur;
+}
+
+static int32_t query_one(const og_ctx* c, v3 p, og_qcount* q) {
+    int64_t tests = 0;
+    int cur = locate_leaf(c, p, &tests);
+    if (cur == -1) { q->plane_tests += tests; q->unlocated++; return 0; }
     q->plane_tests += tests;
     const og_cell* lc = &c->c[cur];
     if (lc->rnum == 0) return lc->type;
@@ -1283,6 +1291,16 @@ void og_query_xyz(og_ctx* c, const double* xyz, int64_t n, int32_t* labels, int3
     tot.plane_tests = pt; tot.boundary_hits = bh; tot.ref_tests = rt; tot.bnum_sum = bs;
     tot.unlocated = un; tot.inside = ins;
     if (counts) *counts = tot;
+}
+
+void og_query_leaf(og_ctx* c, const double* xyz, int64_t n, int32_t* leaf) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        int64_t t = 0;
+        leaf[i] = locate_leaf(c, mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]), &t);
+    }
 }
 
 /* ---------------------------------------------------------------------- */

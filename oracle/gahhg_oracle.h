@@ -107,6 +107,10 @@ void og_export(og_ctx* c, double* v, int32_t* cells, int32_t* borders_i,
 void og_query_xyz(og_ctx* c, const double* xyz, int64_t n, int32_t* labels,
                   int32_t nthreads, og_qcount* counts /*nullable*/);
 
+/* Leaf cell (index into the exported cells, -1 = unlocated) that the
+ * first-match descent of kernel_PISP reaches for each point. */
+void og_query_leaf(og_ctx* c, const double* xyz, int64_t n, int32_t* leaf);
+
 /* Exact FP64 primitives exposed for known-answer tests. */
 void og_kat_normalize(const double* in3, double* out3);
 void og_kat_cross(const double* a3, const double* b3, double* out3);

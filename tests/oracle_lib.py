@@ -75,6 +75,8 @@ def lib():
         L.og_export.argtypes = [P, dp, ip, ip, dp, ip]
         L.og_query_xyz.argtypes = [P, dp, ctypes.c_int64, ip, ctypes.c_int32,
                                    ctypes.POINTER(OgQCount)]
+        L.og_query_leaf.argtypes = [P, dp, ctypes.c_int64, ip]
+        L.og_query_leaf.restype = None
         L.og_kat_normalize.argtypes = [dp, dp]
         L.og_kat_cross.argtypes = [dp, dp, dp]
         L.og_kat_is_ee_inter.argtypes = [dp, dp, dp, dp]
@@ -211,6 +213,14 @@ class Oracle:
         if with_counts:
             return labels, dict((f, getattr(q, f)) for f, _ in OgQCount._fields_)
         return labels
+
+    def query_leaf(self, xyz):
+        """Leaf cell the first-match descent reaches for each point (-1 = none)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        n = xyz.size // 3
+        leaf = np.zeros(n, dtype=np.int32)
+        self.L.og_query_leaf(self.h, xyz, n, leaf)
+        return leaf
 
 
 def load_query_csv(path):

@@ -1,0 +1,131 @@
+"""Trace recorder and per-polygon fixed depths on the GPU.
+
+* The traced query returns the same labels as the untraced one, and its leaf
+  cell for every point is the leaf the oracle's first-match descent reaches.
+* Its top-level cell and level fields agree with the exported hierarchy.
+* The build trace's counts agree with the exported hierarchy.
+* Fixed depths chosen per polygon of a batch give, for each polygon, the
+  hierarchy and labels of the oracle built at that fixed depth.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gahhg_amd
+from gahhg_amd import datasets
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def roots_and_levels(cells, levels):
+    """Top-level ancestor and level of every cell of an exported grid."""
+    up = cells[:, 4].copy()
+    root = np.arange(len(cells))
+    for _ in range(40):
+        nxt = np.where(up[root] >= 0, up[root], root)
+        if np.array_equal(nxt, root):
+            break
+        root = nxt
+    lev = np.zeros(len(cells), dtype=np.int64)
+    for l, L in enumerate(levels):
+        lev[L[4]:L[4] + L[5]] = l
+    return root, lev
+
+
+def test_trace_leaf_matches_oracle():
+    say("trace: fractal 20k ring, 400k mixed queries")
+    w = datasets.fractal_100k(n_queries=400_000, n_outer=20_000)
+    g = gahhg_amd.GPISP(device=0, query_bins=256)
+    g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    lab, tr = g.trace_xyz(xyz)
+    assert (lab == g.query_xyz(xyz)).all()
+    assert np.count_nonzero(lab != o.query_xyz(xyz)) == 0
+    leaf = o.query_leaf(xyz)
+    assert np.count_nonzero(tr[:, 0] != leaf) == 0
+    gg = g.grid()
+    root, lev = roots_and_levels(gg["cells"], gg["levels"])
+    ok = leaf >= 0
+    assert (tr[ok, 1] == root[leaf[ok]]).all()
+    assert (tr[ok, 2] == lev[leaf[ok]]).all()
+    # table-resolved points end in leaves without refs; points with leaf
+    # records end in leaves with refs
+    rnum = gg["cells"][:, 3]
+    assert (tr[tr[:, 3] == -2, 5] == 0).all()
+    assert (rnum[leaf[tr[:, 5] > 0]] > 0).all()
+    assert (tr[:, 4] >= 0).all()
+    # records = refs + one closing record per chain >= refs + 1
+    b = tr[:, 5] > 0
+    assert (tr[b, 5] >= rnum[leaf[b]] + 1).all()
+    g.close()
+
+
+def test_build_trace_counts():
+    say("build trace: 3-polygon batch")
+    ps = datasets.batch_polygons(n_poly=3, seed=9)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygons_lonlat(items).build()
+    cells, edges = g.build_trace()
+    for k in range(len(items)):
+        pg = g.polygon_grid(k)
+        c = pg["cells"]
+        root, lev = roots_and_levels(c, pg["levels"])
+        for l in range(pg["depth"]):
+            sel = lev == l
+            assert cells[k, l, :, 0].tolist() == np.bincount(root[sel], minlength=42).tolist()
+            ref = sel & (c[:, 5] != -1)
+            assert cells[k, l, :, 1].tolist() == np.bincount(root[ref], minlength=42).tolist()
+        assert cells[k, pg["depth"]:].sum() == 0
+        bot = lev == pg["depth"] - 1
+        assert cells[k, :, :, 3].sum() == c[bot, 3].sum() == len(pg["refs"])
+        n_edges = len(items[k][0]) - (len(items[k][2]) - 1)
+        assert edges[k, :, 0].sum() == n_edges
+        assert (edges[k, :, 1] >= 0).all() and (edges[k, :, 2] >= 0).all()
+    g.close()
+
+
+def test_per_polygon_fixed_depths():
+    say("fixed depths per polygon: batch of 4")
+    ps = datasets.batch_polygons(n_poly=4, seed=12)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    depths = [2, 0, 6, 9]
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygons_lonlat(items).set_polygon_depths(depths).build()
+    rng = np.random.default_rng(5)
+    for k, (lon, lat, roff) in enumerate(items):
+        o = ol.Oracle().set_lonlat(lon, lat, roff)
+        o.build(max_depth=depths[k], auto_depth=False) if depths[k] else o.build()
+        pg = g.polygon_grid(k)
+        assert pg["depth"] == o.counts()["depth"]
+        for key in ("cells", "borders", "refs"):
+            assert np.array_equal(pg[key], o.grid()[key]), (k, key)
+        assert np.array_equal(pg["v"].view(np.int64), o.grid()["v"].view(np.int64))
+        qlon, qlat = datasets.cap_queries(rng, ps[k]["centre"][0], ps[k]["centre"][1], 1.3 * ps[k]["radius"], 50_000)
+        xyz = ol.query_lonlat_to_xyz(qlon, qlat)
+        got = g.query_batch_xyz(xyz, np.full(len(qlon), k, dtype=np.int32))
+        assert np.count_nonzero(got != o.query_xyz(xyz)) == 0, k
+    # setting the polygons again clears the depths: the rule applies
+    g.set_polygons_lonlat(items).build()
+    o = ol.Oracle().set_lonlat(*items[0]).build()
+    assert g.polygon_stats(0)["depth"] == o.counts()["depth"]
+    g.close()
+
+
+def test_depth_argument_errors():
+    g = gahhg_amd.GPISP(device=0)
+    w = datasets.star_1k(n_queries=10)
+    g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
+    with pytest.raises(gahhg_amd.GahhgError):
+        g.set_polygon_depths([40])
+    with pytest.raises(gahhg_amd.GahhgError):
+        g.set_polygon_depths([-1])
+    g.close()
