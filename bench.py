@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--bins", type=int, default=1024)
+    ap.add_argument("--bins", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     args = ap.parse_args()
@@ -108,22 +108,32 @@ def main():
 
     for _ in range(args.warmup):
         g.query_device(d_xyz, Q, d_lab)
-    kernel_ms = []
+    kernel_ms, sort_ms, gpu_ms = [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         g.query_device(d_xyz, Q, d_lab)  # synchronises on return
-        kernel_ms.append(g.stats()["query_ms"])
+        s = g.stats()
+        kernel_ms.append(s["query_kernel_ms"])
+        sort_ms.append(s["query_sort_ms"])
+        gpu_ms.append(s["query_ms"])
     t_end = time.perf_counter()
     ms_per_step = (t_end - t_start) * 1e3 / args.steps
     value = Q / (ms_per_step * 1e-3)
     kms = float(np.mean(kernel_ms))
+    log(f"query: {ms_per_step:.3f} ms/step wall, gpu {np.mean(gpu_ms):.3f} ms = grouping {np.mean(sort_ms):.3f} "
+        f"+ k_query {kms:.3f}")
 
     # the plain path: the reference's kernel structure (linear level-0 scan,
     # descent, per-query leaf centre and per-ref edge planes) on the same grid
     plain = None
     if not args.no_plain:
         gp = gahhg_amd.GPISP(device=0, query_layout=1, query_bins=0)
-        gp.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+        gp.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
+        pb = []
+        for i in range(args.builds + 1):
+            gp.build()
+            if i:
+                pb.append(gp.stats()["pre_ms"])
         dl2 = gp.alloc(4 * Q)
         dx2 = gp.alloc(xyz.nbytes)
         dx2.upload(xyz)
@@ -134,7 +144,7 @@ def main():
             pl.append(gp.stats()["query_ms"])
         lab2 = dl2.download(np.empty(Q, dtype=np.int32))
         plain = {"queries_per_s": round(Q / (float(np.median(pl)) * 1e-3), 1), "kernel_ms": round(float(np.median(pl)), 3),
-                 "labels_equal": bool((lab2 == labels).all())}
+                 "build_ms": round(float(np.median(pb)), 3), "labels_equal": bool((lab2 == labels).all())}
         log(f"plain path: {plain}")
         dx2.free()
         dl2.free()
@@ -161,14 +171,17 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "fractal_1m: one polygon, 1M-vertex fractal ring (r=40 deg, centre lat 60 deg) "
                                "+ 3 holes x 50k vertices; 100M FP64 queries (70% uniform, 30% near-edge)",
-                   "n_edges": int(st["n_edges"]), "n_queries": int(Q), "path": "single-polygon GPU, start-cell table + cooperative leaf tests",
+                   "n_edges": int(st["n_edges"]), "n_queries": int(Q),
+                   "path": "single-polygon GPU: start-cell table, points grouped by start bin, lane-group leaf tests",
                    "parallelism": "1 GPU"},
         "build_ms": round(pre_ms, 3), "create_ms": round(create_ms, 3), "classify_ms": round(classify_ms, 3),
         "depth": int(st["depth"]), "grid_bytes": int(st["gpu_space_bytes"]),
         "parity": {"sample": int(ns), "mismatches": mism},
         "query_bins": args.bins,
+        "query_gpu_ms": round(float(np.mean(gpu_ms)), 4), "query_grouping_ms": round(float(np.mean(sort_ms)), 4),
         "plain_path": plain,
         "speedup_vs_plain": round(value / plain["queries_per_s"], 3) if plain else None,
+        "build_ratio_vs_plain": round(pre_ms / plain["build_ms"], 3) if plain else None,
         "roofline": roof,
     }
     if not args.no_cpu:
@@ -216,9 +229,11 @@ def bench_batch(args, oracle_lib):
     g.query_batch_device(dx, dp, Q, dl)
     ks = []
     tq = time.perf_counter()
+    kk = []
     for _ in range(max(3, args.steps // 2)):
         g.query_batch_device(dx, dp, Q, dl)
         ks.append(g.stats()["query_ms"])
+        kk.append(g.stats()["query_kernel_ms"])
     wall_ms = (time.perf_counter() - tq) * 1e3 / len(ks)
     lab = dl.download(np.empty(Q, dtype=np.int32))
     dx.free()
@@ -254,7 +269,8 @@ def bench_batch(args, oracle_lib):
         omism += int(np.count_nonzero(o.query_xyz(np.ascontiguousarray(xyz[idx])) != lab[idx]))
         checked += len(idx)
     res = {"n_polygons": len(items), "n_edges": n_edges, "n_queries": Q,
-           "batched": {"build_ms": round(build_ms, 3), "query_kernel_ms": round(float(np.median(ks)), 3),
+           "batched": {"build_ms": round(build_ms, 3), "query_gpu_ms": round(float(np.median(ks)), 3),
+                       "query_kernel_ms": round(float(np.median(kk)), 3),
                        "query_wall_ms": round(wall_ms, 3), "queries_per_s": round(Q / (float(np.median(ks)) * 1e-3), 1)},
            "per_polygon_plain": {"build_ms_sum": round(pb, 3), "query_kernel_ms_sum": round(pq, 3),
                                  "queries_per_s": round(Q / (pq * 1e-3), 1) if pq else None},

@@ -40,7 +40,8 @@ class GahhgError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
-        "device", "verbose", "query_layout", "query_bins")]
+        "device", "verbose", "query_layout", "query_bins", "query_group", "query_sub",
+    "query_sub_budget", "query_sort")]
 
 
 _STAT_FIELDS = [
@@ -56,6 +57,7 @@ _STAT_FIELDS = [
     ("walk_partner_missing", ctypes.c_int64), ("walk_step_cap_hits", ctypes.c_int64),
     ("walk_unlocated", ctypes.c_int64), ("zero_ref_edges", ctypes.c_int64),
     ("undefined_leaves_snapshot", ctypes.c_int64), ("undefined_leaves_final", ctypes.c_int64),
+    ("query_kernel_ms", ctypes.c_double), ("query_sort_ms", ctypes.c_double),
 ]
 
 
@@ -216,11 +218,13 @@ class GPISP:
     """One polygon's hierarchy on one GPU (the reference's GPISP object)."""
 
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
-                 query_layout=0, query_bins=256):
+                 query_layout=0, query_bins=256, query_group=0, query_sub=1, query_sub_budget=4,
+                 query_sort=1):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
                              verbose=1 if verbose else 0, block_query=block_query,
-                             query_layout=query_layout, query_bins=query_bins)
+                             query_layout=query_layout, query_bins=query_bins, query_group=query_group,
+                             query_sub=query_sub, query_sub_budget=query_sub_budget, query_sort=query_sort)
         h = ctypes.c_void_p()
         rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
         if rc:

@@ -41,6 +41,8 @@ void free_grid(gahhg_ctx* c) {
   g = DeviceGrid();
   QueryLayout& q = c->q;
   dfree(q.table);
+  dfree(q.sub);
+  dfree(q.tmp);
   dfree(q.qc);
   dfree(q.hdr);
   dfree(q.sv);
@@ -236,6 +238,10 @@ int gahhg_params_default(gahhg_params* p) {
   p->verbose = 0;
   p->query_layout = 0;
   p->query_bins = 256;
+  p->query_group = 0;
+  p->query_sub = 1;
+  p->query_sub_budget = 4;
+  p->query_sort = 1;
   return GAHHG_OK;
 }
 
@@ -250,7 +256,10 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
     gahhg_params_default(&c->prm);
   if (c->prm.max_depth < 1 || c->prm.max_depth > kMaxLevels - 1 || c->prm.block_query < 64 ||
       c->prm.block_query > 1024 || (c->prm.block_query % 64) != 0 || c->prm.query_bins < 0 ||
-      c->prm.query_bins > 4096) {
+      c->prm.query_bins > 4096 || c->prm.query_group < 0 || c->prm.query_group > 64 ||
+      (c->prm.query_group & (c->prm.query_group - 1)) != 0 || c->prm.query_sub < 0 || c->prm.query_sub > 256 ||
+      (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
+      c->prm.query_sort > 2) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -281,6 +290,7 @@ void gahhg_destroy(gahhg_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_grid(c);
   free_scratch(c);
+  dfree(c->qsort.buf);
   dfree(c->d_pv);
   dfree(c->d_pe);
   dfree(c->d_edge);
@@ -391,9 +401,15 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   RCK(run_query(c, d_xyz, d_poly, n, d_labels));
   RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
   RCK(hip_check(c, hipEventSynchronize(c->ev[1]), "query sync"));
-  float ms = 0.f;
+  float ms = 0.f, kms = 0.f, sms = 0.f;
   (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+  if (n > 0) {
+    (void)hipEventElapsedTime(&kms, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&sms, c->ev[0], c->ev[2]);
+  }
   c->st.query_ms = ms;
+  c->st.query_kernel_ms = kms;
+  c->st.query_sort_ms = sms;
   c->st.n_queries = n;
   c->st.query_unit_s = n > 0 ? ms * 1e-3 / (double)n : 0.0;
   // label census (outside the timed region)

@@ -50,7 +50,13 @@ struct PolyDev {
   int32_t nvb, nbb0, ncb;   // next level: first vertex, border, cell
   int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
   int32_t depth;        // fixed depth for this polygon (0 = density rule)
+  int32_t tsub;         // sub-table side S of this polygon's start table (1 = none)
+  int32_t pad;
 };
+
+// Start-table entries at or below -kSubCode refer to a sub-table at offset
+// -entry - kSubCode of the sub-table array.
+constexpr int32_t kSubCode = 16;
 
 // First global cell of every level (cstart[n] = total cells), passed to
 // kernels by value to find a cell's level.
@@ -68,7 +74,7 @@ struct DevState {
   int32_t R;        // bottom-level refs
   int32_t changed;  // classification round changed something
   int32_t nrec;     // records of the query edge stream
-  int32_t pad;
+  int32_t nleaf;    // bottom cells with refs
   unsigned long long steps_mark, steps_dispatch, partner_missing, cap_hits, unlocated, bad_index;
   unsigned long long zero_ref_edges, undefined_leaves;
   unsigned long long q_in, q_undef;  // label census of the last query
@@ -141,14 +147,30 @@ struct GridView {
 struct QueryLayout {
   int32_t bins = 0;
   int32_t* table = nullptr;
+  int32_t* sub = nullptr;      // sub-tables of the bins whose entry is not final
+  int64_t nsub = 0;            // sub-table entries
+  int32_t* tmp = nullptr;      // table-build scratch (grow-only)
+  int64_t tmpcap = 0;
   double* qc = nullptr;
   int32_t* hdr = nullptr;
   double* sv = nullptr;
   double* sn = nullptr;
   uint8_t* sf = nullptr;
   int64_t nrec = 0;
+  int64_t leaves = 0;  // bottom cells with refs
   int64_t bytes = 0;
 };
+
+// Grow-only buffers of the query sort: keys and values (two of each, n
+// entries) followed by the sort's temporary storage.
+struct QuerySort {
+  void* buf = nullptr;
+  int64_t cap = 0;
+  int64_t temp_bytes = 0;
+};
+
+// Queries below this count are not sorted (query_sort = 1).
+constexpr int64_t kSortMin = 1 << 20;
 
 struct Polygon {
   int32_t N = 0;
@@ -211,6 +233,7 @@ struct gahhg_ctx {
   int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr, *d_rscan = nullptr;
 
   gahhg::QueryLayout q;
+  gahhg::QuerySort qsort;
   int64_t und_snap = 0, und_final = 0;
   bool built = false;
   gahhg_stats st;
@@ -233,6 +256,8 @@ int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
+int query_group(const gahhg_ctx* c);
+int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, const int32_t** order);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);

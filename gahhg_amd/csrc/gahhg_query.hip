@@ -12,6 +12,8 @@
 
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "gahhg_internal.h"
 
 #pragma clang fp contract(off)
@@ -20,6 +22,10 @@ namespace gahhg {
 
 constexpr int kQBlock = 256;
 
+// The reference's inside test (every border plane n.p >= -eps, in border
+// order, stopping at the first failing plane: the early exit keeps the
+// number of gathered normals, which bounds the query's memory requests,
+// as small as possible).
 __device__ __forceinline__ int q_inside(const GridView& g, V3 p, int cid) {
   const int bs = g.c_bstart[cid];
   const int bn = g.c_bnum[cid];
@@ -45,6 +51,34 @@ __device__ __forceinline__ int q_inside_n(const GridView& g, V3 p, int cid, int&
   }
   tests += bn;
   return cid;
+}
+
+// First-match descent (centre child, then border children in border order)
+// from cell cur to the leaf holding p, -1 when no child holds it.
+template <bool TRACE>
+__device__ __forceinline__ int descend(const GridView& g, V3 p, int cur, int& tests) {
+  int sc;
+  while ((sc = g.c_subcell[cur]) != -1) {
+    int r = TRACE ? q_inside_n(g, p, sc, tests) : q_inside(g, p, sc);
+    if (r != -1) {
+      cur = r;
+      continue;
+    }
+    const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+    bool found = false;
+    for (int i = 0; i < bn; i++) {
+      const int tc = g.b_subcell[bs + i];
+      if (tc == -1) continue;
+      r = TRACE ? q_inside_n(g, p, tc, tests) : q_inside(g, p, tc);
+      if (r != -1) {
+        cur = r;
+        found = true;
+        break;
+      }
+    }
+    if (!found) return -1;
+  }
+  return cur;
 }
 
 // Plain first-match descent from level 0 to the leaf holding p (-1 = none).
@@ -200,7 +234,9 @@ __device__ __forceinline__ V3 face_point(int f, double u, double v) {
   }
 }
 
-__device__ __forceinline__ int bin_of(V3 p, int B) {
+// Bin of p at B bins per face side, and p's position (fx, fy) inside that
+// bin in [0, 1) for the sub-table lookup.
+__device__ __forceinline__ int bin_of(V3 p, int B, double& fx, double& fy) {
   const double ax = fabs(p.x), ay = fabs(p.y), az = fabs(p.z);
   int f;
   double m, a, b;
@@ -222,10 +258,14 @@ __device__ __forceinline__ int bin_of(V3 p, int B) {
   }
   if (!(m > 0.0)) return -1;
   const double inv = 1.0 / m;
-  int i = (int)floor((a * inv + 1.0) * 0.5 * B);
-  int j = (int)floor((b * inv + 1.0) * 0.5 * B);
+  const double x = (a * inv + 1.0) * 0.5 * B;
+  const double y = (b * inv + 1.0) * 0.5 * B;
+  int i = (int)floor(x);
+  int j = (int)floor(y);
   i = min(max(i, 0), B - 1);
   j = min(max(j, 0), B - 1);
+  fx = x - i;
+  fy = y - j;
   return (f * B + i) * B + j;
 }
 
@@ -250,22 +290,15 @@ __device__ int cell_state_cap(const GridView& g, int cid, V3 u, double rho) {
   return all_in ? 1 : 2;
 }
 
-__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly, int B,
-                              int32_t* __restrict__ table_all) {
-  const int64_t per = 6LL * B * B;
-  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tt >= per * npoly) return;
-  const int poly = (int)(tt / per);
-  const int64_t t = tt - per * poly;
-  int32_t* table = table_all + per * poly;
-  const int cell0 = pd[poly].cell0;
-  const int f = (int)(t / ((int64_t)B * B));
-  const int i = (int)((t / B) % B);
-  const int j = (int)(t % B);
-  const double u0 = -1.0 + 2.0 * i / B, u1 = -1.0 + 2.0 * (i + 1) / B;
-  const double w0 = -1.0 + 2.0 * j / B, w1 = -1.0 + 2.0 * (j + 1) / B;
-  const V3 cen = unit(face_point(f, 0.5 * (u0 + u1), 0.5 * (w0 + w1)));
-  double rho = 0.0;
+// Cap (centre, angular radius) around sub-bin (si, sj) of an S x S split of
+// bin (i, j) of face f at B bins per side; S = 1 is the bin itself.  The
+// radius carries a 1e-9 margin over the exact bin, far above the rounding
+// of the bin coordinates a query computes.
+__device__ void bin_cap(int f, int B, int i, int j, int S, int si, int sj, V3& cen, double& rho) {
+  const double u0 = -1.0 + 2.0 * (i + (double)si / S) / B, u1 = -1.0 + 2.0 * (i + (double)(si + 1) / S) / B;
+  const double w0 = -1.0 + 2.0 * (j + (double)sj / S) / B, w1 = -1.0 + 2.0 * (j + (double)(sj + 1) / S) / B;
+  cen = unit(face_point(f, 0.5 * (u0 + u1), 0.5 * (w0 + w1)));
+  rho = 0.0;
   const double us[2] = {u0, u1}, ws[2] = {w0, w1};
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) {
@@ -273,17 +306,23 @@ __global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int np
       rho = fmax(rho, acos(fmin(1.0, dot(cen, cr))));
     }
   rho += 1e-9;
-  // first-match over level 0
-  int cur = -1;
-  for (int c = 0; c < kGrid0Cells; c++) {
-    const int s = cell_state_cap(g, cell0 + c, cen, rho);
-    if (s == 0) continue;
-    if (s == 1) cur = cell0 + c;
-    break;  // certain first match, or undecided: stop either way
-  }
+}
+
+// Table entry for the points of a cap: the deepest cell that the
+// first-match descent certainly passes through for every point of the cap,
+// starting from cell `from` (-1: the level-0 scan of the polygon's cells
+// cell0 .. cell0 + 41).  Codes: >= 0 start cell, -1 start with the level-0
+// scan, -(2 + label) resolved leaf without refs.
+__device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, double rho) {
+  int cur = from;
   if (cur < 0) {
-    table[t] = -1;
-    return;
+    for (int c = 0; c < kGrid0Cells; c++) {
+      const int s = cell_state_cap(g, cell0 + c, cen, rho);
+      if (s == 0) continue;
+      if (s == 1) cur = cell0 + c;
+      break;  // certain first match, or undecided: stop either way
+    }
+    if (cur < 0) return -1;
   }
   // descend while the first-match child is certain
   for (;;) {
@@ -311,10 +350,93 @@ __global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int np
     if (stop || next < 0) break;
     cur = next;
   }
-  if (g.c_subcell[cur] == -1 && g.c_rnum[cur] == 0)
-    table[t] = -(2 + g.c_type[cur]);
-  else
-    table[t] = cur;
+  if (g.c_subcell[cur] == -1 && g.c_rnum[cur] == 0) return -(2 + g.c_type[cur]);
+  return cur;
+}
+
+// Level-1 entries: one per bin of every polygon's cube map.
+__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly, int B,
+                              int32_t* __restrict__ table_all) {
+  const int64_t per = 6LL * B * B;
+  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= per * npoly) return;
+  const int poly = (int)(tt / per);
+  const int64_t t = tt - per * poly;
+  const int f = (int)(t / ((int64_t)B * B));
+  const int i = (int)((t / B) % B);
+  const int j = (int)(t % B);
+  V3 cen;
+  double rho;
+  bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
+  table_all[tt] = cap_entry(g, pd[poly].cell0, -1, cen, rho);
+}
+
+// A bin gets a sub-table when its entry is not final: the level-0 scan or a
+// cell that still has children.
+__global__ void k_sub_flag(GridView g, const int32_t* __restrict__ table, int64_t n, int32_t* __restrict__ flag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int e = table[t];
+  flag[t] = (e == -1 || (e >= 0 && g.c_subcell[e] != -1)) ? 1 : 0;
+}
+
+// Flagged bins per polygon from the exclusive scan of the flags.
+__global__ void k_sub_counts(int npoly, int64_t per, const int32_t* __restrict__ flag,
+                             const int32_t* __restrict__ scan, int32_t* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoly) return;
+  const int64_t a = per * p, z = per * (p + 1) - 1;
+  out[p] = scan[z] + flag[z] - scan[a];
+}
+
+// Sub-table size of every bin (S^2 when flagged and the polygon has S > 1)
+// and whether it gets one.
+__global__ void k_sub_size(const PolyDev* __restrict__ pd, int64_t per, int64_t n, const int32_t* __restrict__ flag,
+                           int32_t* __restrict__ size, int32_t* __restrict__ has) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int S = pd[t / per].tsub;
+  const int h = flag[t] && S > 1;
+  size[t] = h ? S * S : 0;
+  has[t] = h;
+}
+
+__global__ void k_sub_list(int64_t n, const int32_t* __restrict__ has, const int32_t* __restrict__ hscan,
+                           int32_t* __restrict__ list) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (has[t]) list[hscan[t]] = (int32_t)t;
+}
+
+// One 64-lane block per bin with a sub-table: its S^2 entries, each
+// descending from the bin's own entry.
+__global__ void __launch_bounds__(64) k_sub_build(GridView g, const PolyDev* __restrict__ pd, int B, int64_t per,
+                                                  const int32_t* __restrict__ list,
+                                                  const int32_t* __restrict__ table,
+                                                  const int32_t* __restrict__ off, int32_t* __restrict__ sub) {
+  const int64_t tt = list[blockIdx.x];
+  const int poly = (int)(tt / per);
+  const int64_t t = tt - per * poly;
+  const int f = (int)(t / ((int64_t)B * B));
+  const int i = (int)((t / B) % B);
+  const int j = (int)(t % B);
+  const PolyDev& P = pd[poly];
+  const int S = P.tsub;
+  const int from = table[tt];
+  int32_t* o = sub + off[tt];
+  for (int s = threadIdx.x; s < S * S; s += 64) {
+    V3 cen;
+    double rho;
+    bin_cap(f, B, i, j, S, s / S, s % S, cen, rho);
+    o[s] = cap_entry(g, P.cell0, from, cen, rho);
+  }
+}
+
+__global__ void k_sub_link(int64_t n, const int32_t* __restrict__ has, const int32_t* __restrict__ off,
+                           int32_t* __restrict__ table) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (has[t]) table[t] = -(kSubCode + off[t]);
 }
 
 __device__ __forceinline__ bool chained(const int32_t* pe, int e, int e2) { return pe[2 * e + 1] == pe[2 * e2]; }
@@ -327,7 +449,7 @@ __device__ __forceinline__ bool is_bottom(const GridView& g, const PolyDev* pd, 
 }
 
 __global__ void k_leaf_count(GridView g, const PolyDev* __restrict__ pd, const int32_t* __restrict__ pe,
-                             int cnum, int32_t* __restrict__ cnt) {
+                             int cnum, int32_t* __restrict__ cnt, DevState* st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnum) return;
   const int cid = t;
@@ -339,6 +461,8 @@ __global__ void k_leaf_count(GridView g, const PolyDev* __restrict__ pd, const i
     for (int i = 0; i + 1 < rn; i++) chains += chained(pe, r[i], r[i + 1]) ? 0 : 1;
   }
   cnt[t] = rn > 0 ? rn + chains : 0;
+  const unsigned long long w = __ballot(rn > 0);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(&st->nleaf, (int32_t)__popcll(w));
 }
 
 // Centre and edge stream of every bottom cell with refs.
@@ -391,29 +515,40 @@ __global__ void k_leaf_write(GridView g, const PolyDev* __restrict__ pd, const d
 // its top-level cell, its level, level of the table start cell (-1 = level-0
 // scan, -2 = resolved by the table), border planes evaluated, leaf edge
 // records scanned}; labels are unchanged.
-template <bool TRACE>
+template <bool TRACE, int G>
 __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
-                                                   const int32_t* __restrict__ table, int B,
+                                                   const int32_t* __restrict__ table,
+                                                   const int32_t* __restrict__ sub, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
                                                    const double* __restrict__ sv, const double* __restrict__ sn,
                                                    const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
-                                                   const int32_t* __restrict__ qpoly, int64_t n,
+                                                   const int32_t* __restrict__ qpoly,
+                                                   const int32_t* __restrict__ order, int64_t n,
                                                    int32_t* __restrict__ labels, LevelTab lt,
                                                    int32_t* __restrict__ trace) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // thread t handles point order[t] (points grouped by polygon and start
+  // bin, so that a wave's gathers share cache lines), or point t
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = (order && t < n) ? (int64_t)order[t] : t;
   const int lane = threadIdx.x & 63;
   V3 p = v3(0.0, 0.0, 0.0);
   int leaf = -1;  // bottom-cell index still to resolve, or -1
   int type = kUndefined;
   int tests = 0, start_level = -1;
-  if (q < n) {
+  if (t < n) {
     p = load3(xyz + 3 * q);
     const PolyDev& P = pd[qpoly ? qpoly[q] : 0];
     int cur = -1;
     bool done = false;
     if (B > 0) {
-      const int bin = bin_of(p, B);
-      const int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
+      double fx = 0.0, fy = 0.0;
+      const int bin = bin_of(p, B, fx, fy);
+      int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
+      if (e <= -kSubCode) {
+        const int S = P.tsub;
+        const int si = min(max((int)(fx * S), 0), S - 1), sj = min(max((int)(fy * S), 0), S - 1);
+        e = sub[(int64_t)(-e - kSubCode) + si * S + sj];
+      }
       if (e <= -2) {
         labels[q] = -e - 2;
         done = true;
@@ -432,30 +567,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
       }
     }
     if (!done) {
-      int sc;
-      while ((sc = g.c_subcell[cur]) != -1) {
-        int r = TRACE ? q_inside_n(g, p, sc, tests) : q_inside(g, p, sc);
-        if (r != -1) {
-          cur = r;
-          continue;
-        }
-        const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
-        bool found = false;
-        for (int i = 0; i < bn; i++) {
-          const int tc = g.b_subcell[bs + i];
-          if (tc == -1) continue;
-          r = TRACE ? q_inside_n(g, p, tc, tests) : q_inside(g, p, tc);
-          if (r != -1) {
-            cur = r;
-            found = true;
-            break;
-          }
-        }
-        if (!found) {
-          cur = -1;
-          break;
-        }
-      }
+      cur = descend<TRACE>(g, p, cur, tests);
       if (cur == -1) {
         labels[q] = kUndefined;
       } else {
@@ -477,19 +589,25 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
       tr[5] = leaf >= 0 ? hdr[2 * leaf + 1] : 0;
     }
   }
-  // cooperative edge tests for this wave's boundary points
-  unsigned long long pending = __ballot(leaf >= 0);
-  while (pending) {
-    const int src = __ffsll((long long)pending) - 1;
-    pending &= pending - 1;
+  // Edge tests of this wave's boundary points, by groups of G lanes: each
+  // group resolves the points of its own lanes one at a time, its G lanes
+  // splitting the leaf's edge stream (coalesced loads), and the crossing
+  // count is the popcount of the group's part of a ballot.  Every condition
+  // below is uniform within a group, so a group's lanes stay active together.
+  const int gbase = lane & ~(G - 1);
+  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+  unsigned long long mine = __ballot(leaf >= 0) & gmask;
+  while (mine) {
+    const int src = __ffsll((long long)mine) - 1;
+    mine &= mine - 1;
     const int lt = __shfl(leaf, src, 64);
     const V3 pp = v3(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64));
     const V3 c = load3(qc + 3 * (int64_t)lt);
     const V3 nq = arc_normal(c, pp);
     const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
     int cnt = 0;
-    for (int base = 0; base < m - 1; base += 64) {
-      const int k = base + lane;
+    for (int base = 0; base < m - 1; base += G) {
+      const int k = base + (lane - gbase);
       bool cross = false;
       if (k < m - 1) {
         const int rk = r0 + k;
@@ -503,10 +621,82 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
           }
         }
       }
-      cnt += __popcll(__ballot(cross));
+      cnt += __popcll(__ballot(cross) & gmask);
     }
     if (lane == src) labels[q] = (cnt & 1) ? 3 - type : type;
   }
+}
+
+// Sub-tables for the bins whose level-1 entry is not final.  Polygon p gets
+// S_p x S_p sub-bins per such bin: the smallest power of two that makes a
+// sub-bin no wider than a cell of p's bottom level (level-0 cells span about
+// 0.55 rad, halving per level), capped by query_sub and by a budget of
+// query_sub_budget entries per edge of p (S_p = 1: no sub-tables).
+static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
+  QueryLayout& q = c->q;
+  const int np = (int)c->polys.size();
+  const int B = q.bins;
+  const int64_t need = 5 * (nb + 1) + np;
+  if (dalloc(c, (void**)&q.tmp, need * 4)) return GAHHG_E_NOMEM;
+  q.tmpcap = need;
+  int32_t* flag = q.tmp;
+  int32_t* scan = flag + (nb + 1);
+  int32_t* size = scan + (nb + 1);
+  int32_t* off = size + (nb + 1);
+  int32_t* list = off + (nb + 1);
+  int32_t* cnt = list + (nb + 1);
+  GridView g = view(c);
+  const unsigned nblk = (unsigned)((nb + 255) / 256);
+  hipLaunchKernelGGL(k_sub_flag, dim3(nblk), dim3(256), 0, c->stream, g, (const int32_t*)q.table, nb, flag);
+  int rc = hip_check(c, hipGetLastError(), "sub flags");
+  if (!rc) rc = scan_i32(c, flag, scan, nb, nullptr, nullptr, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sub_counts, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, per,
+                     (const int32_t*)flag, (const int32_t*)scan, cnt);
+  std::vector<int32_t> K((size_t)np);
+  rc = hip_check(c, hipGetLastError(), "sub counts");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(K.data(), cnt, np * 4, hipMemcpyDeviceToHost, c->stream), "sub D2H");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sub sync");
+  if (rc) return rc;
+  const double bin_angle = 1.5707963267948966 / B;
+  int64_t total = 0;
+  for (int p = 0; p < np; p++) {
+    PolyDev& P = c->pd[p];
+    const double cell = 0.55 / (double)(1LL << std::min(P.L, 40));
+    const double budget = (double)c->prm.query_sub_budget * P.N + 4096.0;
+    int S = 1;
+    while (S < c->prm.query_sub && bin_angle / S > cell) S *= 2;
+    while (S > 1 && (double)K[p] * S * S > budget) S /= 2;
+    P.tsub = S;
+    if (S > 1) total += (int64_t)K[p] * S * S;
+  }
+  if (total + kSubCode > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "sub-tables exceed int32 indexing");
+  rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                   c->stream),
+                 "polygon upload");
+  if (rc || total == 0) return rc;
+  hipLaunchKernelGGL(k_sub_size, dim3(nblk), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd, per, nb,
+                     (const int32_t*)flag, size, flag);
+  rc = hip_check(c, hipGetLastError(), "sub sizes");
+  if (!rc) rc = scan_i32(c, size, off, nb, nullptr, nullptr, 0);
+  if (!rc) rc = scan_i32(c, flag, scan, nb, nullptr, nullptr, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sub_list, dim3(nblk), dim3(256), 0, c->stream, nb, (const int32_t*)flag,
+                     (const int32_t*)scan, list);
+  rc = hip_check(c, hipGetLastError(), "sub list");
+  if (rc) return rc;
+  int64_t nbin = 0;
+  for (int p = 0; p < np; p++) nbin += c->pd[p].tsub > 1 ? K[p] : 0;
+  if (dalloc(c, (void**)&q.sub, total * 4)) return GAHHG_E_NOMEM;
+  q.nsub = total;
+  q.bytes += total * 4;
+  hipLaunchKernelGGL(k_sub_build, dim3((unsigned)nbin), dim3(64), 0, c->stream, g, (const PolyDev*)c->d_pd, B, per,
+                     (const int32_t*)list, (const int32_t*)q.table, (const int32_t*)off, q.sub);
+  rc = hip_check(c, hipGetLastError(), "sub build");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sub_link, dim3(nblk), dim3(256), 0, c->stream, nb, (const int32_t*)flag, (const int32_t*)off,
+                     q.table);
+  return hip_check(c, hipGetLastError(), "sub link");
 }
 
 int pack_query_layout(gahhg_ctx* c) {
@@ -514,13 +704,14 @@ int pack_query_layout(gahhg_ctx* c) {
   const int np = (int)c->polys.size();
   // start tables: the requested resolution for one polygon; batches keep
   // each polygon's table to at most 128 bins per face side (393 KB)
+  if (c->prm.query_layout == 1) return GAHHG_OK;  // the plain path queries the build arrays only
   q.bins = np == 1 ? c->prm.query_bins : std::min(c->prm.query_bins, 128);
   const int64_t nc = std::max<int64_t>(c->tot_c, 1);
   GridView g = view(c);
   const unsigned cb = (unsigned)((nc + 255) / 256);
   // record counts -> offsets (flag scratch holds >= tot_c entries)
   hipLaunchKernelGGL(k_leaf_count, dim3(cb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
-                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0);
+                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st);
   int rc = hip_check(c, hipGetLastError(), "leaf count");
   if (rc) return rc;
   rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
@@ -529,6 +720,7 @@ int pack_query_layout(gahhg_ctx* c) {
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   if (rc) return rc;
   q.nrec = std::max<int32_t>(c->h_st->nrec, 1);
+  q.leaves = c->h_st->nleaf;
   if (dalloc(c, (void**)&q.qc, nc * 24) || dalloc(c, (void**)&q.hdr, nc * 8) ||
       dalloc(c, (void**)&q.sv, q.nrec * 24) || dalloc(c, (void**)&q.sn, q.nrec * 24) ||
       dalloc(c, (void**)&q.sf, q.nrec))
@@ -544,7 +736,10 @@ int pack_query_layout(gahhg_ctx* c) {
     const int64_t nb = per * np;
     if (dalloc(c, (void**)&q.table, nb * 4)) return GAHHG_E_NOMEM;
     q.bytes += nb * 4;
-    for (int p = 0; p < np; p++) c->pd[p].tbl = (int32_t)(per * p);
+    for (int p = 0; p < np; p++) {
+      c->pd[p].tbl = (int32_t)(per * p);
+      c->pd[p].tsub = 1;
+    }
     if (nb > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
     rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
                                      c->stream),
@@ -553,22 +748,126 @@ int pack_query_layout(gahhg_ctx* c) {
     hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
                        (const PolyDev*)c->d_pd, np, q.bins, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
+    if (!rc && c->prm.query_sub > 1) rc = build_sub_tables(c, per, nb);
   }
   return rc;
+}
+// Sort key of a point: its polygon, then its start-table bin split 4 x 4
+// (a cube map of 256 bins per side when there is no table); value = index.
+__global__ void k_sort_key(const double* __restrict__ xyz, const int32_t* __restrict__ qpoly, int64_t n, int B,
+                           int bin_bits, uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const V3 p = load3(xyz + 3 * t);
+  double fx = 0.0, fy = 0.0;
+  const int bin = bin_of(p, B, fx, fy);
+  const uint32_t si = (uint32_t)min(max((int)(fx * 4.0), 0), 3), sj = (uint32_t)min(max((int)(fy * 4.0), 0), 3);
+  const uint32_t b = bin < 0 ? 0u : (((uint32_t)bin * 4u + si) * 4u + sj);
+  const uint32_t poly = qpoly ? (uint32_t)qpoly[t] : 0u;
+  key[t] = (poly << bin_bits) | b;
+  val[t] = (uint32_t)t;
+}
+
+static int ceil_log2(int64_t v) {
+  int b = 0;
+  while ((1LL << b) < v) b++;
+  return b;
+}
+
+// Groups the points by (polygon, start bin) with a device radix sort
+// (rocPRIM through hipCUB) over the key bits in use; *order receives the
+// point indices in key order.  Buffers are kept by the context and grow.
+int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, const int32_t** order) {
+  if (n > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "sort: more than 2^31 points");
+  const int B = c->q.table ? c->q.bins : 256;
+  const int bin_bits = ceil_log2(6LL * B * B * 16);
+  const int poly_bits = ceil_log2((int64_t)c->polys.size());
+  const int end_bit = std::min(32, bin_bits + poly_bits);
+  QuerySort& s = c->qsort;
+  size_t temp = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0,
+                                                    end_bit, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sort size");
+  if (s.cap < n || s.temp_bytes < (int64_t)temp) {
+    dfree(s.buf);
+    s.buf = nullptr;
+    s.cap = 0;
+    const int64_t cap = std::max<int64_t>(n, s.cap);
+    const int64_t tb = ((int64_t)temp + 255) / 256 * 256;
+    if (dalloc(c, (void**)&s.buf, 16 * cap + tb)) return GAHHG_E_NOMEM;
+    s.cap = cap;
+    s.temp_bytes = tb;
+  }
+  uint32_t* k0 = (uint32_t*)s.buf;
+  uint32_t* k1 = k0 + s.cap;
+  uint32_t* v0 = k1 + s.cap;
+  uint32_t* v1 = v0 + s.cap;
+  void* tmp = (void*)(v1 + s.cap);
+  hipLaunchKernelGGL(k_sort_key, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_xyz, d_poly, n, B,
+                     bin_bits, k0, v0);
+  int rc = hip_check(c, hipGetLastError(), "sort keys");
+  if (rc) return rc;
+  size_t tb = (size_t)s.temp_bytes;
+  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t*)k0, k1, (const uint32_t*)v0, v1, (int)n, 0,
+                                         end_bit, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sort");
+  *order = (const int32_t*)v1;
+  return GAHHG_OK;
+}
+
+// Lanes per edge-test group: the context's setting, or (0) the smallest
+// power of two >= half the mean records of a leaf with refs, in [4, 64].
+int query_group(const gahhg_ctx* c) {
+  if (c->prm.query_group > 0) return c->prm.query_group;
+  const double mean = c->q.leaves > 0 ? (double)c->q.nrec / (double)c->q.leaves : 64.0;
+  int G = 4;
+  while (G < 64 && G < 0.5 * mean) G *= 2;
+  return G;
+}
+
+template <bool TRACE>
+static void launch_query(gahhg_ctx* c, int G, const double* d_xyz, const int32_t* d_poly, const int32_t* d_order,
+                         int64_t n, int32_t* d_labels, int32_t* d_trace) {
+  const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
+  const QueryLayout& L = c->q;
+#define GAHHG_QUERY_LAUNCH(GG)                                                                                   \
+  hipLaunchKernelGGL((k_query<TRACE, GG>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),      \
+                     (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,                     \
+                     L.table ? L.bins : 0, (const double*)L.qc,                                                   \
+                     (const int32_t*)L.hdr, (const double*)L.sv, (const double*)L.sn, (const uint8_t*)L.sf,     \
+                     d_xyz, d_poly, d_order, n, d_labels, level_tab(c), d_trace)
+  switch (G) {
+    case 1: GAHHG_QUERY_LAUNCH(1); break;
+    case 2: GAHHG_QUERY_LAUNCH(2); break;
+    case 4: GAHHG_QUERY_LAUNCH(4); break;
+    case 8: GAHHG_QUERY_LAUNCH(8); break;
+    case 16: GAHHG_QUERY_LAUNCH(16); break;
+    case 32: GAHHG_QUERY_LAUNCH(32); break;
+    default: GAHHG_QUERY_LAUNCH(64); break;
+  }
+#undef GAHHG_QUERY_LAUNCH
 }
 
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {
   if (n <= 0) return GAHHG_OK;
-  const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
+  // events: ev[2] before the query kernel, ev[3] after it (the sort, when
+  // any, runs between the caller's ev[0] and ev[2])
   if (c->prm.query_layout == 1) {
+    const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
+    (void)hipEventRecord(c->ev[2], c->stream);
     hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                        (const PolyDev*)c->d_pd, (const double*)c->d_edge, d_xyz, d_poly, n, d_labels);
+    (void)hipEventRecord(c->ev[3], c->stream);
   } else {
-    hipLaunchKernelGGL(k_query<false>, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                       (const PolyDev*)c->d_pd, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
-                       (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
-                       (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels,
-                       level_tab(c), (int32_t*)nullptr);
+    const int32_t* order = nullptr;
+    if (c->prm.query_sort == 2 || (c->prm.query_sort == 1 && n >= kSortMin)) {
+      const int rc = sort_queries(c, d_xyz, d_poly, n, &order);
+      if (rc) return rc;
+    }
+    (void)hipEventRecord(c->ev[2], c->stream);
+    launch_query<false>(c, query_group(c), d_xyz, d_poly, order, n, d_labels, nullptr);
+    (void)hipEventRecord(c->ev[3], c->stream);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
 }
@@ -586,12 +885,7 @@ int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, in
                     int32_t* d_trace) {
   if (n <= 0) return GAHHG_OK;
   if (!c->q.hdr) return set_err(c, GAHHG_E_STATE, "trace: the accelerated query layout is not built");
-  const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
-  hipLaunchKernelGGL(k_query<true>, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                     (const PolyDev*)c->d_pd, (const int32_t*)c->q.table, c->q.table ? c->q.bins : 0,
-                     (const double*)c->q.qc, (const int32_t*)c->q.hdr, (const double*)c->q.sv,
-                     (const double*)c->q.sn, (const uint8_t*)c->q.sf, d_xyz, d_poly, n, d_labels, level_tab(c),
-                     d_trace);
+  launch_query<true>(c, query_group(c), d_xyz, d_poly, nullptr, n, d_labels, d_trace);
   return hip_check(c, hipGetLastError(), "traced query kernel");
 }
 

@@ -51,6 +51,15 @@ typedef struct {
   int32_t query_layout;    /* 0: table-accelerated query (default), 1: plain
                               descent from level 0 over the build arrays     */
   int32_t query_bins;      /* start-cell table bins per cube face (0 = none)  */
+  int32_t query_group;     /* lanes per edge-test group of the accelerated
+                              query: 1,2,4,...,64, or 0 = from the mean leaf
+                              edge-stream length                             */
+  int32_t query_sub;       /* largest sub-table side S of the start table
+                              (power of two <= 256; 0 or 1 = none, default) */
+  int32_t query_sub_budget;/* sub-table entries allowed per polygon edge     */
+  int32_t query_sort;      /* accelerated query: group points by polygon and
+                              start bin before the kernel (0 never, 1 from
+                              2^20 points, 2 always); labels are unchanged  */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by
@@ -83,6 +92,8 @@ typedef struct {
   int64_t zero_ref_edges;
   int64_t undefined_leaves_snapshot;
   int64_t undefined_leaves_final;
+  double query_kernel_ms; /* last query: the query kernel alone                */
+  double query_sort_ms;   /* last query: grouping of the points before it      */
 } gahhg_stats;
 
 /* Fills defaults (GPISPParam::GPISPParam, source/GPISPParam.h:66-77). */

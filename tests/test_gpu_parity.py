@@ -201,19 +201,47 @@ def test_report_format(dev, tmp_path):
     assert f"Test result: inside {st['inside']}, outside {10000 - st['inside']}" in txt
 
 
-@pytest.mark.parametrize("layout,bins", [(1, 0), (0, 0), (0, 16), (0, 1024)])
-def test_query_paths_agree(layout, bins):
-    say("query path", layout, bins)
+def bin_lattice(B, S, face_axis=2, sign=1.0, centre=(0.0, 0.0), span=40):
+    """Points on the bin and sub-bin boundary lines (u, v = -1 + 2 k / (B S))
+    of one cube face around a given (u, v) centre."""
+    k0 = np.floor((np.asarray(centre) + 1.0) * 0.5 * B * S).astype(int)
+    ks = np.arange(-span, span + 1)
+    uu = -1.0 + 2.0 * (k0[0] + ks) / (B * S)
+    vv = -1.0 + 2.0 * (k0[1] + ks) / (B * S)
+    U, V = np.meshgrid(uu, vv)
+    uv = np.stack([U.ravel(), V.ravel()], axis=1)
+    uv = uv[(np.abs(uv) <= 1.0).all(axis=1)]
+    p = np.zeros((len(uv), 3))
+    other = [a for a in range(3) if a != face_axis]
+    p[:, face_axis] = sign
+    p[:, other[0]] = uv[:, 0]
+    p[:, other[1]] = uv[:, 1]
+    return p / np.linalg.norm(p, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("layout,bins,group,sub,sort", [
+    (1, 0, 0, 64, 0), (0, 0, 0, 64, 0), (0, 16, 0, 64, 0), (0, 1024, 0, 64, 0), (0, 256, 0, 1, 0),
+    (0, 256, 0, 256, 0), (0, 256, 1, 64, 0), (0, 256, 2, 64, 0), (0, 256, 4, 64, 0), (0, 256, 8, 64, 0),
+    (0, 256, 16, 64, 0), (0, 256, 32, 64, 0), (0, 256, 64, 64, 0), (0, 256, 0, 64, 2), (0, 0, 8, 1, 2),
+    (0, 1024, 64, 8, 2)])
+def test_query_paths_agree(layout, bins, group, sub, sort):
+    say("query path", layout, bins, group, sub, sort)
     w = datasets.fractal_100k(n_queries=400_000, n_outer=30_000)
-    g = gahhg_amd.GPISP(device=0, query_layout=layout, query_bins=bins)
+    g = gahhg_amd.GPISP(device=0, query_layout=layout, query_bins=bins, query_group=group, query_sub=sub,
+                        query_sub_budget=64, query_sort=sort)
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
     o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
     xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
-    # exact cube-face and bin boundaries too
+    # exact cube-face, bin and sub-bin boundaries too, on a polygon vertex's
+    # face (x axis dominant for lon -40, lat 10)
     extra = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1],
                       [1, 1, 0], [1, -1, 1], [-1, 1, 1], [0.5, 1, 0]], dtype=np.float64)
     extra /= np.linalg.norm(extra, axis=1)[:, None]
-    xyz = np.concatenate([xyz, extra])
+    v = ol.query_lonlat_to_xyz(w["lon"][:1], w["lat"][:1])[0]
+    ax = int(np.argmax(np.abs(v)))
+    uvc = [v[a] / abs(v[ax]) for a in range(3) if a != ax]
+    lat = [bin_lattice(max(bins, 1), s, ax, np.sign(v[ax]), uvc) for s in (1, 2, 8, 64)]
+    xyz = np.concatenate([xyz, extra] + lat)
     assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
     g.close()
 
@@ -278,10 +306,14 @@ def test_batch_matches_single_polygon_builds():
         assert ps["depth"] == o.counts()["depth"] and ps["gpu_space_bytes"] == o.space_bytes()
         sel = poly == k
         assert np.count_nonzero(got[sel] != o.query_xyz(xyz[sel])) == 0, k
-    # the plain query path on the same batch
+    # the plain query path and the sorted accelerated path on the same batch
     gp = gahhg_amd.GPISP(device=0, query_layout=1, query_bins=0)
     gp.set_polygons_lonlat(items).build()
     assert (gp.query_batch_xyz(xyz, poly) == got).all()
+    gs = gahhg_amd.GPISP(device=0, query_sort=2)
+    gs.set_polygons_lonlat(items).build()
+    assert (gs.query_batch_xyz(xyz, poly) == got).all()
+    gs.close()
     g.close()
     gp.close()
 
