@@ -75,7 +75,8 @@ EXPORTED = [
     "gahhg_export_grid", "gahhg_kat_device", "gahhg_set_polygons_lonlat", "gahhg_num_polygons",
     "gahhg_query_batch_xyz", "gahhg_query_batch_xyz_device", "gahhg_get_polygon_stats",
     "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
-    "gahhg_trace_query_device", "gahhg_build_trace",
+    "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
+    "gahhg_mlp_adam",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -138,6 +139,9 @@ def lib():
         "gahhg_set_polygon_depths": ([P, ip, i32], ctypes.c_int),
         "gahhg_trace_query_device": ([P, P, P, i64, P, P], ctypes.c_int),
         "gahhg_build_trace": ([P, P, P], ctypes.c_int),
+        "gahhg_mlp_forward": ([P, P, i64, P, P], ctypes.c_int),
+        "gahhg_mlp_backward": ([P, P, i64, P, P, P], ctypes.c_int),
+        "gahhg_mlp_adam": ([P, P, P, P, P, i32, ctypes.c_float, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -291,6 +291,7 @@ void gahhg_destroy(gahhg_ctx* c) {
   free_grid(c);
   free_scratch(c);
   dfree(c->qsort.buf);
+  dfree(c->d_mlp_part);
   dfree(c->d_pv);
   dfree(c->d_pe);
   dfree(c->d_edge);
@@ -440,6 +441,34 @@ int gahhg_build_trace(gahhg_ctx* c, int32_t* cells, uint64_t* edges) {
   if (!c->built) return set_err(c, GAHHG_E_STATE, "build trace before build");
   RCK(select_device(c));
   return build_trace(c, cells, edges);
+}
+
+int gahhg_mlp_forward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d_w, float* d_y) {
+  RCK(ctx_ok(c));
+  if (rows < 0 || (rows > 0 && (!d_x || !d_w || !d_y))) return set_err(c, GAHHG_E_ARG, "mlp: bad arguments");
+  RCK(select_device(c));
+  RCK(mlp_forward(c, d_x, rows, d_w, d_y));
+  return hip_check(c, hipStreamSynchronize(c->stream), "mlp sync");
+}
+
+int gahhg_mlp_backward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d_w, const float* d_dy,
+                       float* d_grad) {
+  RCK(ctx_ok(c));
+  if (rows < 0 || !d_grad || (rows > 0 && (!d_x || !d_w || !d_dy)))
+    return set_err(c, GAHHG_E_ARG, "mlp: bad arguments");
+  RCK(select_device(c));
+  RCK(mlp_backward(c, d_x, rows, d_w, d_dy, d_grad));
+  return hip_check(c, hipStreamSynchronize(c->stream), "mlp sync");
+}
+
+int gahhg_mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m1, float* d_m2, int32_t step,
+                   float lr, uint16_t* d_w) {
+  RCK(ctx_ok(c));
+  if (!d_master || !d_grad || !d_m1 || !d_m2 || !d_w || step < 1 || !(lr > 0.0f))
+    return set_err(c, GAHHG_E_ARG, "mlp adam: bad arguments");
+  RCK(select_device(c));
+  RCK(mlp_adam(c, d_master, d_grad, d_m1, d_m2, step, lr, d_w));
+  return hip_check(c, hipStreamSynchronize(c->stream), "mlp sync");
 }
 
 int gahhg_set_polygon_depths(gahhg_ctx* c, const int32_t* depths, int32_t n) {

@@ -54,6 +54,12 @@ struct PolyDev {
   int32_t pad;
 };
 
+// Learned refinement model: parameter layout of the 12-64-64-1 perceptron
+// (W1 rows padded to 16 inputs).
+constexpr int kMlpIn = 12;
+constexpr int kMlpW1 = 0, kMlpB1 = 1024, kMlpW2 = 1088, kMlpB2 = 5184, kMlpW3 = 5248, kMlpB3 = 5312;
+constexpr int kMlpParams = 5313;
+
 // Start-table entries at or below -kSubCode refer to a sub-table at offset
 // -entry - kSubCode of the sub-table array.
 constexpr int32_t kSubCode = 16;
@@ -234,6 +240,8 @@ struct gahhg_ctx {
 
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
+  float* d_mlp_part = nullptr;       // per-wave partial gradients (grow-only)
+  int64_t mlp_part_cap = 0;
   int64_t und_snap = 0, und_final = 0;
   bool built = false;
   gahhg_stats st;
@@ -262,6 +270,11 @@ int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);
 LevelTab level_tab(const gahhg_ctx* c);
+int mlp_forward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d_w, float* d_y);
+int mlp_backward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d_w, const float* d_dy,
+                 float* d_grad);
+int mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m, float* d_v, int step, float lr,
+             uint16_t* d_w);
 int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges);
 
 GridView view(gahhg_ctx* c);

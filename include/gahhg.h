@@ -173,6 +173,25 @@ int gahhg_trace_query_device(gahhg_ctx* ctx, const double* d_xyz, const int32_t*
  * 2^-40 rad.  Either pointer may be NULL. */
 int gahhg_build_trace(gahhg_ctx* ctx, int32_t* cells, uint64_t* edges);
 
+/* Learned refinement model (no counterpart in the reference, whose only
+ * refinement rule is divGrid's density stop, source/GPISPAHHG.cu:1595-1605):
+ * a 12-64-64-1 perceptron, ReLU hidden layers, BF16 weights, FP32
+ * accumulation.  Parameter vector of GAHHG_MLP_PARAMS values, in order:
+ * W1[64][16] (input columns 12..15 are padding and stay zero), b1[64],
+ * W2[64][64], b2[64], w3[64], b3[1].  Device pointers; each call runs on the
+ * context stream and waits for it.
+ * forward:  y[r] = model(x[r*12 .. r*12+11]) for r < rows.
+ * backward: grad = d/dparams of sum_r dy[r] * y[r] (FP32, deterministic).
+ * adam:     one Adam step (beta 0.9 / 0.999, eps 1e-8) on the FP32 master
+ *           parameters with moments m1, m2; writes the BF16 weights w. */
+#define GAHHG_MLP_INPUTS 12
+#define GAHHG_MLP_PARAMS 5313
+int gahhg_mlp_forward(gahhg_ctx* ctx, const float* d_x, int64_t rows, const uint16_t* d_w, float* d_y);
+int gahhg_mlp_backward(gahhg_ctx* ctx, const float* d_x, int64_t rows, const uint16_t* d_w, const float* d_dy,
+                       float* d_grad);
+int gahhg_mlp_adam(gahhg_ctx* ctx, float* d_master, const float* d_grad, float* d_m1, float* d_m2, int32_t step,
+                   float lr, uint16_t* d_w);
+
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */
 int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);
