@@ -76,7 +76,7 @@ EXPORTED = [
     "gahhg_query_batch_xyz", "gahhg_query_batch_xyz_device", "gahhg_get_polygon_stats",
     "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
     "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
-    "gahhg_mlp_adam",
+    "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -142,6 +142,9 @@ def lib():
         "gahhg_mlp_forward": ([P, P, i64, P, P], ctypes.c_int),
         "gahhg_mlp_backward": ([P, P, i64, P, P, P], ctypes.c_int),
         "gahhg_mlp_adam": ([P, P, P, P, P, i32, ctypes.c_float, P], ctypes.c_int),
+        "gahhg_set_depth_model": ([P, P, P], ctypes.c_int),
+        "gahhg_set_query_hint": ([P, P, P, i64, ctypes.c_double], ctypes.c_int),
+        "gahhg_depth_features": ([P, i32, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -379,6 +382,33 @@ class GPISP:
         e = edges.astype(np.float64)
         e[..., 1:] /= FIXED_POINT
         return cells, e
+
+    def set_depth_model(self, params, cost):
+        """Learned refinement depth (params: the model's 5313 values; cost:
+        expected points, query-time scale, ns per edge, ns per next-level
+        cell, ns per level).  params None switches it off."""
+        if params is None:
+            self._check(self.L.gahhg_set_depth_model(self.h, None, None))
+            return self
+        pa = np.ascontiguousarray(params, dtype=np.float32)
+        co = np.ascontiguousarray(cost, dtype=np.float64)
+        assert pa.size == 5313 and co.size == 5
+        self._check(self.L.gahhg_set_depth_model(self.h, pa.ctypes.data, co.ctypes.data))
+        return self
+
+    def set_query_hint(self, xyz, poly=None, weight=1.0):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        pp = None if poly is None else np.ascontiguousarray(poly, dtype=np.int32)
+        self._check(self.L.gahhg_set_query_hint(self.h, xyz.ctypes.data if len(xyz) else None,
+                                                pp.ctypes.data if pp is not None else None, len(xyz),
+                                                float(weight)))
+        self._hint_keep = (xyz, pp)
+        return self
+
+    def depth_features(self, level):
+        x = np.zeros((self.n_polygons, TOP_CELLS, 2, 12), dtype=np.float32)
+        self._check(self.L.gahhg_depth_features(self.h, int(level), x.ctypes.data))
+        return x
 
     def polygon_stats(self, p):
         s = Stats()

@@ -936,6 +936,11 @@ int build_hierarchy(gahhg_ctx* c) {
   c->tot_b = B0;
   c->tot_c = C0;
   CK(pd_upload(c));
+  const bool learned = c->learn.on;
+  if (learned) {
+    CK(learn_prepare(c));
+    c->learn.h_dec.assign(np, 1);
+  }
   memset(c->h_st, 0, sizeof(DevState));
   CK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state init"));
   CK(fill_i32(c, c->d_loc_cur, Nt, 0));
@@ -993,13 +998,22 @@ int build_hierarchy(gahhg_ctx* c) {
                        (const int32_t*)c->d_f3, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1,
                        (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
     LAUNCH_CK("plan totals");
-    CK(pd_download(c));
+    if (learned) CK(learn_level(c, lv, GL.cstart, GL.cnum, c->learn.h_dec.data()));
+    CK(pd_download(c));  // also completes the decisions' copy
     // density rule per polygon (:1597-1604) and the next level's layout
     int64_t nv_all = 0, nb_all = 0, nc_all = 0;
     for (int p = 0; p < np; p++) {
       PolyDev& P = c->pd[p];
       if (!P.refining) continue;
-      if (autod && P.depth == 0) {
+      if (learned && P.depth == 0) {
+        c->pgamma[p] = (float)(P.U * 1.0 / P.N);
+        if (!c->learn.h_dec[p]) {
+          P.refining = 0;
+          P.L = lv;
+          P.use_prev = lv != 0 ? 1 : 0;
+          continue;
+        }
+      } else if (autod && P.depth == 0) {
         float k = P.U * 1.0 / P.N;
         c->pgamma[p] = k;
         if (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01)) {

@@ -292,6 +292,7 @@ void gahhg_destroy(gahhg_ctx* c) {
   free_scratch(c);
   dfree(c->qsort.buf);
   dfree(c->d_mlp_part);
+  learn_free(c);
   dfree(c->d_pv);
   dfree(c->d_pe);
   dfree(c->d_edge);
@@ -469,6 +470,66 @@ int gahhg_mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_
   RCK(select_device(c));
   RCK(mlp_adam(c, d_master, d_grad, d_m1, d_m2, step, lr, d_w));
   return hip_check(c, hipStreamSynchronize(c->stream), "mlp sync");
+}
+
+int gahhg_set_depth_model(gahhg_ctx* c, const float* params, const double* cost) {
+  RCK(ctx_ok(c));
+  RCK(select_device(c));
+  LearnState& L = c->learn;
+  if (!params) {
+    L.on = false;
+    c->built = false;
+    return GAHHG_OK;
+  }
+  if (!cost) return set_err(c, GAHHG_E_ARG, "depth model: cost constants required");
+  for (int i = 0; i < 5; i++)
+    if (!(cost[i] >= 0.0)) return set_err(c, GAHHG_E_ARG, "depth model: cost[%d] must be >= 0", i);
+  L.params.assign(params, params + kMlpParams);
+  std::vector<uint16_t> w((size_t)kMlpParams + 8, 0);
+  for (int k = 0; k < kMlpParams; k++) {
+    uint32_t u;
+    memcpy(&u, &params[k], 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return set_err(c, GAHHG_E_ARG, "depth model: non-finite parameter");
+    u += 0x7fffu + ((u >> 16) & 1u);
+    w[k] = (uint16_t)(u >> 16);
+  }
+  if (!L.d_weights) RCK(dalloc(c, (void**)&L.d_weights, (int64_t)w.size() * 2));
+  RCK(hip_check(c, hipMemcpy(L.d_weights, w.data(), w.size() * 2, hipMemcpyHostToDevice), "model upload"));
+  for (int i = 0; i < 5; i++) L.cost[i] = cost[i];
+  L.on = true;
+  c->built = false;
+  return GAHHG_OK;
+}
+
+int gahhg_set_query_hint(gahhg_ctx* c, const double* xyz, const int32_t* poly, int64_t n, double weight) {
+  RCK(ctx_ok(c));
+  if (n < 0 || (n > 0 && !xyz) || !(weight >= 0.0)) return set_err(c, GAHHG_E_ARG, "query hint: bad arguments");
+  RCK(select_device(c));
+  LearnState& L = c->learn;
+  dfree(L.d_hint_xyz);
+  dfree(L.d_hint_poly);
+  L.d_hint_xyz = nullptr;
+  L.d_hint_poly = nullptr;
+  L.hint_n = 0;
+  if (n == 0) return GAHHG_OK;
+  RCK(dalloc(c, (void**)&L.d_hint_xyz, n * 24));
+  RCK(hip_check(c, hipMemcpy(L.d_hint_xyz, xyz, n * 24, hipMemcpyHostToDevice), "hint upload"));
+  if (poly) {
+    RCK(dalloc(c, (void**)&L.d_hint_poly, n * 4));
+    RCK(hip_check(c, hipMemcpy(L.d_hint_poly, poly, n * 4, hipMemcpyHostToDevice), "hint upload"));
+  }
+  L.hint_n = n;
+  L.hint_weight = weight;
+  c->built = false;
+  return GAHHG_OK;
+}
+
+int gahhg_depth_features(gahhg_ctx* c, int32_t level, float* x) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "depth features before build");
+  if (!x || level < 0 || level >= kMaxLevels) return set_err(c, GAHHG_E_ARG, "depth features: bad arguments");
+  RCK(select_device(c));
+  return learn_features_after_build(c, level, x);
 }
 
 int gahhg_set_polygon_depths(gahhg_ctx* c, const int32_t* depths, int32_t n) {

@@ -54,6 +54,9 @@ struct PolyDev {
   int32_t pad;
 };
 
+// Fixed-point scale of the per top-level cell edge sums (2^-40 rad).
+constexpr double kFix = 1099511627776.0;
+
 // Learned refinement model: parameter layout of the 12-64-64-1 perceptron
 // (W1 rows padded to 16 inputs).
 constexpr int kMlpIn = 12;
@@ -167,6 +170,30 @@ struct QueryLayout {
   int64_t bytes = 0;
 };
 
+// Learned refinement depth: model, cost constants, hint sample and the
+// per (polygon, top-level cell) device arrays of the build.
+struct LearnState {
+  bool on = false;
+  std::vector<float> params;
+  // expected points (total), query_scale, ns per edge, ns per new cell, ns
+  // per level
+  double cost[5] = {0, 1, 0, 0, 0};
+  uint16_t* d_weights = nullptr;  // BF16 model weights
+  double* d_hint_xyz = nullptr;
+  int32_t* d_hint_poly = nullptr;
+  int64_t hint_n = 0;
+  double hint_weight = 1.0;
+  unsigned long long* d_estat = nullptr;  // np x 42 x 3 edge statistics
+  uint32_t* d_hist = nullptr;             // np x 42 hint points
+  float* d_w = nullptr;                   // np x 42 expected points
+  int32_t* d_stat = nullptr;              // np x 42 x 2 cells, crossed cells
+  float* d_x = nullptr;                   // np x 42 x 2 x 12 features
+  float* d_y = nullptr;                   // np x 42 x 2 predictions
+  int32_t* d_dec = nullptr;               // np decisions
+  int64_t cap = 0;                        // polygons the arrays hold
+  std::vector<int32_t> h_dec;
+};
+
 // Grow-only buffers of the query sort: keys and values (two of each, n
 // entries) followed by the sort's temporary storage.
 struct QuerySort {
@@ -240,6 +267,7 @@ struct gahhg_ctx {
 
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
+  gahhg::LearnState learn;
   float* d_mlp_part = nullptr;       // per-wave partial gradients (grow-only)
   int64_t mlp_part_cap = 0;
   int64_t und_snap = 0, und_final = 0;
@@ -276,6 +304,11 @@ int mlp_backward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d
 int mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m, float* d_v, int step, float lr,
              uint16_t* d_w);
 int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges);
+int edge_stats(gahhg_ctx* c, unsigned long long* d_out);
+int learn_prepare(gahhg_ctx* c);
+int learn_level(gahhg_ctx* c, int level, int lo, int n, int32_t* h_dec);
+int learn_features_after_build(gahhg_ctx* c, int level, float* h_x);
+void learn_free(gahhg_ctx* c);
 
 GridView view(gahhg_ctx* c);
 int set_err(gahhg_ctx* c, int code, const char* fmt, ...);

@@ -1,0 +1,250 @@
+// gahhg_learn.hip -- learned refinement depth inside the build.
+//
+// The reference refines a polygon while the crossed cells of a level stay
+// below about 0.09 N (divGrid, source/GPISPAHHG.cu:1595-1605).  With a model
+// set, a polygon without a fixed depth is refined instead while
+//
+//   sum_r w_r (2^y_r(stop) - 2^y_r(refine)) * query_scale
+//       >  ns_edge * N + ns_cell * (cells of the next level) + ns_level
+//
+// where, per top-level cell r of the polygon, w_r is the number of points
+// expected there (from a hint sample, else the expected total spread by
+// edge share) and y_r(k) is the model's log2 query time per point (ns) if
+// refinement stops at this level (k = 0) or after one more (k = 1).  The
+// features, the model and the decision run on the device after the level's
+// plan; the decisions come back with the level's one host synchronisation.
+// Hierarchies stay reference-structured (one bottom level per polygon), so
+// the oracle built at the chosen depth reproduces them exactly.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gahhg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gahhg {
+
+namespace {
+
+// Points per (polygon, top-level cell) of the hint sample; the top-level
+// cell is the first level-0 cell (index order) whose planes all pass, the
+// same test the query's level-0 scan applies (level-0 geometry is the same
+// for every polygon: polygon 0's cells are used).
+__global__ void k_hint_hist(GridView g, const double* __restrict__ xyz, const int32_t* __restrict__ poly,
+                            int64_t n, int npoly, uint32_t* __restrict__ hist) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = poly ? poly[t] : 0;
+  if (p < 0 || p >= npoly) return;
+  const V3 q = load3(xyz + 3 * t);
+  for (int i = 0; i < kGrid0Cells; i++) {
+    const int bs = g.c_bstart[i], bn = g.c_bnum[i];
+    bool in = true;
+    for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), q));
+    if (in) {
+      atomicAdd(hist + p * kGrid0Cells + i, 1u);
+      return;
+    }
+  }
+}
+
+// Expected points per (polygon, top-level cell): the hint histogram times
+// the weight per hint point, or the default spread by edge share.
+__global__ void k_weights(int npoly, const PolyDev* __restrict__ pd, const uint32_t* __restrict__ hist,
+                          double per_point, double q_default, const unsigned long long* __restrict__ estat,
+                          float* __restrict__ w) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npoly * kGrid0Cells) return;
+  const int p = t / kGrid0Cells;
+  if (hist)
+    w[t] = (float)(hist[t] * per_point);
+  else
+    w[t] = (float)(q_default * (double)estat[3 * t] / (double)max(pd[p].N, 1));
+}
+
+// Cells and crossed (build time: marked, rnum != 0) or refined (after the
+// build: subcell != -1) cells of level range [lo, lo + n) per (polygon,
+// top-level cell).  only_refining: count only polygons still refining.
+__global__ void k_level_stats(GridView g, const PolyDev* __restrict__ pd, int lo, int n, int marked,
+                              int only_refining, int32_t* __restrict__ st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int cid = lo + t;
+  const int p = g.c_poly[cid];
+  if (only_refining && !pd[p].refining) return;
+  int32_t* o = st + 2 * (p * kGrid0Cells + g.c_root[cid]);
+  atomicAdd(o, 1);
+  if (marked ? g.c_rnum[cid] != 0 : g.c_subcell[cid] != -1) atomicAdd(o + 1, 1);
+}
+
+// Features of rows (p, r, k), row = (p * 42 + r) * 2 + k.
+__global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
+                           const unsigned long long* __restrict__ estat, const float* __restrict__ w,
+                           float* __restrict__ X) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= npoly * kGrid0Cells * 2) return;
+  const int k = row & 1;
+  const int pr = row >> 1;
+  const int p = pr / kGrid0Cells;
+  const float Nr = (float)estat[3 * pr];
+  const float Ur = (float)st[2 * pr + 1];
+  const float Cr = (float)st[2 * pr];
+  const double len = Nr > 0.f ? (double)estat[3 * pr + 1] / kFix / Nr : 0.0;
+  const double turn = Nr > 0.f ? (double)estat[3 * pr + 2] / kFix / Nr : 0.0;
+  float wsum = 0.f;
+  for (int r = 0; r < kGrid0Cells; r++) wsum += w[p * kGrid0Cells + r];
+  const double cell = 0.55 / (double)(1LL << min(level, 60));
+  float* x = X + (int64_t)row * kMlpIn;
+  x[0] = level / 16.0f;
+  x[1] = (float)k;
+  x[2] = log2f(1.0f + Nr) / 20.0f;
+  x[3] = log2f(1.0f + Ur) / 20.0f;
+  x[4] = log2f(1.0f + Cr) / 20.0f;
+  x[5] = fminf(Ur / fmaxf(Nr, 1.0f), 4.0f);
+  x[6] = log2f(1.0f + Nr / fmaxf(Ur, 1.0f)) / 10.0f;
+  x[7] = (len > 0.0) ? (float)(log2(len / cell) / 10.0) : 0.0f;
+  x[8] = (float)(turn / M_PI);
+  x[9] = Ur / fmaxf(Cr, 1.0f);
+  x[10] = log2f(1.0f + (float)pd[p].N) / 20.0f;
+  x[11] = wsum > 0.f ? w[pr] / wsum : Nr / fmaxf((float)pd[p].N, 1.0f);
+}
+
+// Refine-or-stop per polygon still refining without a fixed depth.
+__global__ void k_decide(int npoly, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
+                         const float* __restrict__ Y, const float* __restrict__ w, double query_scale,
+                         double ns_edge, double ns_cell, double ns_level, int32_t* __restrict__ dec) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoly) return;
+  const PolyDev& P = pd[p];
+  if (!P.refining || P.depth != 0) {
+    dec[p] = 1;
+    return;
+  }
+  double saving = 0.0;
+  for (int r = 0; r < kGrid0Cells; r++) {
+    const int pr = p * kGrid0Cells + r;
+    if (st[2 * pr + 1] == 0) continue;  // nothing of r is refined either way
+    saving += (double)w[pr] * (exp2((double)Y[2 * pr]) - exp2((double)Y[2 * pr + 1]));
+  }
+  const double cost = ns_edge * P.N + ns_cell * ((double)P.U + P.nbc) + ns_level;
+  dec[p] = (P.U > 0 && saving * query_scale > cost) ? 1 : 0;
+}
+
+int ensure_learn(gahhg_ctx* c) {
+  const int64_t np = (int64_t)c->polys.size();
+  LearnState& L = c->learn;
+  if (L.cap >= np) return GAHHG_OK;
+  void* ptrs[] = {L.d_estat, L.d_hist, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec};
+  for (void* q : ptrs) dfree(q);
+  L.d_estat = nullptr;
+  L.d_hist = nullptr;
+  L.d_w = nullptr;
+  L.d_stat = nullptr;
+  L.d_x = nullptr;
+  L.d_y = nullptr;
+  L.d_dec = nullptr;
+  L.cap = 0;
+  const int64_t pr = np * kGrid0Cells;
+  if (dalloc(c, (void**)&L.d_estat, pr * 3 * 8) || dalloc(c, (void**)&L.d_hist, pr * 4) ||
+      dalloc(c, (void**)&L.d_w, pr * 4) || dalloc(c, (void**)&L.d_stat, pr * 2 * 4) ||
+      dalloc(c, (void**)&L.d_x, pr * 2 * kMlpIn * 4) || dalloc(c, (void**)&L.d_y, pr * 2 * 4) ||
+      dalloc(c, (void**)&L.d_dec, np * 4))
+    return GAHHG_E_NOMEM;
+  L.cap = np;
+  return GAHHG_OK;
+}
+
+unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// Edge statistics and expected points per (polygon, top-level cell); needs
+// the level-0 grid and the polygon records on the device.
+int learn_prepare(gahhg_ctx* c) {
+  LearnState& L = c->learn;
+  int rc = ensure_learn(c);
+  if (rc) return rc;
+  const int np = (int)c->polys.size();
+  rc = edge_stats(c, L.d_estat);
+  if (rc) return rc;
+  const uint32_t* hist = nullptr;
+  if (L.d_hint_xyz && L.hint_n > 0) {
+    rc = fill_i32(c, (int32_t*)L.d_hist, (int64_t)np * kGrid0Cells, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_hint_hist, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
+                       (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly, L.hint_n, np, L.d_hist);
+    rc = hip_check(c, hipGetLastError(), "hint histogram");
+    if (rc) return rc;
+    hist = L.d_hist;
+  }
+  const double q_default = np > 0 ? L.cost[0] / np : 0.0;
+  hipLaunchKernelGGL(k_weights, dim3(blocks((int64_t)np * kGrid0Cells, 256)), dim3(256), 0, c->stream, np,
+                     (const PolyDev*)c->d_pd, hist, L.hint_weight, q_default,
+                     (const unsigned long long*)L.d_estat, L.d_w);
+  return hip_check(c, hipGetLastError(), "point weights");
+}
+
+// Features of level `level` over the cell range [lo, lo + n) into L.d_x.
+static int learn_features(gahhg_ctx* c, int level, int lo, int n, int marked, int only_refining) {
+  LearnState& L = c->learn;
+  const int np = (int)c->polys.size();
+  int rc = fill_i32(c, L.d_stat, (int64_t)np * kGrid0Cells * 2, 0);
+  if (rc) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_level_stats, dim3(blocks(n, 256)), dim3(256), 0, c->stream, view(c),
+                       (const PolyDev*)c->d_pd, lo, n, marked, only_refining, L.d_stat);
+    rc = hip_check(c, hipGetLastError(), "level stats");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_features, dim3(blocks((int64_t)np * kGrid0Cells * 2, 256)), dim3(256), 0, c->stream, np,
+                     level, (const PolyDev*)c->d_pd, (const int32_t*)L.d_stat, (const unsigned long long*)L.d_estat,
+                     (const float*)L.d_w, L.d_x);
+  return hip_check(c, hipGetLastError(), "features");
+}
+
+// Decisions for level `level` (cells [lo, lo + n) just marked and planned);
+// queued on the stream into h_dec, valid after the next synchronisation.
+int learn_level(gahhg_ctx* c, int level, int lo, int n, int32_t* h_dec) {
+  LearnState& L = c->learn;
+  const int np = (int)c->polys.size();
+  int rc = learn_features(c, level, lo, n, 1, 1);
+  if (!rc) rc = mlp_forward(c, L.d_x, (int64_t)np * kGrid0Cells * 2, L.d_weights, L.d_y);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_decide, dim3(blocks(np, 64)), dim3(64), 0, c->stream, np, (const PolyDev*)c->d_pd,
+                     (const int32_t*)L.d_stat, (const float*)L.d_y, (const float*)L.d_w, L.cost[1], L.cost[2],
+                     L.cost[3], L.cost[4], L.d_dec);
+  rc = hip_check(c, hipGetLastError(), "decide");
+  if (rc) return rc;
+  return hip_check(c, hipMemcpyAsync(h_dec, L.d_dec, np * 4, hipMemcpyDeviceToHost, c->stream), "decisions");
+}
+
+// Features of a built hierarchy at `level` (refined cells stand for the
+// cells the build crossed there), for training: X[np * 42 * 2 * 12].
+int learn_features_after_build(gahhg_ctx* c, int level, float* h_x) {
+  LearnState& L = c->learn;
+  const int np = (int)c->polys.size();
+  int rc = learn_prepare(c);
+  if (rc) return rc;
+  const bool in = level >= 0 && level < (int)c->glv.size();
+  const LevelDesc G = in ? c->glv[level] : LevelDesc{0, 0, 0, 0, 0, 0, 0};
+  rc = learn_features(c, level, G.cstart, G.cnum, 0, 0);
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(h_x, L.d_x, (int64_t)np * kGrid0Cells * 2 * kMlpIn * 4, hipMemcpyDeviceToHost,
+                                     c->stream),
+                   "features D2H");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "features sync");
+  return rc;
+}
+
+void learn_free(gahhg_ctx* c) {
+  LearnState& L = c->learn;
+  void* ptrs[] = {L.d_estat, L.d_hist, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec, L.d_weights, L.d_hint_xyz,
+                  L.d_hint_poly};
+  for (void* q : ptrs) dfree(q);
+  L = LearnState();
+}
+
+}  // namespace gahhg

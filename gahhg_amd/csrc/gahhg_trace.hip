@@ -42,12 +42,12 @@ __global__ void k_cell_trace(GridView g, const PolyDev* __restrict__ pd, LevelTa
   }
 }
 
+}  // namespace
+
 // Edge aggregates per top-level cell of the edge's first endpoint (first
 // matching level-0 cell): out[(p * 42 + r) * 3 + k] = {edges, sum of arc
 // lengths, sum of turning angles to the next edge of the ring}, the sums in
 // fixed point (2^-40 rad) so that they do not depend on summation order.
-constexpr double kFix = 1099511627776.0;  // 2^40
-
 __global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ edge,
                              const int32_t* __restrict__ epoly, const int32_t* __restrict__ pe, int64_t Nt,
                              unsigned long long* __restrict__ out) {
@@ -77,7 +77,15 @@ __global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const d
   atomicAdd(o + 2, (unsigned long long)(turn * kFix));
 }
 
-}  // namespace
+int edge_stats(gahhg_ctx* c, unsigned long long* d_out) {
+  const int64_t Nt = c->n_edges;
+  int rc = fill_i32(c, (int32_t*)d_out, (int64_t)c->polys.size() * kGrid0Cells * 3 * 2, 0);
+  if (rc || Nt <= 0) return rc;
+  hipLaunchKernelGGL(k_edge_trace, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, c->stream, view(c),
+                     (const PolyDev*)c->d_pd, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                     (const int32_t*)c->d_pe, Nt, d_out);
+  return hip_check(c, hipGetLastError(), "edge stats");
+}
 
 int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges) {
   const int np = (int)c->polys.size();

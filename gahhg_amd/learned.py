@@ -21,7 +21,8 @@ Features per (polygon, top-level cell r, level l, refine k), all O(1):
   8  mean turning angle / pi
   9  U_r / max(C_r, 1)
   10 log2(1 + N) / 20            N: edges of the polygon
-  11 w_r                          share of the polygon's points in r
+  11 w_r                          share of the polygon's points in r (edge
+                                 share N_r / N when no points are expected)
 """
 import ctypes
 import json

@@ -192,6 +192,30 @@ int gahhg_mlp_backward(gahhg_ctx* ctx, const float* d_x, int64_t rows, const uin
 int gahhg_mlp_adam(gahhg_ctx* ctx, float* d_master, const float* d_grad, float* d_m1, float* d_m2, int32_t step,
                    float lr, uint16_t* d_w);
 
+/* Learned refinement depth.  With a model set, every polygon without a
+ * fixed depth is refined level by level while the model's predicted query
+ * time saved exceeds the predicted build time of the next level:
+ *   sum_r w_r (2^y_r(stop) - 2^y_r(refine)) * cost[1]
+ *       > cost[2] * N + cost[3] * (cells of the next level) + cost[4]   (ns)
+ * over the top-level cells r of the polygon, with w_r the points expected
+ * in r (from the query hint, else cost[0] points per polygon spread by edge
+ * share) and y_r(k) the model's log2 query time per point (ns) with
+ * refinement stopping at this level (k = 0) or after one more (k = 1).
+ * The hierarchy keeps the reference's structure (one bottom level per
+ * polygon); max_depth still caps it.  params: GAHHG_MLP_PARAMS values as
+ * for gahhg_mlp_*, rounded to BF16 here; NULL switches the model off.
+ * cost: 5 values {expected points, query-time scale, ns per edge, ns per
+ * next-level cell, ns per level}.  Takes effect at the next build. */
+int gahhg_set_depth_model(gahhg_ctx* ctx, const float* params, const double* cost);
+/* Sample of the points the hierarchy will answer (host XYZ, polygon ids
+ * or NULL for polygon 0), each standing for `weight` points: gives the
+ * expected points per polygon and top-level cell.  n = 0 clears it. */
+int gahhg_set_query_hint(gahhg_ctx* ctx, const double* xyz, const int32_t* poly, int64_t n, double weight);
+/* The model's 12 features for every (polygon, top-level cell, stop/refine)
+ * row at `level` of the last build (cells refined there stand for the
+ * cells crossed): x[((p * 42 + r) * 2 + k) * 12 + f]. */
+int gahhg_depth_features(gahhg_ctx* ctx, int32_t level, float* x);
+
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */
 int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);
