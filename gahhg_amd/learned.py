@@ -147,3 +147,185 @@ def load_model(path=MODEL_PATH):
     with open(path) as f:
         d = json.load(f)
     return np.asarray(d["params"], dtype=np.float32), d["cost"], d.get("meta", {})
+
+
+# ---------------------------------------------------------------------------
+# Training data: measured build and query times of fixed-depth hierarchies
+# ---------------------------------------------------------------------------
+def training_polygons(seed=101, n_batch=40, rings=(50_000, 150_000, 400_000, 800_000)):
+    """Training set (disjoint from the benchmark workloads by seed): polygons
+    of the batch generator and fractal rings of several sizes, each with the
+    point distribution of its benchmark family."""
+    from . import datasets as ds
+    out = []
+    for p in ds.batch_polygons(n_poly=n_batch, seed=seed):
+        out.append(dict(kind="cap", lon=p["lon"], lat=p["lat"], ring_off=p["ring_off"], centre=p["centre"],
+                        radius=p["radius"]))
+    for j, n in enumerate(rings):
+        w = ds.fractal_1m(n_queries=0, seed=seed + 1 + j, n_outer=n, n_hole=max(1000, n // 20))
+        out.append(dict(kind="mixed", lon=w["lon"], lat=w["lat"], ring_off=w["ring_off"]))
+    return out
+
+
+def sample_points(poly, n, seed):
+    from . import datasets as ds, query_lonlat_to_xyz
+    rng = np.random.default_rng(seed)
+    if poly["kind"] == "cap":
+        lon, lat = ds.cap_queries(rng, poly["centre"][0], poly["centre"][1], 1.3 * poly["radius"], n)
+    else:
+        lon, lat = ds.mixed_queries(rng, poly["lon"], poly["lat"], poly["ring_off"], n, 0.7)
+    return query_lonlat_to_xyz(lon, lat)
+
+
+def collect(polys, n_points=2_000_000, span=(-3, 2), reps=3, log=print):
+    """Per polygon: the density-rule depth D, then for every fixed depth d in
+    [D + span[0], D + span[1]] (1..15) the build time (ms) and the query time
+    per point (ns, points grouped as in production), and the model features
+    of every level of the deepest hierarchy with the sample as query hint."""
+    from . import GPISP
+    recs = []
+    for i, poly in enumerate(polys):
+        xyz = sample_points(poly, n_points, 7000 + i)
+        g = GPISP(device=0, query_sort=2)
+        g.set_polygon_lonlat(poly["lon"], poly["lat"], poly["ring_off"]).build()
+        D = g.stats()["depth"]
+        dx = g.alloc(xyz.nbytes)
+        dl = g.alloc(4 * len(xyz))
+        dx.upload(xyz)
+        ds_ = list(range(max(1, D + span[0]), min(15, D + span[1]) + 1))
+        t, b, cells = {}, {}, {}
+        for d in ds_:
+            g.set_polygon_lonlat(poly["lon"], poly["lat"], poly["ring_off"]).set_polygon_depths([d])
+            g.set_query_hint(xyz[::16], None, 16.0)
+            bt, qt = [], []
+            for _ in range(reps + 1):
+                g.build()
+                bt.append(g.stats()["pre_ms"])
+            for _ in range(reps + 1):
+                g.query_device(dx, len(xyz), dl)
+                qt.append(g.stats()["query_ms"])
+            b[d] = float(np.median(bt[1:]))
+            t[d] = float(np.median(qt[1:])) * 1e6 / len(xyz)
+            tr, _ = g.build_trace()
+            cells[d] = tr[0, :, :, 0].sum(axis=1).tolist()
+        # features of the deepest hierarchy (levels below its bottom)
+        X = np.stack([g.depth_features(l)[0] for l in range(max(ds_) - 1)]) if max(ds_) > 1 else np.zeros((0, 42, 2, 12))
+        N = len(poly["lon"]) - (len(poly["ring_off"]) - 1)
+        recs.append(dict(N=N, rule=D, t=t, b=b, cells=cells, X=X))
+        log(f"[collect] polygon {i}: N={N} rule depth {D}; ns/pt " +
+            " ".join(f"{d}:{t[d]:.3f}" for d in ds_) + "; build ms " + " ".join(f"{d}:{b[d]:.2f}" for d in ds_))
+        dx.free()
+        dl.free()
+        g.close()
+    return recs
+
+
+def groups(recs):
+    """(features[42, 12], point shares[42], target ns/pt) per (polygon,
+    level l, k): stop at l (k = 0, depth l + 1) or refine once (depth l + 2)."""
+    G = []
+    for ri, r in enumerate(recs):
+        for l in range(r["X"].shape[0]):
+            for k in (0, 1):
+                d = l + 1 + k
+                if d in r["t"] and (l + 1) in r["t"] and (l + 2) in r["t"]:
+                    x = r["X"][l, :, k, :]
+                    G.append((x, x[:, 11].astype(np.float64), r["t"][d], ri))
+    return G
+
+
+def fit_build_cost(recs):
+    """ns per edge, per next-level cell and per level from the build-time
+    increments of consecutive depths (non-negative least squares)."""
+    from scipy.optimize import nnls
+    A, y = [], []
+    for r in recs:
+        ds_ = sorted(r["b"])
+        for d0, d1 in zip(ds_, ds_[1:]):
+            new_cells = r["cells"][d1][d0] if d0 < len(r["cells"][d1]) else 0
+            A.append([r["N"], new_cells, 1.0])
+            y.append((r["b"][d1] - r["b"][d0]) * 1e6)
+    coef, _ = nnls(np.asarray(A, dtype=np.float64), np.asarray(y, dtype=np.float64))
+    return [float(c) for c in coef]
+
+
+def train(ctx, G, steps=3000, lr=2e-3, seed=0, log=print):
+    """Fits the model so that, per group, log2(sum_r share_r 2^y_r) matches
+    log2 of the measured ns per point.  Forward, backward and Adam run in
+    the HIP kernels; the per-group loss and its upstream gradient are formed
+    on the host."""
+    m = DeviceMLP(ctx, seed=seed)
+    X = np.concatenate([g[0] for g in G]).astype(np.float32)
+    S = np.stack([g[1] for g in G])
+    S = S / np.maximum(S.sum(axis=1, keepdims=True), 1e-30)
+    T = np.log2(np.array([g[2] for g in G]))
+    m.set_inputs(X)
+    ng = len(G)
+    hist = []
+    for step in range(steps):
+        y = m.forward().astype(np.float64).reshape(ng, 42)
+        ymax = y.max(axis=1, keepdims=True)
+        e = S * np.exp2(y - ymax)
+        P = e.sum(axis=1)
+        pred = np.log2(np.maximum(P, 1e-300)) + ymax[:, 0]
+        r = pred - T
+        loss = float(np.mean(r * r))
+        dy = (2.0 * r / ng)[:, None] * e / np.maximum(P, 1e-300)[:, None]
+        m.backward(dy.reshape(-1).astype(np.float32))
+        m.adam(lr if step < steps * 0.7 else lr * 0.2)
+        if step % 500 == 0 or step == steps - 1:
+            log(f"[train] step {step} loss {loss:.5f} (rms log2 error {math.sqrt(loss):.4f})")
+            hist.append((step, loss))
+    params = m.params()
+    m.free()
+    return params, hist
+
+
+def predict_groups(ctx, params, G):
+    m = DeviceMLP(ctx, params=params)
+    X = np.concatenate([g[0] for g in G]).astype(np.float32)
+    y = m.forward(X).astype(np.float64).reshape(len(G), 42)
+    m.free()
+    S = np.stack([g[1] for g in G])
+    S = S / np.maximum(S.sum(axis=1, keepdims=True), 1e-30)
+    return np.log2(np.maximum((S * np.exp2(y)).sum(axis=1), 1e-300))
+
+
+def main(argv=None):
+    import argparse
+    import sys
+    import time
+    from . import GPISP
+    ap = argparse.ArgumentParser(description="Train the learned refinement-depth model on the GPU.")
+    ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--points", type=int, default=2_000_000)
+    ap.add_argument("--batch-polygons", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=3000)
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    t0 = time.time()
+    polys = training_polygons(n_batch=args.batch_polygons)
+    log(f"[train] {len(polys)} training polygons generated in {time.time() - t0:.1f}s")
+    recs = collect(polys, n_points=args.points, log=log)
+    cost = fit_build_cost(recs)
+    log(f"[train] build cost: {cost[0]:.3f} ns/edge, {cost[1]:.2f} ns/cell, {cost[2]:.0f} ns/level")
+    G = groups(recs)
+    hold = {ri for ri in range(len(recs)) if ri % 5 == 4}
+    Gt = [g for g in G if g[3] not in hold]
+    Gv = [g for g in G if g[3] in hold]
+    ctx = GPISP(device=0)
+    params, hist = train(ctx, Gt, steps=args.steps, log=log)
+    pv = predict_groups(ctx, params, Gv)
+    tv = np.log2([g[2] for g in Gv])
+    rmse = float(np.sqrt(np.mean((pv - tv) ** 2))) if len(Gv) else None
+    log(f"[train] held-out groups {len(Gv)}: rms log2 error {rmse}")
+    ctx.close()
+    meta = {"training_polygons": len(polys), "groups": len(G), "held_out_groups": len(Gv),
+            "held_out_rms_log2_error": rmse, "loss_history": hist, "points_per_polygon": args.points,
+            "records": [{"N": r["N"], "rule": r["rule"], "ns_per_point": r["t"], "build_ms": r["b"]} for r in recs]}
+    save_model(args.out, params, {"ns_edge": cost[0], "ns_cell": cost[1], "ns_level": cost[2]}, meta)
+    log(f"[train] model written to {args.out}")
+
+
+if __name__ == "__main__":
+    main()
