@@ -28,6 +28,20 @@ static inline unsigned cblocks(int64_t n) {
   return (unsigned)std::max<int64_t>((n + kCBlock - 1) / kCBlock, 1);
 }
 
+// blocks for G lanes per item
+static inline unsigned gblocks(int64_t n, int G) {
+  return (unsigned)std::max<int64_t>((n * G + kCBlock - 1) / kCBlock, 1);
+}
+
+// Lanes per cell for the per-ref loops of a level: the smallest power of two
+// >= a quarter of the mean refs per bottom cell, in [1, 64] (small groups
+// keep many cells per wave where cells hold few refs).
+int group_for_refs(double mean_refs) {
+  int G = 1;
+  while (G < 64 && G < 0.25 * mean_refs) G *= 2;
+  return G;
+}
+
 __device__ __forceinline__ V3 cvtx(const GridView& g, int i) { return load3(g.v + 3 * (int64_t)i); }
 
 // Leaf centre as the query and classification kernels compute it: the mean
@@ -136,13 +150,117 @@ __device__ __forceinline__ int cell_phase(const GridView& g, const PolyDev* pd, 
   return L == lv ? 0 : (L > lv ? 1 : 2);
 }
 
+// setBottomNECell with a group of G lanes per cell (G a power of two up to
+// 64, chosen from the level's mean refs per leaf): the lanes evaluate the
+// nearest feature of the cell's refs in parallel (the per-ref part of
+// cp_prop), then every lane of the group replays cp_prop's selection over
+// those values in ref order, so the chosen feature -- including the
+// shared-vertex tie rule, whose comparison needs only each ref's own values
+// -- is exactly cp_prop's.  Conditions below are uniform within a group.
+__device__ __forceinline__ double shfl_d(double v, int src) {
+  return __hiloint2double(__shfl(__double2hiint(v), src, 64), __shfl(__double2loint(v), src, 64));
+}
+
+struct Group {
+  int lane, gl, gbase, G;
+  unsigned long long mask;
+};
+
+__device__ __forceinline__ Group group_of(int G) {
+  Group r;
+  r.G = G;
+  r.lane = threadIdx.x & 63;
+  r.gl = r.lane & (G - 1);
+  r.gbase = r.lane & ~(G - 1);
+  r.mask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << r.gbase);
+  return r;
+}
+
 __global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __restrict__ pd, const double* pv,
-                                                const int32_t* pe, const double* edge, int cstart, int cnum, int lv) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnum) return;
-  const int cid = cstart + t;
+                                                const int32_t* pe, const double* edge, int cstart, int cnum, int lv,
+                                                int G) {
+  const Group gr = group_of(G);
+  const int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  if (w >= cnum) return;
+  const int cid = cstart + w;
   if (cell_phase(g, pd, cid, lv) != 0) return;
-  if (g.c_rnum[cid] != 0) g.c_type[cid] = cp_prop(g, pv, pe, edge, cid, mean_center(g, cid));
+  const int rn = g.c_rnum[cid];
+  if (rn == 0) return;
+  const V3 center = mean_center(g, cid);
+  const int rs = g.c_rstart[cid];
+  double mindist = FLT_MAX;
+  int best = -1;            // ref slot of the chosen feature
+  double best_side = 0.0;   // dot(n0, center) of the chosen edge
+  double best_lean = 0.0;
+  int nv_index = -1;
+  for (int base = 0; base < rn; base += G) {
+    const int i = base + gr.gl;
+    double curdist = 0.0, side = 0.0, lean = 0.0;
+    int cur_nv = -1;
+    if (i < rn) {
+      const int se = g.refs[rs + i];
+      const double* er = edge + 9 * (int64_t)se;
+      const V3 v0 = load3(er), v1 = load3(er + 3), n0 = load3(er + 6);
+      V3 n1 = unit(cross(center, n0));
+      V3 inter = unit(cross(n0, n1));
+      if (lt0(dot(inter, v0))) inter = neg(inter);
+      V3 t0 = unit(sub(v0, inter));
+      V3 t1 = unit(sub(v1, inter));
+      const double t3 = dot(t0, t1);
+      if (lt0(t3)) {
+        curdist = sqlen(sub(inter, center));
+        cur_nv = -1;
+      } else {
+        curdist = sqlen(sub(center, v0));
+        const double vd = sqlen(sub(center, v1));
+        if (vd < curdist) {
+          curdist = vd;
+          cur_nv = pe[2 * se + 1];
+        } else {
+          cur_nv = pe[2 * se];
+        }
+      }
+      side = dot(n0, center);
+      if (cur_nv != -1) {
+        const int oth = (pe[2 * se] == cur_nv) ? pe[2 * se + 1] : pe[2 * se];
+        const V3 pnv = load3(pv + 3 * (int64_t)cur_nv);
+        const V3 a1 = unit(sub(load3(pv + 3 * (int64_t)oth), pnv));
+        lean = dot(a1, sub(center, pnv));
+      }
+    }
+    const int m = min(G, rn - base);
+    for (int j = 0; j < m; j++) {
+      const int src = gr.gbase + j;
+      const double cd = shfl_d(curdist, src), sd = shfl_d(side, src), ln = shfl_d(lean, src);
+      const int cn = __shfl(cur_nv, src, 64);
+      if (nv_index != -1 && nv_index == cn) {
+        const double a = best_side, b = sd;
+        if ((le0(a) && ge0(b)) || (ge0(a) && le0(b))) {
+          if (ln > best_lean) {
+            mindist = cd;
+            best = base + j;
+            best_side = sd;
+            best_lean = ln;
+            nv_index = cn;
+            continue;
+          }
+        }
+      }
+      if (cd < mindist) {
+        mindist = cd;
+        best = base + j;
+        best_side = sd;
+        best_lean = ln;
+        nv_index = cn;
+      }
+    }
+  }
+  if (gr.gl == 0) {
+    // no feature chosen (all distances NaN): the reference keeps a zero
+    // normal, whose side test fails
+    const double s = best >= 0 ? best_side : 0.0;
+    g.c_type[cid] = gt0(s) ? kIn : kOut;
+  }
 }
 
 __global__ void k_snap(const int32_t* type, int cstart, int cnum, int32_t* snap) {
@@ -150,12 +268,36 @@ __global__ void k_snap(const int32_t* type, int cstart, int cnum, int32_t* snap)
   if (t < cnum) snap[t] = type[cstart + t];
 }
 
-// setBottomDECell: every border whose neighbour was labelled in the
-// snapshot decides in turn; the last one wins, as in the reference loop.
+// Crossing parity as parity_label by a lane group: the lanes split the refs
+// of `from` and the count is the popcount of the group's ballots.
+__device__ int parity_label_group(const GridView& g, const double* __restrict__ edge, int from, int mb,
+                                  int from_type, const Group& gr) {
+  const V3 m = unit(mul(add(cvtx(g, g.b_v0[mb]), cvtx(g, g.b_v1[mb])), 0.5));
+  const V3 c = mean_center(g, from);
+  const V3 n1 = arc_normal(c, m);
+  int cnt = 0;
+  const int rs = g.c_rstart[from], rn = g.c_rnum[from];
+  for (int base = 0; base < rn; base += gr.G) {
+    const int i = base + gr.gl;
+    bool x = false;
+    if (i < rn) {
+      const double* er = edge + 9 * (int64_t)g.refs[rs + i];
+      const V3 u0 = load3(er), u1 = load3(er + 3), n0 = load3(er + 6);
+      x = !same_side(dot(n0, c), dot(n0, m)) && !same_side(dot(n1, u0), dot(n1, u1));
+    }
+    cnt += __popcll(__ballot(x) & gr.mask);
+  }
+  return (cnt % 2 == 0) ? from_type : 3 - from_type;
+}
+
+// setBottomDECell (a lane group per cell): every border whose neighbour was
+// labelled in the snapshot decides in turn; the last one wins, as in the
+// reference loop.
 __global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev* __restrict__ pd,
                                                        const double* edge, int cstart, int cnum, int lv,
-                                                       const int32_t* snap) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                       const int32_t* snap, int G) {
+  const Group gr = group_of(G);
+  const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   if (t >= cnum || snap[t] != kUndefined) return;
   const int cid = cstart + t;
   if (cell_phase(g, pd, cid, lv) != 0) return;
@@ -168,10 +310,10 @@ __global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev
     const int nb = g.b_parent[pe_];
     const int nt = snap[nb - cstart];
     if (nt == kUndefined) continue;
-    lab = parity_label(g, edge, nb, pe_, nt);
+    lab = parity_label_group(g, edge, nb, pe_, nt, gr);
     set = true;
   }
-  if (set) g.c_type[cid] = lab;
+  if (set && gr.gl == 0) g.c_type[cid] = lab;
 }
 
 // setIECell round: an unlabelled leaf adopts the label of the first
@@ -317,13 +459,22 @@ int classify_hierarchy(gahhg_ctx* c) {
   for (int lv = maxL; lv >= 0; lv--) {
     if (c->prm.verbose) printf("Classify level %d cells.\n", lv);
     const LevelDesc L = c->glv[lv];
+    // lane group per cell from the mean refs of the leaves with refs of the
+    // polygons whose bottom level this is
+    int64_t refs = 0, leaves = 0;
+    for (const PolyDev& P : c->pd)
+      if (P.L == lv) {
+        refs += P.R;
+        leaves += P.bcn;
+      }
+    const int G = group_for_refs(leaves > 0 ? (double)refs / leaves : 1.0);
     // bottom polygons: setBottomNECell, then setBottomDECell on a snapshot
-    hipLaunchKernelGGL(k_ne, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_pv,
-                       (const int32_t*)c->d_pe, (const double*)c->d_edge, L.cstart, L.cnum, lv);
+    hipLaunchKernelGGL(k_ne, dim3(gblocks(L.cnum, G)), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_pv,
+                       (const int32_t*)c->d_pe, (const double*)c->d_edge, L.cstart, L.cnum, lv, G);
     hipLaunchKernelGGL(k_snap, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, (const int32_t*)c->g.c_type,
                        L.cstart, L.cnum, snap);
-    hipLaunchKernelGGL(k_de_bottom, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd,
-                       (const double*)c->d_edge, L.cstart, L.cnum, lv, (const int32_t*)snap);
+    hipLaunchKernelGGL(k_de_bottom, dim3(gblocks(L.cnum, G)), dim3(kCBlock), 0, c->stream, g, pd,
+                       (const double*)c->d_edge, L.cstart, L.cnum, lv, (const int32_t*)snap, G);
     // deeper polygons: setDECell
     hipLaunchKernelGGL(k_de_upper, dim3(cblocks(L.cnum)), dim3(kCBlock), 0, c->stream, g, pd,
                        (const double*)c->d_edge, L.cstart, L.cnum, lv);

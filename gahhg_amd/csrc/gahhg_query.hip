@@ -448,61 +448,96 @@ __device__ __forceinline__ bool is_bottom(const GridView& g, const PolyDev* pd, 
   return cid >= P.bcs && cid < P.bcs + P.bcn;
 }
 
-__global__ void k_leaf_count(GridView g, const PolyDev* __restrict__ pd, const int32_t* __restrict__ pe,
-                             int cnum, int32_t* __restrict__ cnt, DevState* st) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// G lanes per cell (G a power of two up to 64): lane i of the group looks
+// at ref i (chunks of G); a ref closes a chain when the next ref does not
+// continue it (or it is the cell's last ref).  Conditions are uniform within
+// a group.
+__device__ __forceinline__ bool chain_break(const int32_t* pe, const int32_t* r, int rn, int i) {
+  return i + 1 == rn || !chained(pe, r[i], r[i + 1]);
+}
+
+__device__ __forceinline__ unsigned long long group_mask(int lane, int G) {
+  return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
+}
+
+__global__ void __launch_bounds__(256) k_leaf_count(GridView g, const PolyDev* __restrict__ pd,
+                                                    const int32_t* __restrict__ pe, int cnum,
+                                                    int32_t* __restrict__ cnt, DevState* st, int G) {
+  const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  const int lane = threadIdx.x & 63, gl = lane & (G - 1);
+  const unsigned long long gm = group_mask(lane, G);
   if (t >= cnum) return;
   const int cid = t;
   const int rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
   int chains = 0;
   if (rn > 0) {
     const int32_t* r = g.refs + g.c_rstart[cid];
-    chains = 1;
-    for (int i = 0; i + 1 < rn; i++) chains += chained(pe, r[i], r[i + 1]) ? 0 : 1;
+    for (int base = 0; base < rn; base += G) {
+      const int i = base + gl;
+      chains += __popcll(__ballot(i < rn && chain_break(pe, r, rn, i)) & gm);
+    }
   }
-  cnt[t] = rn > 0 ? rn + chains : 0;
-  const unsigned long long w = __ballot(rn > 0);
-  if ((threadIdx.x & 63) == 0 && w) atomicAdd(&st->nleaf, (int32_t)__popcll(w));
+  if (gl == 0) {
+    cnt[t] = rn > 0 ? rn + chains : 0;
+    if (rn > 0) atomicAdd(&st->nleaf, 1);
+  }
 }
 
-// Centre and edge stream of every bottom cell with refs.
-__global__ void k_leaf_write(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ pv,
-                             const int32_t* __restrict__ pe, const double* __restrict__ edge, int cnum,
-                             const int32_t* __restrict__ roff, double* __restrict__ qc, int32_t* __restrict__ hdr,
-                             double* __restrict__ sv, double* __restrict__ sn, uint8_t* __restrict__ sf) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// Centre and edge stream of every bottom cell with refs (G lanes per cell):
+// ref i goes to record k0 + i + (chain breaks before i), and a closing
+// record (the chain's last vertex) follows every break.
+__global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* __restrict__ pd,
+                                                    const double* __restrict__ pv, const int32_t* __restrict__ pe,
+                                                    const double* __restrict__ edge, int cnum,
+                                                    const int32_t* __restrict__ roff, double* __restrict__ qc,
+                                                    int32_t* __restrict__ hdr, double* __restrict__ sv,
+                                                    double* __restrict__ sn, uint8_t* __restrict__ sf, int G) {
+  const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  const int lane = threadIdx.x & 63, gl = lane & (G - 1);
+  const unsigned long long gm = group_mask(lane, G);
   if (t >= cnum) return;
   const int cid = t;
   const int rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
-  int k = roff[t];
-  hdr[2 * t] = k;
+  const int k0 = roff[t];
   if (rn <= 0) {
-    hdr[2 * t + 1] = 0;
+    if (gl == 0) {
+      hdr[2 * t] = k0;
+      hdr[2 * t + 1] = 0;
+    }
     return;
   }
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
   V3 c = v3(0.0, 0.0, 0.0);
   for (int i = 0; i < bn; i++) c = add(c, load3(g.v + 3 * (int64_t)g.b_v0[bs + i]));
   c = unit(divk(c, bn));
-  store3(qc + 3 * (int64_t)t, c);
+  if (gl == 0) store3(qc + 3 * (int64_t)t, c);
   const int32_t* r = g.refs + g.c_rstart[cid];
-  const int k0 = k;
-  for (int i = 0; i < rn; i++) {
-    const int e = r[i];
-    const V3 ne = load3(edge + 9 * (int64_t)e + 6);
-    const double r2 = dot(ne, c);
-    store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e]));
-    store3(sn + 3 * (int64_t)k, ne);
-    sf[k] = (uint8_t)(4 | (lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
-    k++;
-    if (i + 1 == rn || !chained(pe, e, r[i + 1])) {
-      store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e + 1]));
-      store3(sn + 3 * (int64_t)k, v3(0.0, 0.0, 0.0));
-      sf[k] = 0;
-      k++;
+  int carry = 0;  // chain breaks in earlier chunks
+  for (int base = 0; base < rn; base += G) {
+    const int i = base + gl;
+    const bool brk = i < rn && chain_break(pe, r, rn, i);
+    const unsigned long long bal = __ballot(brk) & gm;
+    if (i < rn) {
+      const int before = carry + __popcll(bal & ((1ull << lane) - 1ull));
+      const int k = k0 + i + before;
+      const int e = r[i];
+      const V3 ne = load3(edge + 9 * (int64_t)e + 6);
+      const double r2 = dot(ne, c);
+      store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e]));
+      store3(sn + 3 * (int64_t)k, ne);
+      sf[k] = (uint8_t)(4 | (lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
+      if (brk) {
+        store3(sv + 3 * (int64_t)(k + 1), load3(pv + 3 * (int64_t)pe[2 * e + 1]));
+        store3(sn + 3 * (int64_t)(k + 1), v3(0.0, 0.0, 0.0));
+        sf[k + 1] = 0;
+      }
     }
+    carry += __popcll(bal);
   }
-  hdr[2 * t + 1] = k - k0;
+  if (gl == 0) {
+    hdr[2 * t] = k0;
+    hdr[2 * t + 1] = rn + carry;
+  }
 }
 
 // Table-accelerated query; labels identical to k_query_soa.
@@ -708,10 +743,18 @@ int pack_query_layout(gahhg_ctx* c) {
   q.bins = np == 1 ? c->prm.query_bins : std::min(c->prm.query_bins, 128);
   const int64_t nc = std::max<int64_t>(c->tot_c, 1);
   GridView g = view(c);
-  const unsigned cb = (unsigned)((nc + 255) / 256);
   // record counts -> offsets (flag scratch holds >= tot_c entries)
-  hipLaunchKernelGGL(k_leaf_count, dim3(cb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
-                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st);
+  // lanes per cell from the mean refs of the bottom cells
+  int64_t refs = 0, bottom = 0;
+  for (const PolyDev& P : c->pd) {
+    refs += P.R;
+    bottom += P.bcn;
+  }
+  int G = 1;
+  while (G < 64 && G < (bottom > 0 ? 0.25 * refs / bottom : 1.0)) G *= 2;
+  const unsigned wb = (unsigned)((nc * G + 255) / 256);
+  hipLaunchKernelGGL(k_leaf_count, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st, G);
   int rc = hip_check(c, hipGetLastError(), "leaf count");
   if (rc) return rc;
   rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
@@ -726,9 +769,9 @@ int pack_query_layout(gahhg_ctx* c) {
       dalloc(c, (void**)&q.sf, q.nrec))
     return GAHHG_E_NOMEM;
   q.bytes = nc * 32 + q.nrec * 49;
-  hipLaunchKernelGGL(k_leaf_write, dim3(cb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
+  hipLaunchKernelGGL(k_leaf_write, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
                      (const double*)c->d_pv, (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
-                     (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf);
+                     (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf, G);
   rc = hip_check(c, hipGetLastError(), "leaf write");
   if (rc) return rc;
   if (q.bins > 0) {
