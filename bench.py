@@ -39,6 +39,36 @@ def per_query_flops(qc, n):
     return (5 * qc["plane_tests"] + 3 * qc["bnum_sum"] + 14 * qc["boundary_hits"] + 64 * qc["ref_tests"]) / n
 
 
+def load_depth_model(args):
+    """The trained refinement-depth model (None when --rule or absent)."""
+    if args.rule:
+        return None
+    from gahhg_amd import learned
+    if not os.path.exists(learned.MODEL_PATH):
+        return None
+    params, cost, meta = learned.load_model()
+    return params, cost
+
+
+def apply_model(g, model, xyz, poly, Q, sample=1_000_000):
+    """Query hint (every k-th point standing for k points) and the model."""
+    params, cost = model
+    k = max(1, len(xyz) // sample)
+    g.set_query_hint(xyz[::k], None if poly is None else poly[::k], float(k))
+    g.set_depth_model(params, [float(Q), 1.0, cost["ns_edge"], cost["ns_cell"], cost["ns_level"]])
+
+
+def median_build(g, n):
+    b = []
+    for i in range(n + 1):
+        g.build()
+        s = g.stats()
+        if i:
+            b.append((s["pre_ms"], s["create_ms"], s["classify_ms"]))
+    b.sort()
+    return b[len(b) // 2]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +82,7 @@ def main():
     ap.add_argument("--bins", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
+    ap.add_argument("--rule", action="store_true", help="density-rule depth only (no learned model)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -67,18 +98,20 @@ def main():
     Q = xyz.shape[0]
     log(f"generated N={len(w['lon']) - (len(w['ring_off']) - 1)} edges, Q={Q} in {time.time() - t0:.1f}s")
 
+    model = load_depth_model(args)
+    # the density-rule build of the accelerated path (the learned build is
+    # compared against it)
     g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
-    builds = []
-    for i in range(args.builds + 1):
-        g.build()
-        s = g.stats()
-        if i:
-            builds.append((s["pre_ms"], s["create_ms"], s["classify_ms"]))
-    builds.sort()
-    pre_ms, create_ms, classify_ms = builds[len(builds) // 2]
+    rule_pre, rule_create, rule_classify = median_build(g, args.builds)
+    rule_depth = g.stats()["depth"]
+    log(f"rule build: depth {rule_depth} pre {rule_pre:.2f} ms (create {rule_create:.2f}, classify {rule_classify:.2f})")
+    if model is not None:
+        apply_model(g, model, xyz, None, Q)
+    pre_ms, create_ms, classify_ms = median_build(g, args.builds)
     st = g.stats()
-    log(f"build: depth {st['depth']} pre {pre_ms:.2f} ms (create {create_ms:.2f}, classify {classify_ms:.2f})")
+    log(f"build ({'learned' if model else 'rule'} depth): depth {st['depth']} pre {pre_ms:.2f} ms "
+        f"(create {create_ms:.2f}, classify {classify_ms:.2f})")
 
     d_xyz = g.alloc(xyz.nbytes)
     d_lab = g.alloc(4 * Q)
@@ -90,7 +123,10 @@ def main():
 
     o = oracle_lib.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"])
     tb = time.time()
-    o.build()
+    if model is not None and st["depth"] != rule_depth:
+        o.build(max_depth=int(st["depth"]), auto_depth=False)  # the same hierarchy
+    else:
+        o.build()
     cpu_build_s = time.time() - tb
     g.query_device(d_xyz, Q, d_lab)
     labels = d_lab.download(np.empty(Q, dtype=np.int32))
@@ -144,7 +180,11 @@ def main():
             pl.append(gp.stats()["query_ms"])
         lab2 = dl2.download(np.empty(Q, dtype=np.int32))
         plain = {"queries_per_s": round(Q / (float(np.median(pl)) * 1e-3), 1), "kernel_ms": round(float(np.median(pl)), 3),
-                 "build_ms": round(float(np.median(pb)), 3), "labels_equal": bool((lab2 == labels).all())}
+                 "build_ms": round(float(np.median(pb)), 3),
+                 # differences can only come from a different depth (points
+                 # within 1e-8 of an edge, whose label depends on the hierarchy)
+                 "label_differences": int(np.count_nonzero(lab2 != labels)),
+                 "same_depth": int(gp.stats()["depth"]) == int(st["depth"])}
         log(f"plain path: {plain}")
         dx2.free()
         dl2.free()
@@ -175,7 +215,9 @@ def main():
                    "path": "single-polygon GPU: start-cell table, points grouped by start bin, lane-group leaf tests",
                    "parallelism": "1 GPU"},
         "build_ms": round(pre_ms, 3), "create_ms": round(create_ms, 3), "classify_ms": round(classify_ms, 3),
-        "depth": int(st["depth"]), "grid_bytes": int(st["gpu_space_bytes"]),
+        "depth": int(st["depth"]), "depth_rule": "learned" if model is not None else "density",
+        "density_rule_build": {"depth": int(rule_depth), "build_ms": round(rule_pre, 3)},
+        "grid_bytes": int(st["gpu_space_bytes"]),
         "parity": {"sample": int(ns), "mismatches": mism},
         "query_bins": args.bins,
         "query_gpu_ms": round(float(np.mean(gpu_ms)), 4), "query_grouping_ms": round(float(np.mean(sort_ms)), 4),
@@ -213,14 +255,17 @@ def bench_batch(args, oracle_lib):
     Q = len(qp)
     n_edges = int(sum(len(p[0]) - (len(p[2]) - 1) for p in items))
     log(f"batch_256: {len(items)} polygons, {n_edges} edges, {Q} queries generated in {time.time() - t0:.1f}s")
+    model = load_depth_model(args)
     g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
     g.set_polygons_lonlat(items)
-    bt = []
-    for i in range(4):
-        g.build()
-        if i:
-            bt.append(g.stats()["pre_ms"])
-    build_ms = float(np.median(bt))
+    rule_build = median_build(g, 3)[0]
+    rule_depths = [g.polygon_stats(k)["depth"] for k in range(len(items))]
+    if model is not None:
+        apply_model(g, model, xyz, qp, Q)
+    build_ms = median_build(g, 3)[0]
+    depths = [g.polygon_stats(k)["depth"] for k in range(len(items))]
+    log(f"batch_256: build {build_ms:.2f} ms ({'learned' if model else 'rule'} depths), rule build {rule_build:.2f} ms; "
+        f"depth changes vs rule: {sum(a != b for a, b in zip(depths, rule_depths))}")
     dx = g.alloc(xyz.nbytes)
     dp = g.alloc(4 * Q)
     dl = g.alloc(4 * Q)
@@ -265,10 +310,13 @@ def bench_batch(args, oracle_lib):
         idx = order[int(cnt[:k].sum()):int(cnt[:k + 1].sum())][:20000]
         if not len(idx):
             continue
-        o = oracle_lib.Oracle().set_lonlat(*items[k]).build()
+        o = oracle_lib.Oracle().set_lonlat(*items[k]).build(max_depth=depths[k], auto_depth=False)
         omism += int(np.count_nonzero(o.query_xyz(np.ascontiguousarray(xyz[idx])) != lab[idx]))
         checked += len(idx)
     res = {"n_polygons": len(items), "n_edges": n_edges, "n_queries": Q,
+           "depth_rule": "learned" if model is not None else "density",
+           "density_rule_build_ms": round(rule_build, 3),
+           "depths_changed_vs_density_rule": int(sum(a != b for a, b in zip(depths, rule_depths))),
            "batched": {"build_ms": round(build_ms, 3), "query_gpu_ms": round(float(np.median(ks)), 3),
                        "query_kernel_ms": round(float(np.median(kk)), 3),
                        "query_wall_ms": round(wall_ms, 3), "queries_per_s": round(Q / (float(np.median(ks)) * 1e-3), 1)},
@@ -276,7 +324,9 @@ def bench_batch(args, oracle_lib):
                                  "queries_per_s": round(Q / (pq * 1e-3), 1) if pq else None},
            "build_ratio": round(build_ms / pb, 4) if pb else None,
            "query_speedup": round(pq / float(np.median(ks)), 3) if pq else None,
-           "labels_equal_to_per_polygon": mism == 0, "oracle_sample": checked, "oracle_mismatches": omism}
+           "label_differences_vs_per_polygon": mism,
+           "per_polygon_same_depths": depths == rule_depths,
+           "oracle_sample": checked, "oracle_mismatches": omism}
     log(f"batch_256: {res}")
     return res
 

@@ -152,7 +152,7 @@ def load_model(path=MODEL_PATH):
 # ---------------------------------------------------------------------------
 # Training data: measured build and query times of fixed-depth hierarchies
 # ---------------------------------------------------------------------------
-def training_polygons(seed=101, n_batch=40, rings=(50_000, 150_000, 400_000, 800_000)):
+def training_polygons(seed=101, n_batch=120, rings=(30_000, 60_000, 120_000, 250_000, 500_000, 800_000, 1_200_000)):
     """Training set (disjoint from the benchmark workloads by seed): polygons
     of the batch generator and fractal rings of several sizes, each with the
     point distribution of its benchmark family."""
@@ -249,36 +249,59 @@ def fit_build_cost(recs):
     return [float(c) for c in coef]
 
 
-def train(ctx, G, steps=3000, lr=2e-3, seed=0, log=print):
+def _group_loss_grad(y, S, T):
+    ng = len(T)
+    y = y.astype(np.float64).reshape(ng, 42)
+    ymax = y.max(axis=1, keepdims=True)
+    e = S * np.exp2(y - ymax)
+    P = e.sum(axis=1)
+    pred = np.log2(np.maximum(P, 1e-300)) + ymax[:, 0]
+    r = pred - T
+    dy = (2.0 * r / ng)[:, None] * e / np.maximum(P, 1e-300)[:, None]
+    return float(np.mean(r * r)), dy
+
+
+def _shares(G):
+    S = np.stack([g[1] for g in G])
+    return S / np.maximum(S.sum(axis=1, keepdims=True), 1e-30)
+
+
+def train(ctx, G, steps=4000, lr=2e-3, seed=0, val=None, every=250, log=print):
     """Fits the model so that, per group, log2(sum_r share_r 2^y_r) matches
     log2 of the measured ns per point.  Forward, backward and Adam run in
     the HIP kernels; the per-group loss and its upstream gradient are formed
-    on the host."""
+    on the host.  With validation groups, the parameters of the step with
+    the lowest validation loss are returned (early stopping)."""
     m = DeviceMLP(ctx, seed=seed)
-    X = np.concatenate([g[0] for g in G]).astype(np.float32)
-    S = np.stack([g[1] for g in G])
-    S = S / np.maximum(S.sum(axis=1, keepdims=True), 1e-30)
-    T = np.log2(np.array([g[2] for g in G]))
-    m.set_inputs(X)
-    ng = len(G)
+    m.set_inputs(np.concatenate([g[0] for g in G]).astype(np.float32))
+    S, T = _shares(G), np.log2(np.array([g[2] for g in G]))
+    mv = None
+    if val:
+        mv = DeviceMLP(ctx, seed=seed)
+        mv.set_inputs(np.concatenate([g[0] for g in val]).astype(np.float32))
+        Sv, Tv = _shares(val), np.log2(np.array([g[2] for g in val]))
+    best = (float("inf"), m.params(), -1)
     hist = []
-    for step in range(steps):
-        y = m.forward().astype(np.float64).reshape(ng, 42)
-        ymax = y.max(axis=1, keepdims=True)
-        e = S * np.exp2(y - ymax)
-        P = e.sum(axis=1)
-        pred = np.log2(np.maximum(P, 1e-300)) + ymax[:, 0]
-        r = pred - T
-        loss = float(np.mean(r * r))
-        dy = (2.0 * r / ng)[:, None] * e / np.maximum(P, 1e-300)[:, None]
+    for step in range(steps + 1):
+        if mv is not None and step % every == 0:
+            mv.set_params(m.params())
+            lv, _ = _group_loss_grad(mv.forward(), Sv, Tv)
+            if lv < best[0]:
+                best = (lv, m.params(), step)
+        if step == steps:
+            break
+        loss, dy = _group_loss_grad(m.forward(), S, T)
         m.backward(dy.reshape(-1).astype(np.float32))
         m.adam(lr if step < steps * 0.7 else lr * 0.2)
         if step % 500 == 0 or step == steps - 1:
-            log(f"[train] step {step} loss {loss:.5f} (rms log2 error {math.sqrt(loss):.4f})")
+            log(f"[train] step {step} loss {loss:.5f} (rms log2 error {math.sqrt(loss):.4f})"
+                + (f", best validation {math.sqrt(best[0]):.4f} at step {best[2]}" if mv is not None else ""))
             hist.append((step, loss))
-    params = m.params()
+    params = best[1] if mv is not None else m.params()
     m.free()
-    return params, hist
+    if mv is not None:
+        mv.free()
+    return params, hist, (math.sqrt(best[0]) if mv is not None else None, best[2])
 
 
 def predict_groups(ctx, params, G):
@@ -299,8 +322,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="Train the learned refinement-depth model on the GPU.")
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--points", type=int, default=2_000_000)
-    ap.add_argument("--batch-polygons", type=int, default=40)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch-polygons", type=int, default=120)
+    ap.add_argument("--steps", type=int, default=4000)
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     t0 = time.time()
@@ -310,17 +333,23 @@ def main(argv=None):
     cost = fit_build_cost(recs)
     log(f"[train] build cost: {cost[0]:.3f} ns/edge, {cost[1]:.2f} ns/cell, {cost[2]:.0f} ns/level")
     G = groups(recs)
+    # polygons split three ways: training, validation (early stopping) and
+    # held out (reported)
+    val_ids = {ri for ri in range(len(recs)) if ri % 5 == 3}
     hold = {ri for ri in range(len(recs)) if ri % 5 == 4}
-    Gt = [g for g in G if g[3] not in hold]
+    Gt = [g for g in G if g[3] not in hold and g[3] not in val_ids]
+    Gval = [g for g in G if g[3] in val_ids]
     Gv = [g for g in G if g[3] in hold]
     ctx = GPISP(device=0)
-    params, hist = train(ctx, Gt, steps=args.steps, log=log)
+    params, hist, (vbest, vstep) = train(ctx, Gt, steps=args.steps, val=Gval, log=log)
+    log(f"[train] validation rms log2 error {vbest:.4f} at step {vstep}")
     pv = predict_groups(ctx, params, Gv)
     tv = np.log2([g[2] for g in Gv])
     rmse = float(np.sqrt(np.mean((pv - tv) ** 2))) if len(Gv) else None
     log(f"[train] held-out groups {len(Gv)}: rms log2 error {rmse}")
     ctx.close()
     meta = {"training_polygons": len(polys), "groups": len(G), "held_out_groups": len(Gv),
+            "validation_rms_log2_error": vbest, "validation_best_step": vstep,
             "held_out_rms_log2_error": rmse, "loss_history": hist, "points_per_polygon": args.points,
             "records": [{"N": r["N"], "rule": r["rule"], "ns_per_point": r["t"], "build_ms": r["b"]} for r in recs]}
     save_model(args.out, params, {"ns_edge": cost[0], "ns_cell": cost[1], "ns_level": cost[2]}, meta)
