@@ -77,6 +77,7 @@ EXPORTED = [
     "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
     "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
+    "gahhg_autotune_query",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -145,6 +146,7 @@ def lib():
         "gahhg_set_depth_model": ([P, P, P], ctypes.c_int),
         "gahhg_set_query_hint": ([P, P, P, i64, ctypes.c_double], ctypes.c_int),
         "gahhg_depth_features": ([P, i32, P], ctypes.c_int),
+        "gahhg_autotune_query": ([P, P, P, i64, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -382,6 +384,14 @@ class GPISP:
         e = edges.astype(np.float64)
         e[..., 1:] /= FIXED_POINT
         return cells, e
+
+    def autotune_device(self, d_xyz, d_poly, n, d_labels):
+        """Times the query's lane-group width and block size on these points
+        and keeps the fastest; returns (group, block, microseconds)."""
+        out = np.zeros(3, dtype=np.int32)
+        self._check(self.L.gahhg_autotune_query(self.h, d_xyz.ptr, d_poly.ptr if d_poly is not None else None,
+                                                int(n), d_labels.ptr, out.ctypes.data))
+        return int(out[0]), int(out[1]), int(out[2])
 
     def set_depth_model(self, params, cost):
         """Learned refinement depth (params: the model's 5313 values; cost:

@@ -940,6 +940,7 @@ int build_hierarchy(gahhg_ctx* c) {
   if (learned) {
     CK(learn_prepare(c));
     c->learn.h_dec.assign(np, 1);
+    c->learn.rule_stop.assign(np, -1);
   }
   memset(c->h_st, 0, sizeof(DevState));
   CK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state init"));
@@ -1006,8 +1007,22 @@ int build_hierarchy(gahhg_ctx* c) {
       PolyDev& P = c->pd[p];
       if (!P.refining) continue;
       if (learned && P.depth == 0) {
-        c->pgamma[p] = (float)(P.U * 1.0 / P.N);
-        if (!c->learn.h_dec[p]) {
+        // the model decides only inside the window it was trained on:
+        // refine while crossed cells / N < kLearnFloor (about three levels
+        // before the density rule would stop), never more than kLearnAbove
+        // levels past the level where the density rule stops
+        float k = P.U * 1.0 / P.N;
+        c->pgamma[p] = k;
+        int& rs = c->learn.rule_stop[p];
+        if (rs < 0 && (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01))) rs = lv;
+        bool stop;
+        if (k < kLearnFloor)
+          stop = false;
+        else if (rs >= 0 && lv >= rs + kLearnAbove)
+          stop = true;
+        else
+          stop = !c->learn.h_dec[p];
+        if (stop) {
           P.refining = 0;
           P.L = lv;
           P.use_prev = lv != 0 ? 1 : 0;

@@ -43,6 +43,7 @@ void free_grid(gahhg_ctx* c) {
   dfree(q.table);
   dfree(q.sub);
   dfree(q.tmp);
+  dfree(q.pcnt);
   dfree(q.qc);
   dfree(q.hdr);
   dfree(q.sv);
@@ -255,7 +256,7 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
   else
     gahhg_params_default(&c->prm);
   if (c->prm.max_depth < 1 || c->prm.max_depth > kMaxLevels - 1 || c->prm.block_query < 64 ||
-      c->prm.block_query > 1024 || (c->prm.block_query % 64) != 0 || c->prm.query_bins < 0 ||
+      c->prm.block_query > 256 || (c->prm.block_query % 64) != 0 || c->prm.query_bins < 0 ||
       c->prm.query_bins > 4096 || c->prm.query_group < 0 || c->prm.query_group > 64 ||
       (c->prm.query_group & (c->prm.query_group - 1)) != 0 || c->prm.query_sub < 0 || c->prm.query_sub > 256 ||
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
@@ -422,6 +423,54 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   c->st.inside = (int64_t)h_two[0];
   c->st.unlocated = (int64_t)h_two[1];
   c->st.outside = n - c->st.inside;
+  return GAHHG_OK;
+}
+
+int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                         int32_t* out3) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "autotune before build");
+  if (n <= 0 || !d_xyz || !d_labels) return set_err(c, GAHHG_E_ARG, "autotune: bad arguments");
+  if (!d_poly && c->polys.size() != 1)
+    return set_err(c, GAHHG_E_ARG, "autotune: polygon ids required for a batch of %d", (int)c->polys.size());
+  if (c->prm.query_layout != 0) return set_err(c, GAHHG_E_STATE, "autotune: the accelerated query is off");
+  RCK(select_device(c));
+  const int groups[] = {0, 4, 8, 16, 32, 64};  // 0: each polygon's own width
+  const int blocks[] = {64, 128, 256};  // the query kernels' launch bound is 256
+  const gahhg_params keep = c->prm;
+  float best = 0.f;
+  int bg = keep.query_group, bb = keep.block_query;
+  for (int b : blocks)
+    for (int G : groups) {
+      c->prm.block_query = b;
+      c->prm.query_group = G;
+      float ms = 0.f;
+      for (int rep = 0; rep < 3; rep++) {  // first run warms up; keep the faster of two
+        RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
+        const int rc = run_query(c, d_xyz, d_poly, n, d_labels);
+        if (rc) {
+          c->prm = keep;
+          return rc;
+        }
+        RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
+        RCK(hip_check(c, hipEventSynchronize(c->ev[1]), "autotune sync"));
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+        if (rep == 1 || (rep == 2 && t < ms)) ms = t;
+      }
+      if (best == 0.f || ms < best) {
+        best = ms;
+        bg = G;
+        bb = b;
+      }
+    }
+  c->prm.query_group = bg;
+  c->prm.block_query = bb;
+  if (out3) {
+    out3[0] = bg;
+    out3[1] = bb;
+    out3[2] = (int32_t)(best * 1000.0f);  // microseconds
+  }
   return GAHHG_OK;
 }
 

@@ -51,8 +51,30 @@ struct PolyDev {
   int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
   int32_t depth;        // fixed depth for this polygon (0 = density rule)
   int32_t tsub;         // sub-table side S of this polygon's start table (1 = none)
-  int32_t pad;
+  int32_t qg;           // lanes per edge-test group of this polygon's queries
 };
+
+// Adds v to counter[key] for every active lane with one atomic per distinct
+// key of the wave (lanes sharing a key are summed first).  Keys of a
+// contiguous range of cells or edges are mostly equal, so this removes the
+// contention a plain per-lane atomic has on the few counters involved.
+#ifdef __HIPCC__
+template <typename T>
+__device__ __forceinline__ void wave_add(T* counter, int key, T v) {
+  unsigned long long todo = __ballot(1);
+  const int lane = threadIdx.x & 63;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int k = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(key == k) & todo;
+    T s = (same >> lane) & 1ull ? v : (T)0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == leader && k >= 0) atomicAdd(counter + k, s);
+    todo &= ~same;
+  }
+}
+#endif
 
 // Fixed-point scale of the per top-level cell edge sums (2^-40 rad).
 constexpr double kFix = 1099511627776.0;
@@ -167,6 +189,7 @@ struct QueryLayout {
   uint8_t* sf = nullptr;
   int64_t nrec = 0;
   int64_t leaves = 0;  // bottom cells with refs
+  int32_t* pcnt = nullptr;  // per polygon: edge-stream records, leaves with refs
   int64_t bytes = 0;
 };
 
@@ -192,7 +215,13 @@ struct LearnState {
   int32_t* d_dec = nullptr;               // np decisions
   int64_t cap = 0;                        // polygons the arrays hold
   std::vector<int32_t> h_dec;
+  std::vector<int32_t> rule_stop;  // level where the density rule stops, -1 = not yet
 };
+
+// Window of the learned depth decision (crossed cells / N below which a
+// polygon always refines; levels past the density rule's stop it may add).
+constexpr float kLearnFloor = 0.01f;
+constexpr int kLearnAbove = 2;
 
 // Grow-only buffers of the query sort: keys and values (two of each, n
 // entries) followed by the sort's temporary storage.
@@ -292,7 +321,6 @@ int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
-int query_group(const gahhg_ctx* c);
 int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, const int32_t** order);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,

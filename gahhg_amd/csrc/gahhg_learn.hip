@@ -36,19 +36,20 @@ namespace {
 __global__ void k_hint_hist(GridView g, const double* __restrict__ xyz, const int32_t* __restrict__ poly,
                             int64_t n, int npoly, uint32_t* __restrict__ hist) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int p = poly ? poly[t] : 0;
-  if (p < 0 || p >= npoly) return;
-  const V3 q = load3(xyz + 3 * t);
-  for (int i = 0; i < kGrid0Cells; i++) {
-    const int bs = g.c_bstart[i], bn = g.c_bnum[i];
-    bool in = true;
-    for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), q));
-    if (in) {
-      atomicAdd(hist + p * kGrid0Cells + i, 1u);
-      return;
+  int key = -1;
+  if (t < n) {
+    const int p = poly ? poly[t] : 0;
+    if (p >= 0 && p < npoly) {
+      const V3 q = load3(xyz + 3 * t);
+      for (int i = 0; i < kGrid0Cells && key < 0; i++) {
+        const int bs = g.c_bstart[i], bn = g.c_bnum[i];
+        bool in = true;
+        for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), q));
+        if (in) key = p * kGrid0Cells + i;
+      }
     }
   }
+  wave_add<uint32_t>(hist, key, 1u);
 }
 
 // Expected points per (polygon, top-level cell): the hint histogram times
@@ -71,13 +72,17 @@ __global__ void k_weights(int npoly, const PolyDev* __restrict__ pd, const uint3
 __global__ void k_level_stats(GridView g, const PolyDev* __restrict__ pd, int lo, int n, int marked,
                               int only_refining, int32_t* __restrict__ st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int cid = lo + t;
-  const int p = g.c_poly[cid];
-  if (only_refining && !pd[p].refining) return;
-  int32_t* o = st + 2 * (p * kGrid0Cells + g.c_root[cid]);
-  atomicAdd(o, 1);
-  if (marked ? g.c_rnum[cid] != 0 : g.c_subcell[cid] != -1) atomicAdd(o + 1, 1);
+  int key = -1, crossed = 0;
+  if (t < n) {
+    const int cid = lo + t;
+    const int p = g.c_poly[cid];
+    if (!only_refining || pd[p].refining) {
+      key = 2 * (p * kGrid0Cells + g.c_root[cid]);
+      crossed = (marked ? g.c_rnum[cid] != 0 : g.c_subcell[cid] != -1) ? 1 : 0;
+    }
+  }
+  wave_add<int32_t>(st, key, key >= 0 ? 1 : 0);
+  wave_add<int32_t>(st, key >= 0 ? key + 1 : -1, crossed);
 }
 
 // Features of rows (p, r, k), row = (p * 42 + r) * 2 + k.
@@ -113,9 +118,12 @@ __global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd,
 }
 
 // Refine-or-stop per polygon still refining without a fixed depth.
+// The per-level cost is shared by the polygons of a batch that still refine
+// (n_refining), so each is charged its share.
 __global__ void k_decide(int npoly, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
                          const float* __restrict__ Y, const float* __restrict__ w, double query_scale,
-                         double ns_edge, double ns_cell, double ns_level, int32_t* __restrict__ dec) {
+                         double ns_edge, double ns_cell, double ns_level, int n_refining,
+                         int32_t* __restrict__ dec) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npoly) return;
   const PolyDev& P = pd[p];
@@ -129,7 +137,7 @@ __global__ void k_decide(int npoly, const PolyDev* __restrict__ pd, const int32_
     if (st[2 * pr + 1] == 0) continue;  // nothing of r is refined either way
     saving += (double)w[pr] * (exp2((double)Y[2 * pr]) - exp2((double)Y[2 * pr + 1]));
   }
-  const double cost = ns_edge * P.N + ns_cell * ((double)P.U + P.nbc) + ns_level;
+  const double cost = ns_edge * P.N + ns_cell * ((double)P.U + P.nbc) + ns_level / max(n_refining, 1);
   dec[p] = (P.U > 0 && saving * query_scale > cost) ? 1 : 0;
 }
 
@@ -210,12 +218,14 @@ static int learn_features(gahhg_ctx* c, int level, int lo, int n, int marked, in
 int learn_level(gahhg_ctx* c, int level, int lo, int n, int32_t* h_dec) {
   LearnState& L = c->learn;
   const int np = (int)c->polys.size();
+  int n_refining = 0;
+  for (const PolyDev& P : c->pd) n_refining += P.refining != 0;
   int rc = learn_features(c, level, lo, n, 1, 1);
   if (!rc) rc = mlp_forward(c, L.d_x, (int64_t)np * kGrid0Cells * 2, L.d_weights, L.d_y);
   if (rc) return rc;
   hipLaunchKernelGGL(k_decide, dim3(blocks(np, 64)), dim3(64), 0, c->stream, np, (const PolyDev*)c->d_pd,
                      (const int32_t*)L.d_stat, (const float*)L.d_y, (const float*)L.d_w, L.cost[1], L.cost[2],
-                     L.cost[3], L.cost[4], L.d_dec);
+                     L.cost[3], L.cost[4], n_refining, L.d_dec);
   rc = hip_check(c, hipGetLastError(), "decide");
   if (rc) return rc;
   return hip_check(c, hipMemcpyAsync(h_dec, L.d_dec, np * 4, hipMemcpyDeviceToHost, c->stream), "decisions");

@@ -462,25 +462,31 @@ __device__ __forceinline__ unsigned long long group_mask(int lane, int G) {
 
 __global__ void __launch_bounds__(256) k_leaf_count(GridView g, const PolyDev* __restrict__ pd,
                                                     const int32_t* __restrict__ pe, int cnum,
-                                                    int32_t* __restrict__ cnt, DevState* st, int G) {
+                                                    int32_t* __restrict__ cnt, DevState* st,
+                                                    int32_t* __restrict__ pcnt, int G) {
+  // every lane reaches the wave-aggregated adds at the end
   const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   const int lane = threadIdx.x & 63, gl = lane & (G - 1);
   const unsigned long long gm = group_mask(lane, G);
-  if (t >= cnum) return;
-  const int cid = t;
-  const int rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
-  int chains = 0;
-  if (rn > 0) {
-    const int32_t* r = g.refs + g.c_rstart[cid];
-    for (int base = 0; base < rn; base += G) {
-      const int i = base + gl;
-      chains += __popcll(__ballot(i < rn && chain_break(pe, r, rn, i)) & gm);
+  int rn = 0, chains = 0;
+  if (t < cnum) {
+    const int cid = t;
+    rn = is_bottom(g, pd, cid) ? g.c_rnum[cid] : 0;
+    if (rn > 0) {
+      const int32_t* r = g.refs + g.c_rstart[cid];
+      for (int base = 0; base < rn; base += G) {
+        const int i = base + gl;
+        chains += __popcll(__ballot(i < rn && chain_break(pe, r, rn, i)) & gm);
+      }
     }
+    if (gl == 0) cnt[t] = rn > 0 ? rn + chains : 0;
   }
-  if (gl == 0) {
-    cnt[t] = rn > 0 ? rn + chains : 0;
-    if (rn > 0) atomicAdd(&st->nleaf, 1);
-  }
+  const bool leaf = t < cnum && gl == 0 && rn > 0;
+  const int key = leaf ? 2 * g.c_poly[t] : -1;
+  wave_add<int32_t>(pcnt, key, leaf ? rn + chains : 0);
+  wave_add<int32_t>(pcnt, leaf ? key + 1 : -1, 1);
+  const unsigned long long w = __ballot(leaf);
+  if (lane == 0 && w) atomicAdd(&st->nleaf, (int32_t)__popcll(w));
 }
 
 // Centre and edge stream of every bottom cell with refs (G lanes per cell):
@@ -540,6 +546,49 @@ __global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* _
   }
 }
 
+// Edge tests of a wave's boundary points, by groups of G lanes: each group
+// resolves the points of its own lanes one at a time, its G lanes splitting
+// the leaf's edge stream (coalesced loads), and the crossing count is the
+// popcount of the group's part of a ballot.  Every condition below is
+// uniform within a group, so a group's lanes stay active together.
+template <int G>
+__device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, int64_t q,
+                                           const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                           const double* __restrict__ sv, const double* __restrict__ sn,
+                                           const uint8_t* __restrict__ sf, int32_t* __restrict__ labels) {
+  const int gbase = lane & ~(G - 1);
+  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+  unsigned long long mine = __ballot(leaf >= 0) & gmask;
+  while (mine) {
+    const int src = __ffsll((long long)mine) - 1;
+    mine &= mine - 1;
+    const int lt = __shfl(leaf, src, 64);
+    const V3 pp = v3(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64));
+    const V3 c = load3(qc + 3 * (int64_t)lt);
+    const V3 nq = arc_normal(c, pp);
+    const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
+    int cnt = 0;
+    for (int base = 0; base < m - 1; base += G) {
+      const int k = base + (lane - gbase);
+      bool cross = false;
+      if (k < m - 1) {
+        const int rk = r0 + k;
+        const int fl = sf[rk];
+        if (fl & 4) {
+          const double ra = dot(nq, load3(sv + 3 * (int64_t)rk));
+          const double rb = dot(nq, load3(sv + 3 * (int64_t)(rk + 1)));
+          if (!same_side(ra, rb)) {
+            const double r3 = dot(load3(sn + 3 * (int64_t)rk), pp);
+            cross = !(((fl & 1) && lt0(r3)) || ((fl & 2) && gt0(r3)));
+          }
+        }
+      }
+      cnt += __popcll(__ballot(cross) & gmask);
+    }
+    if (lane == src) labels[q] = (cnt & 1) ? 3 - type : type;
+  }
+}
+
 // Table-accelerated query; labels identical to k_query_soa.
 // Each lane locates its own point (table start cell, then the reference's
 // first-match descent).  Points that end in a leaf with refs are then
@@ -550,7 +599,7 @@ __global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* _
 // its top-level cell, its level, level of the table start cell (-1 = level-0
 // scan, -2 = resolved by the table), border planes evaluated, leaf edge
 // records scanned}; labels are unchanged.
-template <bool TRACE, int G>
+template <bool TRACE>
 __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
                                                    const int32_t* __restrict__ table,
                                                    const int32_t* __restrict__ sub, int B,
@@ -560,7 +609,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
                                                    const int32_t* __restrict__ qpoly,
                                                    const int32_t* __restrict__ order, int64_t n,
                                                    int32_t* __restrict__ labels, LevelTab lt,
-                                                   int32_t* __restrict__ trace) {
+                                                   int32_t* __restrict__ trace, int G_override) {
   // thread t handles point order[t] (points grouped by polygon and start
   // bin, so that a wave's gathers share cache lines), or point t
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -624,41 +673,21 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
       tr[5] = leaf >= 0 ? hdr[2 * leaf + 1] : 0;
     }
   }
-  // Edge tests of this wave's boundary points, by groups of G lanes: each
-  // group resolves the points of its own lanes one at a time, its G lanes
-  // splitting the leaf's edge stream (coalesced loads), and the crossing
-  // count is the popcount of the group's part of a ballot.  Every condition
-  // below is uniform within a group, so a group's lanes stay active together.
-  const int gbase = lane & ~(G - 1);
-  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
-  unsigned long long mine = __ballot(leaf >= 0) & gmask;
-  while (mine) {
-    const int src = __ffsll((long long)mine) - 1;
-    mine &= mine - 1;
-    const int lt = __shfl(leaf, src, 64);
-    const V3 pp = v3(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64));
-    const V3 c = load3(qc + 3 * (int64_t)lt);
-    const V3 nq = arc_normal(c, pp);
-    const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
-    int cnt = 0;
-    for (int base = 0; base < m - 1; base += G) {
-      const int k = base + (lane - gbase);
-      bool cross = false;
-      if (k < m - 1) {
-        const int rk = r0 + k;
-        const int fl = sf[rk];
-        if (fl & 4) {
-          const double ra = dot(nq, load3(sv + 3 * (int64_t)rk));
-          const double rb = dot(nq, load3(sv + 3 * (int64_t)(rk + 1)));
-          if (!same_side(ra, rb)) {
-            const double r3 = dot(load3(sn + 3 * (int64_t)rk), pp);
-            cross = !(((fl & 1) && lt0(r3)) || ((fl & 2) && gt0(r3)));
-          }
-        }
-      }
-      cnt += __popcll(__ballot(cross) & gmask);
-    }
-    if (lane == src) labels[q] = (cnt & 1) ? 3 - type : type;
+  // lanes per edge-test group: the smallest of the wave's polygons' widths
+  // (points arrive grouped by polygon, so a wave rarely spans two); the
+  // width decides only how the work is split, never a label
+  int gw = 64;
+  if (t < n) gw = G_override > 0 ? G_override : pd[qpoly ? qpoly[q] : 0].qg;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gw = min(gw, __shfl_xor(gw, o, 64));
+  switch (gw) {
+    case 1: leaf_tests<1>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 2: leaf_tests<2>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 4: leaf_tests<4>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 8: leaf_tests<8>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 16: leaf_tests<16>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 32: leaf_tests<32>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    default: leaf_tests<64>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
   }
 }
 
@@ -753,9 +782,12 @@ int pack_query_layout(gahhg_ctx* c) {
   int G = 1;
   while (G < 64 && G < (bottom > 0 ? 0.25 * refs / bottom : 1.0)) G *= 2;
   const unsigned wb = (unsigned)((nc * G + 255) / 256);
+  if (dalloc(c, (void**)&q.pcnt, (int64_t)np * 8)) return GAHHG_E_NOMEM;
+  int rc = fill_i32(c, q.pcnt, (int64_t)np * 2, 0);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_leaf_count, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
-                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st, G);
-  int rc = hip_check(c, hipGetLastError(), "leaf count");
+                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st, q.pcnt, G);
+  rc = hip_check(c, hipGetLastError(), "leaf count");
   if (rc) return rc;
   rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
   if (rc) return rc;
@@ -764,6 +796,23 @@ int pack_query_layout(gahhg_ctx* c) {
   if (rc) return rc;
   q.nrec = std::max<int32_t>(c->h_st->nrec, 1);
   q.leaves = c->h_st->nleaf;
+  // lanes per edge-test group of each polygon's queries: the smallest power
+  // of two >= half its mean edge-stream records per leaf, in [4, 64]
+  {
+    std::vector<int32_t> pc((size_t)np * 2);
+    rc = hip_check(c, hipMemcpy(pc.data(), q.pcnt, np * 8, hipMemcpyDeviceToHost), "leaf counts");
+    if (rc) return rc;
+    for (int p = 0; p < np; p++) {
+      const double mean = pc[2 * p + 1] > 0 ? (double)pc[2 * p] / pc[2 * p + 1] : 1.0;
+      int Gp = 4;
+      while (Gp < 64 && Gp < 0.5 * mean) Gp *= 2;
+      c->pd[p].qg = Gp;
+    }
+    rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                     c->stream),
+                   "polygon upload");
+    if (rc) return rc;
+  }
   if (dalloc(c, (void**)&q.qc, nc * 24) || dalloc(c, (void**)&q.hdr, nc * 8) ||
       dalloc(c, (void**)&q.sv, q.nrec * 24) || dalloc(c, (void**)&q.sn, q.nrec * 24) ||
       dalloc(c, (void**)&q.sf, q.nrec))
@@ -859,37 +908,16 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
   return GAHHG_OK;
 }
 
-// Lanes per edge-test group: the context's setting, or (0) the smallest
-// power of two >= half the mean records of a leaf with refs, in [4, 64].
-int query_group(const gahhg_ctx* c) {
-  if (c->prm.query_group > 0) return c->prm.query_group;
-  const double mean = c->q.leaves > 0 ? (double)c->q.nrec / (double)c->q.leaves : 64.0;
-  int G = 4;
-  while (G < 64 && G < 0.5 * mean) G *= 2;
-  return G;
-}
-
 template <bool TRACE>
-static void launch_query(gahhg_ctx* c, int G, const double* d_xyz, const int32_t* d_poly, const int32_t* d_order,
+static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, const int32_t* d_order,
                          int64_t n, int32_t* d_labels, int32_t* d_trace) {
   const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
   const QueryLayout& L = c->q;
-#define GAHHG_QUERY_LAUNCH(GG)                                                                                   \
-  hipLaunchKernelGGL((k_query<TRACE, GG>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),      \
-                     (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,                     \
-                     L.table ? L.bins : 0, (const double*)L.qc,                                                   \
-                     (const int32_t*)L.hdr, (const double*)L.sv, (const double*)L.sn, (const uint8_t*)L.sf,     \
-                     d_xyz, d_poly, d_order, n, d_labels, level_tab(c), d_trace)
-  switch (G) {
-    case 1: GAHHG_QUERY_LAUNCH(1); break;
-    case 2: GAHHG_QUERY_LAUNCH(2); break;
-    case 4: GAHHG_QUERY_LAUNCH(4); break;
-    case 8: GAHHG_QUERY_LAUNCH(8); break;
-    case 16: GAHHG_QUERY_LAUNCH(16); break;
-    case 32: GAHHG_QUERY_LAUNCH(32); break;
-    default: GAHHG_QUERY_LAUNCH(64); break;
-  }
-#undef GAHHG_QUERY_LAUNCH
+  hipLaunchKernelGGL((k_query<TRACE>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                     (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
+                     L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const double*)L.sv,
+                     (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_labels, level_tab(c),
+                     d_trace, c->prm.query_group);
 }
 
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {
@@ -909,7 +937,7 @@ int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t 
       if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[2], c->stream);
-    launch_query<false>(c, query_group(c), d_xyz, d_poly, order, n, d_labels, nullptr);
+    launch_query<false>(c, d_xyz, d_poly, order, n, d_labels, nullptr);
     (void)hipEventRecord(c->ev[3], c->stream);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
@@ -928,7 +956,7 @@ int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, in
                     int32_t* d_trace) {
   if (n <= 0) return GAHHG_OK;
   if (!c->q.hdr) return set_err(c, GAHHG_E_STATE, "trace: the accelerated query layout is not built");
-  launch_query<true>(c, query_group(c), d_xyz, d_poly, nullptr, n, d_labels, d_trace);
+  launch_query<true>(c, d_xyz, d_poly, nullptr, n, d_labels, d_trace);
   return hip_check(c, hipGetLastError(), "traced query kernel");
 }
 

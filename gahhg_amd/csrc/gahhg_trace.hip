@@ -23,23 +23,27 @@ __device__ __forceinline__ int level_of_cell(const LevelTab& lt, int cid) {
 }
 
 // out[((p * kMaxLevels + l) * 42 + r) * 4 + k], k = {cells, refined cells,
-// bottom cells with refs, refs of those cells}.  LDS-free: one atomic per
-// counted quantity (a trace, not a hot path).
+// bottom cells with refs, refs of those cells}; wave-aggregated adds (every
+// lane reaches them, lanes past the end add to key -1).
 __global__ void k_cell_trace(GridView g, const PolyDev* __restrict__ pd, LevelTab lt, int64_t C,
                              int32_t* __restrict__ out) {
   const int64_t cid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cid >= C) return;
-  const int p = g.c_poly[cid];
-  const int l = level_of_cell(lt, (int)cid);
-  const int r = g.c_root[cid];
-  int32_t* o = out + (((int64_t)p * kMaxLevels + l) * kGrid0Cells + r) * 4;
-  atomicAdd(o + 0, 1);
-  if (g.c_subcell[cid] != -1) atomicAdd(o + 1, 1);
-  const PolyDev& P = pd[p];
-  if (cid >= P.bcs && cid < P.bcs + P.bcn && g.c_rnum[cid] > 0) {
-    atomicAdd(o + 2, 1);
-    atomicAdd(o + 3, g.c_rnum[cid]);
+  int key = -1, refined = 0, leaf = 0, refs = 0;
+  if (cid < C) {
+    const int p = g.c_poly[cid];
+    const int l = level_of_cell(lt, (int)cid);
+    key = (int)((((int64_t)p * kMaxLevels + l) * kGrid0Cells + g.c_root[cid]) * 4);
+    refined = g.c_subcell[cid] != -1 ? 1 : 0;
+    const PolyDev& P = pd[p];
+    if (cid >= P.bcs && cid < P.bcs + P.bcn && g.c_rnum[cid] > 0) {
+      leaf = 1;
+      refs = g.c_rnum[cid];
+    }
   }
+  wave_add<int32_t>(out, key, key >= 0 ? 1 : 0);
+  wave_add<int32_t>(out, key >= 0 ? key + 1 : -1, refined);
+  wave_add<int32_t>(out, key >= 0 ? key + 2 : -1, leaf);
+  wave_add<int32_t>(out, key >= 0 ? key + 3 : -1, refs);
 }
 
 }  // namespace
@@ -51,30 +55,37 @@ __global__ void k_cell_trace(GridView g, const PolyDev* __restrict__ pd, LevelTa
 __global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ edge,
                              const int32_t* __restrict__ epoly, const int32_t* __restrict__ pe, int64_t Nt,
                              unsigned long long* __restrict__ out) {
+  // every lane reaches the wave-aggregated adds (invalid lanes add to key -1)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= Nt) return;
-  const int p = epoly[e];
-  const PolyDev& P = pd[p];
-  const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
-  int root = -1;
-  for (int i = 0; i < kGrid0Cells && root < 0; i++) {
-    const int cid = P.cell0 + i;
-    const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
-    bool in = true;
-    for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), p0));
-    if (in) root = i;
+  int key = -1;
+  unsigned long long len_fx = 0, turn_fx = 0;
+  if (e < Nt) {
+    const int p = epoly[e];
+    const PolyDev& P = pd[p];
+    const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
+    int root = -1;
+    for (int i = 0; i < kGrid0Cells && root < 0; i++) {
+      const int cid = P.cell0 + i;
+      const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+      bool in = true;
+      for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), p0));
+      if (in) root = i;
+    }
+    if (root >= 0) {
+      const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
+      double turn = 0.0;
+      if (e + 1 < Nt && epoly[e + 1] == p && pe[2 * e + 1] == pe[2 * (e + 1)]) {
+        const V3 n2 = load3(edge + 9 * (e + 1) + 6);
+        turn = acos(fmin(1.0, fmax(-1.0, dot(ne, n2))));
+      }
+      key = (p * kGrid0Cells + root) * 3;
+      len_fx = (unsigned long long)(len * kFix);
+      turn_fx = (unsigned long long)(turn * kFix);
+    }
   }
-  if (root < 0) return;
-  const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
-  double turn = 0.0;
-  if (e + 1 < Nt && epoly[e + 1] == p && pe[2 * e + 1] == pe[2 * (e + 1)]) {
-    const V3 n2 = load3(edge + 9 * (e + 1) + 6);
-    turn = acos(fmin(1.0, fmax(-1.0, dot(ne, n2))));
-  }
-  unsigned long long* o = out + ((int64_t)p * kGrid0Cells + root) * 3;
-  atomicAdd(o + 0, 1ULL);
-  atomicAdd(o + 1, (unsigned long long)(len * kFix));
-  atomicAdd(o + 2, (unsigned long long)(turn * kFix));
+  wave_add<unsigned long long>(out, key, key >= 0 ? 1ull : 0ull);
+  wave_add<unsigned long long>(out, key >= 0 ? key + 1 : -1, len_fx);
+  wave_add<unsigned long long>(out, key >= 0 ? key + 2 : -1, turn_fx);
 }
 
 int edge_stats(gahhg_ctx* c, unsigned long long* d_out) {

@@ -249,7 +249,11 @@ def fit_build_cost(recs):
     return [float(c) for c in coef]
 
 
-def _group_loss_grad(y, S, T):
+def _group_loss_grad(y, S, T, lam=4.0):
+    """Loss over groups (consecutive pairs are the stop / refine groups of
+    one polygon and level): squared log2 error of each group, plus lam times
+    the squared error of each pair's difference -- the quantity the build's
+    decision turns on."""
     ng = len(T)
     y = y.astype(np.float64).reshape(ng, 42)
     ymax = y.max(axis=1, keepdims=True)
@@ -257,8 +261,16 @@ def _group_loss_grad(y, S, T):
     P = e.sum(axis=1)
     pred = np.log2(np.maximum(P, 1e-300)) + ymax[:, 0]
     r = pred - T
-    dy = (2.0 * r / ng)[:, None] * e / np.maximum(P, 1e-300)[:, None]
-    return float(np.mean(r * r)), dy
+    g = 2.0 * r / ng
+    loss = float(np.mean(r * r))
+    npair = ng // 2
+    if npair and lam > 0:
+        d = (pred[1::2] - pred[0::2]) - (T[1::2] - T[0::2])
+        loss += lam * float(np.mean(d * d))
+        g[1::2] += 2.0 * lam * d / npair
+        g[0::2] -= 2.0 * lam * d / npair
+    dy = g[:, None] * e / np.maximum(P, 1e-300)[:, None]
+    return loss, dy
 
 
 def _shares(G):
@@ -314,6 +326,59 @@ def predict_groups(ctx, params, G):
     return np.log2(np.maximum((S * np.exp2(y)).sum(axis=1), 1e-300))
 
 
+K_FLOOR = 0.01   # the build's window: refine while crossed cells / N < K_FLOOR
+K_ABOVE = 2      # at most this many levels past the density rule's stop
+
+
+def simulate_depth(y, U, crossed, N, w_total, cells_next, cost):
+    """The build's decision sequence for one polygon from per-level model
+    outputs y[l][r][k], crossed cells U[l] (crossed[l][r] per top-level
+    cell), cells of level l + 1 and the
+    cost constants: returns the depth the build would pick."""
+    rs = -1
+    for l in range(len(y)):
+        k = U[l] / N
+        if rs < 0 and (k > 0.1 or (k < 0.1 and 0.1 - k < 0.01)):
+            rs = l
+        if k < K_FLOOR:
+            continue
+        if rs >= 0 and l >= rs + K_ABOVE:
+            return l + 1
+        live = crossed[l] > 0  # top-level cells with nothing crossed keep their cost
+        saving = float(np.sum((w_total[l] * (np.exp2(y[l][:, 0]) - np.exp2(y[l][:, 1])))[live]))
+        if saving <= cost[1] * N + cost[2] * cells_next[l] + cost[3]:
+            return l + 1
+    return len(y) + 1
+
+
+def evaluate(ctx, params, cost, recs, ids, Q):
+    """A/B of the learned depth against the density rule on polygons the
+    model did not train on: total time build + Q points per polygon (ms),
+    from the measured build and query times of each depth."""
+    m = DeviceMLP(ctx, params=params)
+    out = []
+    for ri in ids:
+        r = recs[ri]
+        X = r["X"]
+        if X.shape[0] == 0:
+            continue
+        y = m.forward(X.reshape(-1, N_IN)).astype(np.float64).reshape(X.shape[0], 42, 2)
+        Ur = np.rint(np.exp2(X[:, :, 0, 3].astype(np.float64) * 20.0) - 1.0)
+        U = Ur.sum(axis=1)
+        shares = X[:, :, 0, 11].astype(np.float64)
+        deep = max(r["cells"])
+        cells_next = [r["cells"][deep][l + 1] if l + 1 < len(r["cells"][deep]) else 0 for l in range(X.shape[0])]
+        d = simulate_depth(y, U, Ur, r["N"], shares * Q, cells_next, [1.0, cost[0], cost[1], cost[2]])
+        if d not in r["t"]:
+            continue
+        tot = {dd: r["b"][dd] + Q * r["t"][dd] * 1e-6 for dd in r["t"]}
+        best = min(tot, key=tot.get)
+        out.append(dict(N=r["N"], rule=r["rule"], learned=d, best=best, rule_ms=tot[r["rule"]],
+                        learned_ms=tot[d], best_ms=tot[best]))
+    m.free()
+    return out
+
+
 def main(argv=None):
     import argparse
     import sys
@@ -347,9 +412,21 @@ def main(argv=None):
     tv = np.log2([g[2] for g in Gv])
     rmse = float(np.sqrt(np.mean((pv - tv) ** 2))) if len(Gv) else None
     log(f"[train] held-out groups {len(Gv)}: rms log2 error {rmse}")
+    cost_v = [cost[0], cost[1], cost[2]]
+    ab = {}
+    for Q in (1e5, 1e6, 1e7, 1e8):
+        ev = evaluate(ctx, params, cost_v, recs, sorted(hold), Q)
+        if ev:
+            rt = sum(e["rule_ms"] for e in ev)
+            lt = sum(e["learned_ms"] for e in ev)
+            bt = sum(e["best_ms"] for e in ev)
+            ab[str(int(Q))] = {"polygons": len(ev), "rule_ms": round(rt, 3), "learned_ms": round(lt, 3),
+                               "best_ms": round(bt, 3), "learned_vs_rule": round(lt / rt, 4),
+                               "depth_changes": sum(e["learned"] != e["rule"] for e in ev)}
+            log(f"[train] held-out A/B at {Q:.0e} points per polygon: {ab[str(int(Q))]}")
     ctx.close()
     meta = {"training_polygons": len(polys), "groups": len(G), "held_out_groups": len(Gv),
-            "validation_rms_log2_error": vbest, "validation_best_step": vstep,
+            "validation_rms_log2_error": vbest, "validation_best_step": vstep, "held_out_ab": ab,
             "held_out_rms_log2_error": rmse, "loss_history": hist, "points_per_polygon": args.points,
             "records": [{"N": r["N"], "rule": r["rule"], "ns_per_point": r["t"], "build_ms": r["b"]} for r in recs]}
     save_model(args.out, params, {"ns_edge": cost[0], "ns_cell": cost[1], "ns_level": cost[2]}, meta)

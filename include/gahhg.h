@@ -45,15 +45,16 @@ typedef struct {
   int32_t auto_depth;      /* 1: stop refining once crossed cells / N > ~0.09 */
   int32_t block_create;    /* threads per block of build kernels              */
   int32_t block_classify;  /* threads per block of classification kernels     */
-  int32_t block_query;     /* threads per block of the query kernel           */
+  int32_t block_query;     /* threads per block of the query kernel (64..256,
+                              a multiple of 64)                              */
   int32_t device;          /* HIP device ordinal                              */
   int32_t verbose;         /* 1: print per-level progress like the reference  */
   int32_t query_layout;    /* 0: table-accelerated query (default), 1: plain
                               descent from level 0 over the build arrays     */
   int32_t query_bins;      /* start-cell table bins per cube face (0 = none)  */
   int32_t query_group;     /* lanes per edge-test group of the accelerated
-                              query: 1,2,4,...,64, or 0 = from the mean leaf
-                              edge-stream length                             */
+                              query: 1,2,4,...,64, or 0 = per polygon, from
+                              its mean leaf edge-stream length               */
   int32_t query_sub;       /* largest sub-table side S of the start table
                               (power of two <= 256; 0 or 1 = none, default) */
   int32_t query_sub_budget;/* sub-table entries allowed per polygon edge     */
@@ -156,6 +157,16 @@ int gahhg_export_polygon_grid(gahhg_ctx* ctx, int32_t p, double* v, int32_t* cel
  * effect at the next gahhg_build. */
 int gahhg_set_polygon_depths(gahhg_ctx* ctx, const int32_t* depths, int32_t n);
 
+/* Autotuner (no counterpart in the reference): times the accelerated query
+ * on the given device points (a representative sample) for every lane-group
+ * width {per polygon, 4, 8, 16, 32, 64} (edge records tested together per
+ * point; reported as 0 for per polygon) and
+ * block size {64, 128, 256}, keeps the fastest in the context's parameters
+ * and writes {group, block, microseconds} to out3 (may be NULL).  d_labels
+ * receives the labels (the same for every setting). */
+int gahhg_autotune_query(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                         int32_t* out3);
+
 /* Trace recorder (no counterpart in the reference).  Query trace: the
  * accelerated query of gahhg_query_batch_xyz_device (same labels) that also
  * writes 6 int32 per point to d_trace: {leaf cell of the first-match
@@ -196,11 +207,15 @@ int gahhg_mlp_adam(gahhg_ctx* ctx, float* d_master, const float* d_grad, float* 
  * fixed depth is refined level by level while the model's predicted query
  * time saved exceeds the predicted build time of the next level:
  *   sum_r w_r (2^y_r(stop) - 2^y_r(refine)) * cost[1]
- *       > cost[2] * N + cost[3] * (cells of the next level) + cost[4]   (ns)
+ *       > cost[2] * N + cost[3] * (cells of the next level) + cost[4] / m   (ns)
  * over the top-level cells r of the polygon, with w_r the points expected
  * in r (from the query hint, else cost[0] points per polygon spread by edge
- * share) and y_r(k) the model's log2 query time per point (ns) with
- * refinement stopping at this level (k = 0) or after one more (k = 1).
+ * share), y_r(k) the model's log2 query time per point (ns) with
+ * refinement stopping at this level (k = 0) or after one more (k = 1), and
+ * m the polygons of the batch still refining (they share a level's cost).
+ * The model decides only inside the window it is trained on: a polygon
+ * always refines while its crossed cells are below 0.01 N, and refines at
+ * most 2 levels past the level where the density rule stops.
  * The hierarchy keeps the reference's structure (one bottom level per
  * polygon); max_depth still caps it.  params: GAHHG_MLP_PARAMS values as
  * for gahhg_mlp_*, rounded to BF16 here; NULL switches the model off.

@@ -1,8 +1,10 @@
 """Learned refinement depth in the build.
 
 * Decision rule: a model whose prediction does not depend on stop/refine
-  never refines past level 0; one that predicts a large saving refines to
-  max_depth.  The density rule is untouched when no model is set.
+  stops as soon as its window opens (never deeper than the density rule);
+  one that predicts a large saving refines to the top of the window (2
+  levels past the density rule's stop).  The density rule is untouched when
+  no model is set.
 * Whatever depth the model picks, the hierarchy and the labels equal the
   oracle's built at that fixed depth (0 mismatches, same hierarchy).
 * The features a build uses are the ones the post-build entry point
@@ -49,20 +51,24 @@ def items(n, seed):
 def test_decision_extremes():
     ps, it = items(3, 21)
     cost = [1e6, 1.0, 1.0, 1.0, 1000.0]
-    g = gahhg_amd.GPISP(device=0, max_depth=7)
+    rule = [ol.Oracle().set_lonlat(*x).build(max_depth=9).counts()["depth"] for x in it]
+    g = gahhg_amd.GPISP(device=0, max_depth=9)
+    # no predicted saving: stops as soon as the window opens, never deeper
+    # than the density rule
     g.set_polygons_lonlat(it).set_depth_model(const_model(3.0), cost).build()
-    assert [g.polygon_stats(k)["depth"] for k in range(3)] == [1, 1, 1]
+    d0 = [g.polygon_stats(k)["depth"] for k in range(3)]
+    assert all(1 <= a <= b for a, b in zip(d0, rule)), (d0, rule)
+    # a large predicted saving: refines to the top of the window, 2 levels
+    # past the density rule's stop (max_depth caps it)
     g.set_depth_model(refine_model(8.0), cost).build()
-    d = [g.polygon_stats(k)["depth"] for k in range(3)]
-    assert d == [7, 7, 7], d
+    d1 = [g.polygon_stats(k)["depth"] for k in range(3)]
+    assert d1 == [min(r + 2, 9) for r in rule], (d1, rule)
     # model off: the density rule
     g.set_depth_model(None, None).build()
-    for k in range(3):
-        o = ol.Oracle().set_lonlat(*it[k]).build(max_depth=7)
-        assert g.polygon_stats(k)["depth"] == o.counts()["depth"]
+    assert [g.polygon_stats(k)["depth"] for k in range(3)] == rule
     # fixed depths still win over the model
     g.set_polygons_lonlat(it).set_polygon_depths([2, 0, 4]).set_depth_model(refine_model(8.0), cost).build()
-    assert [g.polygon_stats(k)["depth"] for k in range(3)] == [2, 7, 4]
+    assert [g.polygon_stats(k)["depth"] for k in range(3)] == [2, min(rule[1] + 2, 9), 4]
     g.close()
 
 

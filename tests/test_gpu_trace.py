@@ -129,3 +129,30 @@ def test_depth_argument_errors():
     with pytest.raises(gahhg_amd.GahhgError):
         g.set_polygon_depths([-1])
     g.close()
+
+
+def test_autotune_keeps_labels():
+    say("autotune: 5-polygon batch")
+    ps = datasets.batch_polygons(n_poly=5, seed=31)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygons_lonlat(items).build()
+    rng = np.random.default_rng(4)
+    qx, qp = [], []
+    for k, p in enumerate(ps):
+        lon, lat = datasets.cap_queries(rng, p["centre"][0], p["centre"][1], 1.3 * p["radius"], 60_000)
+        qx.append(gahhg_amd.query_lonlat_to_xyz(lon, lat))
+        qp.append(np.full(len(lon), k, dtype=np.int32))
+    xyz = np.concatenate(qx)
+    poly = np.concatenate(qp)
+    ref = g.query_batch_xyz(xyz, poly)
+    dx, dp, dl = g.alloc(xyz.nbytes), g.alloc(4 * len(poly)), g.alloc(4 * len(poly))
+    dx.upload(xyz)
+    dp.upload(poly)
+    G, B, us = g.autotune_device(dx, dp, len(poly), dl)
+    assert G in (0, 4, 8, 16, 32, 64) and B in (64, 128, 256) and us > 0
+    assert (dl.download(np.empty(len(poly), dtype=np.int32)) == ref).all()
+    assert (g.query_batch_xyz(xyz, poly) == ref).all()  # with the tuned settings
+    for b in (dx, dp, dl):
+        b.free()
+    g.close()
