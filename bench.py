@@ -50,12 +50,25 @@ def load_depth_model(args):
     return params, cost
 
 
-def apply_model(g, model, xyz, poly, Q, sample=1_000_000):
+def apply_model(g, model, xyz, poly, Q, sample=262_144):
     """Query hint (every k-th point standing for k points) and the model."""
     params, cost = model
     k = max(1, len(xyz) // sample)
     g.set_query_hint(xyz[::k], None if poly is None else poly[::k], float(k))
     g.set_depth_model(params, [float(Q), 1.0, cost["ns_edge"], cost["ns_cell"], cost["ns_level"]])
+
+
+def copy_256mib(g):
+    """One device-to-device copy of 256 MiB (the runtime's copy kernel reads
+    and writes exactly 256 MiB): the counter reference for traffic."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = g.alloc(256 << 20), g.alloc(256 << 20)
+    rc = hip.hipMemcpyDtoD(b.ptr, a.ptr, ctypes.c_size_t(256 << 20))
+    hip.hipDeviceSynchronize()
+    log(f"calibration copy: 256 MiB device-to-device, rc {rc}")
+    a.free()
+    b.free()
 
 
 def median_build(g, n):
@@ -83,6 +96,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--rule", action="store_true", help="density-rule depth only (no learned model)")
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a known-size reference for "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -116,6 +132,8 @@ def main():
     d_xyz = g.alloc(xyz.nbytes)
     d_lab = g.alloc(4 * Q)
     d_xyz.upload(xyz)
+    if args.pmc_calibrate:
+        copy_256mib(g)
 
     # parity spot check against the oracle on a sample (checker only)
     sys.path.insert(0, os.path.join(ROOT, "tests"))

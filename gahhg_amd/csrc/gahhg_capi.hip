@@ -569,8 +569,7 @@ int gahhg_set_query_hint(gahhg_ctx* c, const double* xyz, const int32_t* poly, i
   }
   L.hint_n = n;
   L.hint_weight = weight;
-  c->built = false;
-  return GAHHG_OK;
+  return GAHHG_OK;  // the built hierarchy stays valid: the hint steers the next build
 }
 
 int gahhg_depth_features(gahhg_ctx* c, int32_t level, float* x) {

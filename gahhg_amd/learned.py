@@ -19,7 +19,8 @@ Features per (polygon, top-level cell r, level l, refine k), all O(1):
   6  log2(1 + N_r / max(U_r, 1)) / 10
   7  log2(mean edge length / cell size at l) / 10, cell size 0.55 / 2^l rad
   8  mean turning angle / pi
-  9  U_r / max(C_r, 1)
+  9  share of r's hint points in crossed cells of level l (the boundary
+     band); U_r / max(C_r, 1) when there is no hint
   10 log2(1 + N) / 20            N: edges of the polygon
   11 w_r                          share of the polygon's points in r (edge
                                  share N_r / N when no points are expected)
@@ -152,7 +153,8 @@ def load_model(path=MODEL_PATH):
 # ---------------------------------------------------------------------------
 # Training data: measured build and query times of fixed-depth hierarchies
 # ---------------------------------------------------------------------------
-def training_polygons(seed=101, n_batch=120, rings=(30_000, 60_000, 120_000, 250_000, 500_000, 800_000, 1_200_000)):
+def training_polygons(seed=101, n_batch=120, rings=(30_000, 60_000, 120_000, 250_000, 500_000, 800_000, 1_200_000,
+                                                   700_000, 1_000_000, 1_400_000, 1_700_000, 2_000_000, 900_000)):
     """Training set (disjoint from the benchmark workloads by seed): polygons
     of the batch generator and fractal rings of several sizes, each with the
     point distribution of its benchmark family."""
@@ -167,54 +169,68 @@ def training_polygons(seed=101, n_batch=120, rings=(30_000, 60_000, 120_000, 250
     return out
 
 
-def sample_points(poly, n, seed):
+def sample_points(poly, n, seed, uniform=0.7):
     from . import datasets as ds, query_lonlat_to_xyz
     rng = np.random.default_rng(seed)
     if poly["kind"] == "cap":
         lon, lat = ds.cap_queries(rng, poly["centre"][0], poly["centre"][1], 1.3 * poly["radius"], n)
     else:
-        lon, lat = ds.mixed_queries(rng, poly["lon"], poly["lat"], poly["ring_off"], n, 0.7)
+        lon, lat = ds.mixed_queries(rng, poly["lon"], poly["lat"], poly["ring_off"], n, uniform)
     return query_lonlat_to_xyz(lon, lat)
 
 
-def collect(polys, n_points=2_000_000, span=(-3, 2), reps=3, log=print):
+def collect(polys, n_points=2_000_000, span=(-3, 2), reps=3, mixes=(0.5, 0.7, 0.9), log=print):
     """Per polygon: the density-rule depth D, then for every fixed depth d in
     [D + span[0], D + span[1]] (1..15) the build time (ms) and the query time
     per point (ns, points grouped as in production), and the model features
-    of every level of the deepest hierarchy with the sample as query hint."""
+    of every level of the deepest hierarchy with the sample as query hint.
+    Rings are timed under several point mixes (uniform share), one record
+    each: the hierarchy is the same, the points and hence the features and
+    times differ."""
     from . import GPISP
     recs = []
     for i, poly in enumerate(polys):
-        xyz = sample_points(poly, n_points, 7000 + i)
+        mix_list = mixes if poly["kind"] == "mixed" else (None,)
+        samples = [sample_points(poly, n_points, 7000 + i + 100 * j, u if u is not None else 0.7)
+                   for j, u in enumerate(mix_list)]
         g = GPISP(device=0, query_sort=2)
         g.set_polygon_lonlat(poly["lon"], poly["lat"], poly["ring_off"]).build()
         D = g.stats()["depth"]
-        dx = g.alloc(xyz.nbytes)
-        dl = g.alloc(4 * len(xyz))
-        dx.upload(xyz)
+        dxs = []
+        for xyz in samples:
+            dx = g.alloc(xyz.nbytes)
+            dx.upload(xyz)
+            dxs.append(dx)
+        dl = g.alloc(4 * n_points)
         ds_ = list(range(max(1, D + span[0]), min(15, D + span[1]) + 1))
-        t, b, cells = {}, {}, {}
+        t = [dict() for _ in samples]
+        b, cells = {}, {}
         for d in ds_:
             g.set_polygon_lonlat(poly["lon"], poly["lat"], poly["ring_off"]).set_polygon_depths([d])
-            g.set_query_hint(xyz[::16], None, 16.0)
-            bt, qt = [], []
+            bt = []
             for _ in range(reps + 1):
                 g.build()
                 bt.append(g.stats()["pre_ms"])
-            for _ in range(reps + 1):
-                g.query_device(dx, len(xyz), dl)
-                qt.append(g.stats()["query_ms"])
             b[d] = float(np.median(bt[1:]))
-            t[d] = float(np.median(qt[1:])) * 1e6 / len(xyz)
+            for j, xyz in enumerate(samples):
+                qt = []
+                for _ in range(reps + 1):
+                    g.query_device(dxs[j], len(xyz), dl)
+                    qt.append(g.stats()["query_ms"])
+                t[j][d] = float(np.median(qt[1:])) * 1e6 / len(xyz)
             tr, _ = g.build_trace()
             cells[d] = tr[0, :, :, 0].sum(axis=1).tolist()
-        # features of the deepest hierarchy (levels below its bottom)
-        X = np.stack([g.depth_features(l)[0] for l in range(max(ds_) - 1)]) if max(ds_) > 1 else np.zeros((0, 42, 2, 12))
         N = len(poly["lon"]) - (len(poly["ring_off"]) - 1)
-        recs.append(dict(N=N, rule=D, t=t, b=b, cells=cells, X=X))
-        log(f"[collect] polygon {i}: N={N} rule depth {D}; ns/pt " +
-            " ".join(f"{d}:{t[d]:.3f}" for d in ds_) + "; build ms " + " ".join(f"{d}:{b[d]:.2f}" for d in ds_))
-        dx.free()
+        for j, xyz in enumerate(samples):
+            # features of the deepest hierarchy (levels below its bottom)
+            g.set_query_hint(xyz[::16], None, 16.0)
+            X = (np.stack([g.depth_features(l)[0] for l in range(max(ds_) - 1)]) if max(ds_) > 1
+                 else np.zeros((0, 42, 2, 12)))
+            recs.append(dict(N=N, rule=D, t=t[j], b=b, cells=cells, X=X, poly=i))
+            log(f"[collect] polygon {i} mix {j}: N={N} rule depth {D}; ns/pt " +
+                " ".join(f"{d}:{t[j][d]:.3f}" for d in ds_) + "; build ms " + " ".join(f"{d}:{b[d]:.2f}" for d in ds_))
+        for dx in dxs:
+            dx.free()
         dl.free()
         g.close()
     return recs
@@ -249,7 +265,7 @@ def fit_build_cost(recs):
     return [float(c) for c in coef]
 
 
-def _group_loss_grad(y, S, T, lam=4.0):
+def _group_loss_grad(y, S, T, lam=4.0, W=None):
     """Loss over groups (consecutive pairs are the stop / refine groups of
     one polygon and level): squared log2 error of each group, plus lam times
     the squared error of each pair's difference -- the quantity the build's
@@ -261,14 +277,17 @@ def _group_loss_grad(y, S, T, lam=4.0):
     P = e.sum(axis=1)
     pred = np.log2(np.maximum(P, 1e-300)) + ymax[:, 0]
     r = pred - T
-    g = 2.0 * r / ng
-    loss = float(np.mean(r * r))
+    W = np.ones(ng) if W is None else W
+    wsum = W.sum()
+    g = 2.0 * W * r / wsum
+    loss = float(np.sum(W * r * r) / wsum)
     npair = ng // 2
     if npair and lam > 0:
         d = (pred[1::2] - pred[0::2]) - (T[1::2] - T[0::2])
-        loss += lam * float(np.mean(d * d))
-        g[1::2] += 2.0 * lam * d / npair
-        g[0::2] -= 2.0 * lam * d / npair
+        wp = W[0::2]
+        loss += lam * float(np.sum(wp * d * d) / wp.sum())
+        g[1::2] += 2.0 * lam * wp * d / wp.sum()
+        g[0::2] -= 2.0 * lam * wp * d / wp.sum()
     dy = g[:, None] * e / np.maximum(P, 1e-300)[:, None]
     return loss, dy
 
@@ -287,6 +306,8 @@ def train(ctx, G, steps=4000, lr=2e-3, seed=0, val=None, every=250, log=print):
     m = DeviceMLP(ctx, seed=seed)
     m.set_inputs(np.concatenate([g[0] for g in G]).astype(np.float32))
     S, T = _shares(G), np.log2(np.array([g[2] for g in G]))
+    # larger polygons weigh more: their decisions move more time
+    Wt = np.array([1.0 + (float(g[0][0, 10]) * 20.0 / 5.0) ** 2 for g in G])
     mv = None
     if val:
         mv = DeviceMLP(ctx, seed=seed)
@@ -302,7 +323,7 @@ def train(ctx, G, steps=4000, lr=2e-3, seed=0, val=None, every=250, log=print):
                 best = (lv, m.params(), step)
         if step == steps:
             break
-        loss, dy = _group_loss_grad(m.forward(), S, T)
+        loss, dy = _group_loss_grad(m.forward(), S, T, W=Wt)
         m.backward(dy.reshape(-1).astype(np.float32))
         m.adam(lr if step < steps * 0.7 else lr * 0.2)
         if step % 500 == 0 or step == steps - 1:
@@ -326,8 +347,9 @@ def predict_groups(ctx, params, G):
     return np.log2(np.maximum((S * np.exp2(y)).sum(axis=1), 1e-300))
 
 
-K_FLOOR = 0.01   # the build's window: refine while crossed cells / N < K_FLOOR
+K_FLOOR = 0.02   # the build's window: refine while crossed cells / N < K_FLOOR
 K_ABOVE = 2      # at most this many levels past the density rule's stop
+MARGIN = 2.0     # predicted gain factor needed to depart from the density rule
 
 
 def simulate_depth(y, U, crossed, N, w_total, cells_next, cost):
@@ -346,7 +368,10 @@ def simulate_depth(y, U, crossed, N, w_total, cells_next, cost):
             return l + 1
         live = crossed[l] > 0  # top-level cells with nothing crossed keep their cost
         saving = float(np.sum((w_total[l] * (np.exp2(y[l][:, 0]) - np.exp2(y[l][:, 1])))[live]))
-        if saving <= cost[1] * N + cost[2] * cells_next[l] + cost[3]:
+        c = cost[1] * N + cost[2] * cells_next[l] + cost[3]
+        past = k > 0.1 or (k < 0.1 and 0.1 - k < 0.01)
+        refine = saving > MARGIN * c if past else saving * MARGIN > c
+        if not refine:
             return l + 1
     return len(y) + 1
 
@@ -400,8 +425,8 @@ def main(argv=None):
     G = groups(recs)
     # polygons split three ways: training, validation (early stopping) and
     # held out (reported)
-    val_ids = {ri for ri in range(len(recs)) if ri % 5 == 3}
-    hold = {ri for ri in range(len(recs)) if ri % 5 == 4}
+    val_ids = {ri for ri in range(len(recs)) if recs[ri]["poly"] % 5 == 3}
+    hold = {ri for ri in range(len(recs)) if recs[ri]["poly"] % 5 == 4}
     Gt = [g for g in G if g[3] not in hold and g[3] not in val_ids]
     Gval = [g for g in G if g[3] in val_ids]
     Gv = [g for g in G if g[3] in hold]
