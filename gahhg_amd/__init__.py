@@ -77,7 +77,7 @@ EXPORTED = [
     "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
     "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
-    "gahhg_autotune_query",
+    "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -147,6 +147,8 @@ def lib():
         "gahhg_set_query_hint": ([P, P, P, i64, ctypes.c_double], ctypes.c_int),
         "gahhg_depth_features": ([P, i32, P], ctypes.c_int),
         "gahhg_autotune_query": ([P, P, P, i64, P, P], ctypes.c_int),
+        "gahhg_save_grid": ([P, ctypes.c_char_p], ctypes.c_int),
+        "gahhg_load_grid": ([P, ctypes.c_char_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -384,6 +386,15 @@ class GPISP:
         e = edges.astype(np.float64)
         e[..., 1:] /= FIXED_POINT
         return cells, e
+
+    def save_grid(self, path):
+        self._check(self.L.gahhg_save_grid(self.h, str(path).encode()))
+        return self
+
+    def load_grid(self, path):
+        """Loads hierarchies saved by save_grid; the context is then built."""
+        self._check(self.L.gahhg_load_grid(self.h, str(path).encode()))
+        return self
 
     def autotune_device(self, d_xyz, d_poly, n, d_labels):
         """Times the query's lane-group width and block size on these points

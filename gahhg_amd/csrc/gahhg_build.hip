@@ -848,6 +848,23 @@ static int pd_upload(gahhg_ctx* c) {
 // polygon still refining is processed by the same launches; one host
 // synchronisation per level applies the density rule per polygon and sizes
 // the next level.
+// Device arrays for a hierarchy read from a file: sized for its counts (the
+// contents are copied in by the caller), per-edge records and scratch.
+int attach_loaded_grid(gahhg_ctx* c, int64_t V, int64_t B, int64_t C, int64_t R) {
+  const int64_t Nt = c->n_edges;
+  if (Nt > 0) {
+    hipLaunchKernelGGL(k_edges, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const double*)c->d_pv,
+                       (const int32_t*)c->d_pe, (int)Nt, c->d_edge);
+    CK(hip_check(c, hipGetLastError(), "edge records"));
+  }
+  CK(ensure_v(c, 0, std::max<int64_t>(V, 1)));
+  CK(ensure_b(c, 0, std::max<int64_t>(B, 1)));
+  CK(ensure_c(c, 0, std::max<int64_t>(C, 1)));
+  CK(ensure_flags(c, std::max<int64_t>(Nt, C) + 1));
+  CK(ensure_refs(c, R + 1));
+  return GAHHG_OK;
+}
+
 int build_hierarchy(gahhg_ctx* c) {
   const int np = (int)c->polys.size();
   const int64_t Nt = c->n_edges;
@@ -999,7 +1016,20 @@ int build_hierarchy(gahhg_ctx* c) {
                        (const int32_t*)c->d_f3, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1,
                        (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
     LAUNCH_CK("plan totals");
-    if (learned) CK(learn_level(c, lv, GL.cstart, GL.cnum, c->learn.h_dec.data()));
+    // the model runs only at levels where some polygon may be inside its
+    // window (crossed ratio within a factor 4 of the floor at the previous
+    // level); below it every polygon refines anyway
+    bool model_level = false;
+    if (learned)
+      for (int p = 0; p < np; p++) {
+        const PolyDev& P = c->pd[p];
+        model_level |= P.refining && P.depth == 0 && c->pgamma[p] * 4.0f >= kLearnFloor;
+      }
+    if (model_level) {
+      CK(learn_level(c, lv, GL.cstart, GL.cnum, c->learn.h_dec.data()));
+    } else if (learned) {
+      std::fill(c->learn.h_dec.begin(), c->learn.h_dec.end(), 1);
+    }
     CK(pd_download(c));  // also completes the decisions' copy
     // density rule per polygon (:1597-1604) and the next level's layout
     int64_t nv_all = 0, nb_all = 0, nc_all = 0;
@@ -1008,7 +1038,7 @@ int build_hierarchy(gahhg_ctx* c) {
       if (!P.refining) continue;
       if (learned && P.depth == 0) {
         // the model decides only inside the window it was trained on:
-        // refine while crossed cells / N < kLearnFloor (about three levels
+        // refine while crossed cells / N < kLearnFloor (one to two levels
         // before the density rule would stop), never more than kLearnAbove
         // levels past the level where the density rule stops
         float k = P.U * 1.0 / P.N;

@@ -392,6 +392,188 @@ int gahhg_build(gahhg_ctx* c) {
   return GAHHG_OK;
 }
 
+// ---- grid serialisation ----------------------------------------------------
+// File: "GAHHGGR1", int32 version, polygons, PolyDev size; int64 V, B, C, R;
+// int32 global levels + LevelDesc each; per polygon: N, flips, pv[3N], pe[2N],
+// PolyDev record, its levels, gamma; then the device arrays in the order of
+// grid_arrays(); finally the two undefined-leaf counts.  Native byte order.
+namespace {
+
+constexpr char kGridMagic[8] = {'G', 'A', 'H', 'H', 'G', 'G', 'R', '1'};
+constexpr int32_t kGridVersion = 1;
+
+struct Arr {
+  void* p;
+  int64_t bytes;
+};
+
+std::vector<Arr> grid_arrays(gahhg_ctx* c, int64_t V, int64_t B, int64_t C, int64_t R) {
+  DeviceGrid& g = c->g;
+  return {{g.v, 24 * V},         {g.b_parent, 4 * B},  {g.b_partner, 4 * B}, {g.b_v0, 4 * B},
+          {g.b_v1, 4 * B},       {g.b_subcell, 4 * B}, {g.b_id, 4 * B},      {g.b_n, 24 * B},
+          {g.c_bstart, 4 * C},   {g.c_bnum, 4 * C},    {g.c_rstart, 4 * C},  {g.c_rnum, 4 * C},
+          {g.c_upcell, 4 * C},   {g.c_subcell, 4 * C}, {g.c_type, 4 * C},    {g.c_poly, 4 * C},
+          {g.c_root, 4 * C},     {g.refs, 4 * R}};
+}
+
+bool wr(FILE* f, const void* p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
+bool rd(FILE* f, void* p, size_t n) { return n == 0 || fread(p, 1, n, f) == n; }
+
+void finish_stats(gahhg_ctx* c) {
+  if (c->polys.size() == 1) {
+    poly_stats(c, 0, &c->st);
+  } else {
+    gahhg_stats tot, ps;
+    memset(&tot, 0, sizeof(tot));
+    for (int p = 0; p < (int)c->polys.size(); p++) {
+      poly_stats(c, p, &ps);
+      tot.vnum += ps.vnum, tot.bnum += ps.bnum, tot.cnum += ps.cnum, tot.rnum += ps.rnum;
+      tot.vspace += ps.vspace, tot.bspace += ps.bspace, tot.cspace += ps.cspace, tot.rspace += ps.rspace;
+      tot.gpu_space_bytes += ps.gpu_space_bytes;
+      tot.cpu_space_bytes += ps.cpu_space_bytes;
+      tot.depth = std::max(tot.depth, ps.depth);
+    }
+    tot.n_edges = c->n_edges;
+    c->st = tot;
+  }
+}
+
+}  // namespace
+
+int gahhg_save_grid(gahhg_ctx* c, const char* path) {
+  RCK(ctx_ok(c));
+  if (!path) return set_err(c, GAHHG_E_ARG, "save: null path");
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "save before build");
+  RCK(select_device(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  FILE* f = fopen(path, "wb");
+  if (!f) return set_err(c, GAHHG_E_IO, "save: cannot open %s", path);
+  const int32_t np = (int32_t)c->polys.size(), pds = (int32_t)sizeof(PolyDev);
+  const int64_t cnt[4] = {c->tot_v, c->tot_b, c->tot_c, c->tot_r};
+  const int32_t ng = (int32_t)c->glv.size();
+  bool ok = wr(f, kGridMagic, 8) && wr(f, &kGridVersion, 4) && wr(f, &np, 4) && wr(f, &pds, 4) && wr(f, cnt, 32) &&
+            wr(f, &ng, 4) && wr(f, c->glv.data(), ng * sizeof(LevelDesc));
+  for (int p = 0; ok && p < np; p++) {
+    const Polygon& P = c->polys[p];
+    const int32_t nl = (int32_t)c->plv[p].size();
+    ok = wr(f, &P.N, 4) && wr(f, &P.flips, 4) && wr(f, P.pv.data(), P.pv.size() * 8) &&
+         wr(f, P.pe.data(), P.pe.size() * 4) && wr(f, &c->pd[p], sizeof(PolyDev)) && wr(f, &nl, 4) &&
+         wr(f, c->plv[p].data(), nl * sizeof(LevelDesc)) && wr(f, &c->pgamma[p], 4);
+  }
+  std::vector<char> buf;
+  for (const Arr& a : grid_arrays(c, cnt[0], cnt[1], cnt[2], cnt[3])) {
+    if (!ok) break;
+    buf.resize((size_t)a.bytes);
+    if (a.bytes && hip_check(c, hipMemcpy(buf.data(), a.p, a.bytes, hipMemcpyDeviceToHost), "save D2H")) {
+      fclose(f);
+      return GAHHG_E_HIP;
+    }
+    ok = wr(f, buf.data(), (size_t)a.bytes);
+  }
+  const int64_t und[2] = {c->und_snap, c->und_final};
+  ok = ok && wr(f, und, 16);
+  if (fclose(f) != 0) ok = false;
+  return ok ? GAHHG_OK : set_err(c, GAHHG_E_IO, "save: write to %s failed", path);
+}
+
+int gahhg_load_grid(gahhg_ctx* c, const char* path) {
+  RCK(ctx_ok(c));
+  if (!path) return set_err(c, GAHHG_E_ARG, "load: null path");
+  RCK(select_device(c));
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_err(c, GAHHG_E_IO, "load: cannot open %s", path);
+  char magic[8];
+  int32_t ver = 0, np = 0, pds = 0, ng = 0;
+  int64_t cnt[4];
+  bool ok = rd(f, magic, 8) && memcmp(magic, kGridMagic, 8) == 0 && rd(f, &ver, 4) && ver == kGridVersion &&
+            rd(f, &np, 4) && np >= 1 && rd(f, &pds, 4) && pds == (int32_t)sizeof(PolyDev) && rd(f, cnt, 32) &&
+            rd(f, &ng, 4) && ng >= 1 && ng <= kMaxLevels;
+  for (int k = 0; ok && k < 4; k++) ok = cnt[k] >= 0 && cnt[k] <= INT32_MAX;
+  std::vector<LevelDesc> glv(ok ? ng : 0);
+  ok = ok && rd(f, glv.data(), ng * sizeof(LevelDesc));
+  std::vector<Polygon> polys(ok ? np : 0);
+  std::vector<PolyDev> pd(ok ? np : 0);
+  std::vector<std::vector<LevelDesc>> plv(ok ? np : 0);
+  std::vector<float> gam(ok ? np : 0);
+  for (int p = 0; ok && p < np; p++) {
+    Polygon& P = polys[p];
+    int32_t nl = 0;
+    ok = rd(f, &P.N, 4) && P.N >= 3 && P.N <= INT32_MAX / 4 && rd(f, &P.flips, 4);
+    if (!ok) break;
+    P.pv.resize(3 * (size_t)P.N);
+    P.pe.resize(2 * (size_t)P.N);
+    ok = rd(f, P.pv.data(), P.pv.size() * 8) && rd(f, P.pe.data(), P.pe.size() * 4) && rd(f, &pd[p], sizeof(PolyDev)) &&
+         rd(f, &nl, 4) && nl >= 1 && nl <= kMaxLevels;
+    if (!ok) break;
+    plv[p].resize(nl);
+    ok = rd(f, plv[p].data(), nl * sizeof(LevelDesc)) && rd(f, &gam[p], 4);
+    for (int32_t v : P.pe) ok = ok && v >= 0 && v < P.N;
+  }
+  // ranges the kernels index with must lie inside the arrays
+  int64_t e0 = 0;
+  for (int p = 0; ok && p < np; p++) {
+    const PolyDev& P = pd[p];
+    ok = P.e0 == e0 && P.N == polys[p].N && P.cell0 >= 0 && P.cell0 + kGrid0Cells <= cnt[2] && P.bcs >= 0 &&
+         P.bcn >= 0 && (int64_t)P.bcs + P.bcn <= cnt[2] && P.roff >= 0 && P.R >= 0 &&
+         (int64_t)P.roff + P.R <= cnt[3] && P.L >= 0 && P.L < (int)plv[p].size();
+    for (const LevelDesc& L : plv[p])
+      ok = ok && L.cstart >= 0 && L.cnum >= 0 && (int64_t)L.cstart + L.cnum <= cnt[2] && L.bstart >= 0 &&
+           L.bnum >= 0 && (int64_t)L.bstart + L.bnum <= cnt[1] && L.vstart >= 0 && (int64_t)L.vstart + L.vnum <= cnt[0];
+    e0 += P.N;
+  }
+  for (const LevelDesc& L : glv) ok = ok && L.cstart >= 0 && L.cnum >= 0 && (int64_t)L.cstart + L.cnum <= cnt[2];
+  if (!ok) {
+    fclose(f);
+    return set_err(c, GAHHG_E_FORMAT, "load: %s is not a grid file of this library version", path);
+  }
+  c->polys = std::move(polys);
+  int rc = upload_polygons(c);
+  if (!rc) rc = attach_loaded_grid(c, cnt[0], cnt[1], cnt[2], cnt[3]);
+  std::vector<char> buf;
+  for (const Arr& a : grid_arrays(c, cnt[0], cnt[1], cnt[2], cnt[3])) {
+    if (rc) break;
+    buf.resize((size_t)a.bytes);
+    if (!rd(f, buf.data(), (size_t)a.bytes)) {
+      rc = set_err(c, GAHHG_E_FORMAT, "load: %s is truncated", path);
+      break;
+    }
+    if (a.bytes) rc = hip_check(c, hipMemcpy(a.p, buf.data(), a.bytes, hipMemcpyHostToDevice), "load H2D");
+  }
+  int64_t und[2] = {0, 0};
+  if (!rc && !rd(f, und, 16)) rc = set_err(c, GAHHG_E_FORMAT, "load: %s is truncated", path);
+  fclose(f);
+  if (rc) {
+    free_grid(c);
+    return rc;
+  }
+  c->pd = std::move(pd);
+  c->plv = std::move(plv);
+  c->pgamma = std::move(gam);
+  c->glv = std::move(glv);
+  c->gnum = (int32_t)c->glv.size();
+  c->tot_v = cnt[0];
+  c->tot_b = cnt[1];
+  c->tot_c = cnt[2];
+  c->tot_r = cnt[3];
+  c->und_snap = und[0];
+  c->und_final = und[1];
+  for (PolyDev& P : c->pd) P.tbl = -1;
+  memset(c->h_st, 0, sizeof(DevState));
+  RCK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state"));
+  RCK(hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                  c->stream),
+                "polygon upload"));
+  const double t0 = now_ms();
+  RCK(pack_query_layout(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  c->built = true;
+  finish_stats(c);
+  c->st.pre_ms = now_ms() - t0;  // the query layout packed after loading
+  c->st.undefined_leaves_snapshot = c->und_snap;
+  c->st.undefined_leaves_final = c->und_final;
+  return GAHHG_OK;
+}
+
 int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n,
                                   int32_t* d_labels) {
   RCK(ctx_ok(c));

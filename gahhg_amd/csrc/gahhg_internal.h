@@ -208,6 +208,10 @@ struct LearnState {
   double hint_weight = 1.0;
   unsigned long long* d_estat = nullptr;  // np x 42 x 3 edge statistics
   uint32_t* d_hist = nullptr;             // np x 42 hint points
+  uint32_t* d_band = nullptr;             // np x 42 hint points in crossed cells
+  int32_t* d_hloc = nullptr;              // per hint point: its cell at hloc_level (-1 = done)
+  int64_t hloc_cap = 0;
+  int hloc_level = 0;
   float* d_w = nullptr;                   // np x 42 expected points
   int32_t* d_stat = nullptr;              // np x 42 x 2 cells, crossed cells
   float* d_x = nullptr;                   // np x 42 x 2 x 12 features
@@ -220,8 +224,10 @@ struct LearnState {
 
 // Window of the learned depth decision (crossed cells / N below which a
 // polygon always refines; levels past the density rule's stop it may add).
-constexpr float kLearnFloor = 0.01f;
+constexpr float kLearnFloor = 0.02f;
 constexpr int kLearnAbove = 2;
+// Predicted gain factor the model needs to depart from the density rule.
+constexpr double kLearnMargin = 2.0;
 
 // Grow-only buffers of the query sort: keys and values (two of each, n
 // entries) followed by the sort's temporary storage.
@@ -334,6 +340,7 @@ int mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m, flo
 int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges);
 int edge_stats(gahhg_ctx* c, unsigned long long* d_out);
 int learn_prepare(gahhg_ctx* c);
+int attach_loaded_grid(gahhg_ctx* c, int64_t V, int64_t B, int64_t C, int64_t R);
 int learn_level(gahhg_ctx* c, int level, int lo, int n, int32_t* h_dec);
 int learn_features_after_build(gahhg_ctx* c, int level, float* h_x);
 void learn_free(gahhg_ctx* c);

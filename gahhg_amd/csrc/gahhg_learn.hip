@@ -29,27 +29,74 @@ namespace gahhg {
 
 namespace {
 
+// The reference's inside test: every border plane n.p >= -eps.
+__device__ __forceinline__ bool inside(const GridView& g, V3 q, int cid) {
+  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+  for (int k = 0; k < bn; k++)
+    if (lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), q))) return false;
+  return true;
+}
+
 // Points per (polygon, top-level cell) of the hint sample; the top-level
 // cell is the first level-0 cell (index order) whose planes all pass, the
 // same test the query's level-0 scan applies (level-0 geometry is the same
 // for every polygon: polygon 0's cells are used).
-__global__ void k_hint_hist(GridView g, const double* __restrict__ xyz, const int32_t* __restrict__ poly,
-                            int64_t n, int npoly, uint32_t* __restrict__ hist) {
+__global__ void k_hint_hist(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ xyz,
+                            const int32_t* __restrict__ poly, int64_t n, int npoly, uint32_t* __restrict__ hist,
+                            int32_t* __restrict__ loc) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   if (t < n) {
+    int cell = -1;
     const int p = poly ? poly[t] : 0;
     if (p >= 0 && p < npoly) {
       const V3 q = load3(xyz + 3 * t);
-      for (int i = 0; i < kGrid0Cells && key < 0; i++) {
-        const int bs = g.c_bstart[i], bn = g.c_bnum[i];
-        bool in = true;
-        for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), q));
-        if (in) key = p * kGrid0Cells + i;
-      }
+      for (int i = 0; i < kGrid0Cells && key < 0; i++)
+        if (inside(g, q, i)) {
+          key = p * kGrid0Cells + i;
+          cell = pd[p].cell0 + i;
+        }
     }
+    loc[t] = cell;
   }
   wave_add<uint32_t>(hist, key, 1u);
+}
+
+// Hint points in the boundary band: per (polygon, top-level cell), the hint
+// points whose first-match descent reaches a cell of the current level that
+// is crossed (build time: marked; after the build: refined).  loc[t] holds
+// the point's cell `steps` levels above (or -1 once it ended in a leaf above
+// the band, where it stays); the kernel descends it and stores the cell.
+__global__ void k_hint_band(GridView g, const double* __restrict__ xyz, int32_t* __restrict__ loc, int64_t n,
+                            int steps, int marked, uint32_t* __restrict__ band) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int key = -1;
+  if (t < n) {
+    int cur = loc[t];
+    if (cur >= 0) {
+      const V3 q = load3(xyz + 3 * t);
+      for (int s = 0; s < steps && cur >= 0; s++) {
+        const int sc = g.c_subcell[cur];
+        if (sc == -1) {
+          cur = -1;
+          break;
+        }
+        int next = inside(g, q, sc) ? sc : -1;
+        if (next < 0) {
+          const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+          for (int i = 0; i < bn && next < 0; i++) {
+            const int tc = g.b_subcell[bs + i];
+            if (tc != -1 && inside(g, q, tc)) next = tc;
+          }
+        }
+        cur = next;
+      }
+      loc[t] = cur;
+      if (cur >= 0 && (marked ? g.c_rnum[cur] != 0 : g.c_subcell[cur] != -1))
+        key = g.c_poly[cur] * kGrid0Cells + g.c_root[cur];
+    }
+  }
+  wave_add<uint32_t>(band, key, 1u);
 }
 
 // Expected points per (polygon, top-level cell): the hint histogram times
@@ -88,6 +135,7 @@ __global__ void k_level_stats(GridView g, const PolyDev* __restrict__ pd, int lo
 // Features of rows (p, r, k), row = (p * 42 + r) * 2 + k.
 __global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
                            const unsigned long long* __restrict__ estat, const float* __restrict__ w,
+                           const uint32_t* __restrict__ hist, const uint32_t* __restrict__ band,
                            float* __restrict__ X) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= npoly * kGrid0Cells * 2) return;
@@ -112,7 +160,10 @@ __global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd,
   x[6] = log2f(1.0f + Nr / fmaxf(Ur, 1.0f)) / 10.0f;
   x[7] = (len > 0.0) ? (float)(log2(len / cell) / 10.0) : 0.0f;
   x[8] = (float)(turn / M_PI);
-  x[9] = Ur / fmaxf(Cr, 1.0f);
+  // share of r's points in the boundary band (crossed cells of this level):
+  // from the hint sample, else the crossed share of r's cells (the same
+  // quantity for evenly spread points)
+  x[9] = hist ? (float)band[pr] / fmaxf((float)hist[pr], 1.0f) : Ur / fmaxf(Cr, 1.0f);
   x[10] = log2f(1.0f + (float)pd[p].N) / 20.0f;
   x[11] = wsum > 0.f ? w[pr] / wsum : Nr / fmaxf((float)pd[p].N, 1.0f);
 }
@@ -138,15 +189,24 @@ __global__ void k_decide(int npoly, const PolyDev* __restrict__ pd, const int32_
     saving += (double)w[pr] * (exp2((double)Y[2 * pr]) - exp2((double)Y[2 * pr + 1]));
   }
   const double cost = ns_edge * P.N + ns_cell * ((double)P.U + P.nbc) + ns_level / max(n_refining, 1);
-  dec[p] = (P.U > 0 && saving * query_scale > cost) ? 1 : 0;
+  // leave the density rule's choice only for a predicted gain of
+  // kLearnMargin: before its stop level, stop only if the saving is below
+  // cost / margin; at or past it, refine only if the saving exceeds
+  // margin * cost
+  const float k = P.U * 1.0 / P.N;
+  const bool past_rule = k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01);
+  const double gain = saving * query_scale;
+  const bool refine = past_rule ? gain > kLearnMargin * cost : gain * kLearnMargin > cost;
+  dec[p] = (P.U > 0 && refine) ? 1 : 0;
 }
 
 int ensure_learn(gahhg_ctx* c) {
   const int64_t np = (int64_t)c->polys.size();
   LearnState& L = c->learn;
   if (L.cap >= np) return GAHHG_OK;
-  void* ptrs[] = {L.d_estat, L.d_hist, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec};
+  void* ptrs[] = {L.d_estat, L.d_hist, L.d_band, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec};
   for (void* q : ptrs) dfree(q);
+  L.d_band = nullptr;
   L.d_estat = nullptr;
   L.d_hist = nullptr;
   L.d_w = nullptr;
@@ -157,6 +217,7 @@ int ensure_learn(gahhg_ctx* c) {
   L.cap = 0;
   const int64_t pr = np * kGrid0Cells;
   if (dalloc(c, (void**)&L.d_estat, pr * 3 * 8) || dalloc(c, (void**)&L.d_hist, pr * 4) ||
+      dalloc(c, (void**)&L.d_band, pr * 4) ||
       dalloc(c, (void**)&L.d_w, pr * 4) || dalloc(c, (void**)&L.d_stat, pr * 2 * 4) ||
       dalloc(c, (void**)&L.d_x, pr * 2 * kMlpIn * 4) || dalloc(c, (void**)&L.d_y, pr * 2 * 4) ||
       dalloc(c, (void**)&L.d_dec, np * 4))
@@ -182,8 +243,17 @@ int learn_prepare(gahhg_ctx* c) {
   if (L.d_hint_xyz && L.hint_n > 0) {
     rc = fill_i32(c, (int32_t*)L.d_hist, (int64_t)np * kGrid0Cells, 0);
     if (rc) return rc;
+    if (L.hloc_cap < L.hint_n) {
+      dfree(L.d_hloc);
+      L.d_hloc = nullptr;
+      L.hloc_cap = 0;
+      if (dalloc(c, (void**)&L.d_hloc, L.hint_n * 4)) return GAHHG_E_NOMEM;
+      L.hloc_cap = L.hint_n;
+    }
+    L.hloc_level = 0;
     hipLaunchKernelGGL(k_hint_hist, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
-                       (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly, L.hint_n, np, L.d_hist);
+                       (const PolyDev*)c->d_pd, (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly, L.hint_n,
+                       np, L.d_hist, L.d_hloc);
     rc = hip_check(c, hipGetLastError(), "hint histogram");
     if (rc) return rc;
     hist = L.d_hist;
@@ -207,9 +277,21 @@ static int learn_features(gahhg_ctx* c, int level, int lo, int n, int marked, in
     rc = hip_check(c, hipGetLastError(), "level stats");
     if (rc) return rc;
   }
+  const bool hint = L.d_hint_xyz && L.hint_n > 0;
+  if (hint) {
+    rc = fill_i32(c, (int32_t*)L.d_band, (int64_t)np * kGrid0Cells, 0);
+    if (rc) return rc;
+    if (level < L.hloc_level) return set_err(c, GAHHG_E_STATE, "features: levels must not decrease");
+    hipLaunchKernelGGL(k_hint_band, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
+                       (const double*)L.d_hint_xyz, L.d_hloc, L.hint_n, level - L.hloc_level, marked, L.d_band);
+    L.hloc_level = level;
+    rc = hip_check(c, hipGetLastError(), "hint band");
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_features, dim3(blocks((int64_t)np * kGrid0Cells * 2, 256)), dim3(256), 0, c->stream, np,
                      level, (const PolyDev*)c->d_pd, (const int32_t*)L.d_stat, (const unsigned long long*)L.d_estat,
-                     (const float*)L.d_w, L.d_x);
+                     (const float*)L.d_w, hint ? (const uint32_t*)L.d_hist : nullptr,
+                     hint ? (const uint32_t*)L.d_band : nullptr, L.d_x);
   return hip_check(c, hipGetLastError(), "features");
 }
 
@@ -251,8 +333,8 @@ int learn_features_after_build(gahhg_ctx* c, int level, float* h_x) {
 
 void learn_free(gahhg_ctx* c) {
   LearnState& L = c->learn;
-  void* ptrs[] = {L.d_estat, L.d_hist, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec, L.d_weights, L.d_hint_xyz,
-                  L.d_hint_poly};
+  void* ptrs[] = {L.d_estat, L.d_hist, L.d_band, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec, L.d_weights,
+                  L.d_hint_xyz, L.d_hint_poly, L.d_hloc};
   for (void* q : ptrs) dfree(q);
   L = LearnState();
 }

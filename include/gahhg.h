@@ -214,8 +214,11 @@ int gahhg_mlp_adam(gahhg_ctx* ctx, float* d_master, const float* d_grad, float* 
  * refinement stopping at this level (k = 0) or after one more (k = 1), and
  * m the polygons of the batch still refining (they share a level's cost).
  * The model decides only inside the window it is trained on: a polygon
- * always refines while its crossed cells are below 0.01 N, and refines at
- * most 2 levels past the level where the density rule stops.
+ * always refines while its crossed cells are below 0.02 N, and refines at
+ * most 2 levels past the level where the density rule stops.  It departs
+ * from the density rule only for a predicted factor of 2: before the
+ * rule's stop a polygon stops when 2 x saving < cost, at or past it a
+ * polygon refines when saving > 2 x cost.
  * The hierarchy keeps the reference's structure (one bottom level per
  * polygon); max_depth still caps it.  params: GAHHG_MLP_PARAMS values as
  * for gahhg_mlp_*, rounded to BF16 here; NULL switches the model off.
@@ -230,6 +233,14 @@ int gahhg_set_query_hint(gahhg_ctx* ctx, const double* xyz, const int32_t* poly,
  * row at `level` of the last build (cells refined there stand for the
  * cells crossed): x[((p * 42 + r) * 2 + k) * 12 + f]. */
 int gahhg_depth_features(gahhg_ctx* ctx, int32_t level, float* x);
+
+/* Grid serialisation (no counterpart in the reference, which rebuilds per
+ * run): save the built hierarchies (polygons, every level, refs, labels) to
+ * a file; load them into a context, which is then built (the query layout
+ * is packed on load; pre_ms reports that time).  Files are tied to this
+ * library version and byte order. */
+int gahhg_save_grid(gahhg_ctx* ctx, const char* path);
+int gahhg_load_grid(gahhg_ctx* ctx, const char* path);
 
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */

@@ -156,3 +156,45 @@ def test_autotune_keeps_labels():
     for b in (dx, dp, dl):
         b.free()
     g.close()
+
+
+def test_grid_save_load(tmp_path):
+    """A saved hierarchy loads into a fresh context with identical arrays,
+    statistics and labels (single polygon and batch)."""
+    import os
+    say("grid save/load: NA0 and a 4-polygon batch")
+    na0 = os.path.join(os.path.dirname(__file__), "golden", "reference", "NA0.txt")
+    ps = datasets.batch_polygons(n_poly=4, seed=41)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    rng = np.random.default_rng(6)
+    lon, lat = datasets.uniform_queries(rng, 200_000)
+    xyz = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    poly = rng.integers(0, 4, len(lon)).astype(np.int32)
+    for case in ("single", "batch"):
+        a = gahhg_amd.GPISP(device=0)
+        if case == "single":
+            a.load_polygon_txt(na0).build()
+        else:
+            a.set_polygons_lonlat(items).build()
+        f = tmp_path / f"{case}.grid"
+        a.save_grid(f)
+        b = gahhg_amd.GPISP(device=0).load_grid(f)
+        assert b.n_polygons == a.n_polygons
+        for k in range(a.n_polygons):
+            ga, gb = a.polygon_grid(k), b.polygon_grid(k)
+            for key in ("cells", "borders", "refs", "levels"):
+                assert np.array_equal(ga[key], gb[key]), (case, k, key)
+            assert np.array_equal(ga["v"].view(np.int64), gb["v"].view(np.int64))
+            sa, sb = a.polygon_stats(k), b.polygon_stats(k)
+            for key in ("depth", "gpu_space_bytes", "vnum", "bnum", "cnum", "rnum"):
+                assert sa[key] == sb[key], (case, k, key)
+        if case == "single":
+            assert (a.query_xyz(xyz) == b.query_xyz(xyz)).all()
+        else:
+            assert (a.query_batch_xyz(xyz, poly) == b.query_batch_xyz(xyz, poly)).all()
+        a.close()
+        b.close()
+    bad = tmp_path / "bad.grid"
+    bad.write_bytes(b"not a grid")
+    with pytest.raises(gahhg_amd.GahhgError):
+        gahhg_amd.GPISP(device=0).load_grid(bad)
