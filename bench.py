@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--bins", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
+    ap.add_argument("--no-series", action="store_true")
     ap.add_argument("--rule", action="store_true", help="density-rule depth only (no learned model)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size reference for "
@@ -271,7 +272,54 @@ def main():
     del xyz, labels
     if not args.no_batch:
         out["batch_256"] = bench_batch(args, oracle_lib)
+    if not args.no_series:
+        out["series_mars"] = bench_series(oracle_lib)
     print(json.dumps(out), flush=True)
+
+
+def bench_series(oracle_lib):
+    """The reference's Mars series (624 polygon files x the 10000 points of
+    Mars_query.csv): one batch (file reads, build, query, label download) against
+    one context per polygon in turn (the reference starts one process per file;
+    process and device start-up are not counted here)."""
+    import gahhg_amd
+    from gahhg_amd import sweep
+
+    gold = os.path.join(ROOT, "tests", "golden")
+    files = sweep.series_files(os.path.join(gold, "mars"))
+    qcsv = os.path.join(gold, "reference", "Mars_query.csv")
+    sweep.run(files[:8], qcsv)  # warm the library and device
+    walls = []
+    for _ in range(3):
+        r = sweep.run(files, qcsv)
+        walls.append(r["wall_ms"])
+    lab = r["labels"]
+    lon, lat = gahhg_amd.read_query_csv(qcsv)
+    xyz = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    t = time.perf_counter()
+    g = gahhg_amd.GPISP(device=0)
+    pb = pq = 0.0
+    mism = 0
+    for k, f in enumerate(files):
+        g.load_polygon_txt(f).build()
+        pb += g.stats()["pre_ms"]
+        mism += int(np.count_nonzero(g.query_xyz(xyz) != lab[k]))
+        pq += g.stats()["query_ms"]
+    g.close()
+    per_wall = (time.perf_counter() - t) * 1e3
+    omism = 0
+    for k, f in enumerate(files):
+        omism += int(np.count_nonzero(oracle_lib.Oracle().load_txt(f).build().query_xyz(xyz) != lab[k]))
+    P, Q = lab.shape
+    res = {"n_polygons": P, "n_points": Q, "pairs": P * Q,
+           "batched": {"build_ms": round(r["build_ms"], 3), "query_ms": round(r["query_ms"], 3),
+                       "wall_ms": round(float(np.median(walls)), 1),
+                       "pairs_per_s": round(P * Q / (float(np.median(walls)) * 1e-3), 1)},
+           "per_polygon": {"build_ms_sum": round(pb, 3), "query_ms_sum": round(pq, 3), "wall_ms": round(per_wall, 1)},
+           "wall_speedup": round(per_wall / float(np.median(walls)), 2),
+           "label_differences_vs_per_polygon": mism, "oracle_mismatches": omism}
+    log(f"series_mars: {res}")
+    return res
 
 
 def bench_batch(args, oracle_lib):

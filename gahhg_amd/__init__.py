@@ -77,7 +77,8 @@ EXPORTED = [
     "gahhg_polygon_grid_counts", "gahhg_export_polygon_grid", "gahhg_set_polygon_depths",
     "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
-    "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid",
+    "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
+    "gahhg_load_polygons_txt",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -149,6 +150,9 @@ def lib():
         "gahhg_autotune_query": ([P, P, P, i64, P, P], ctypes.c_int),
         "gahhg_save_grid": ([P, ctypes.c_char_p], ctypes.c_int),
         "gahhg_load_grid": ([P, ctypes.c_char_p], ctypes.c_int),
+        "gahhg_load_polygons_txt": ([P, ctypes.POINTER(ctypes.c_char_p), i32], ctypes.c_int),
+        "gahhg_write_polygon_report": ([P, i32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64,
+                                        ctypes.c_double, ctypes.c_double], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -326,6 +330,13 @@ class GPISP:
             np.asarray(roff, dtype=np.int64), np.asarray(proff, dtype=np.int32), len(polys)))
         return self
 
+    def load_polygons_txt(self, paths):
+        """A batch from polygon text files (polygon k = paths[k])."""
+        paths = [str(x).encode() for x in paths]
+        arr = (ctypes.c_char_p * len(paths))(*paths)
+        self._check(self.L.gahhg_load_polygons_txt(self.h, arr, len(paths)))
+        return self
+
     @property
     def n_polygons(self):
         n = ctypes.c_int32()
@@ -386,6 +397,11 @@ class GPISP:
         e = edges.astype(np.float64)
         e[..., 1:] /= FIXED_POINT
         return cells, e
+
+    def write_polygon_report(self, p, path, poly_name, query_name, n_queries, inside, query_ms, pre_ms):
+        self._check(self.L.gahhg_write_polygon_report(self.h, int(p), str(path).encode(), str(poly_name).encode(),
+                                                      str(query_name).encode(), int(n_queries), int(inside),
+                                                      float(query_ms), float(pre_ms)))
 
     def save_grid(self, path):
         self._check(self.L.gahhg_save_grid(self.h, str(path).encode()))

@@ -320,6 +320,20 @@ int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
   return upload_polygons(c);
 }
 
+int gahhg_load_polygons_txt(gahhg_ctx* c, const char* const* paths, int32_t n) {
+  RCK(ctx_ok(c));
+  if (n < 1 || !paths) return set_err(c, GAHHG_E_ARG, "batch: need polygon files");
+  RCK(select_device(c));
+  std::vector<Polygon> ps((size_t)n);
+  for (int k = 0; k < n; k++) {
+    if (!paths[k]) return set_err(c, GAHHG_E_ARG, "batch: null path %d", k);
+    const int rc = load_polygon_txt(paths[k], &ps[k], &c->err);
+    if (rc) return rc;
+  }
+  c->polys = std::move(ps);
+  return upload_polygons(c);
+}
+
 int gahhg_set_polygon_lonlat(gahhg_ctx* c, const double* lon, const double* lat, const int64_t* ring_off,
                              int32_t n_rings) {
   RCK(ctx_ok(c));
@@ -397,6 +411,8 @@ int gahhg_build(gahhg_ctx* c) {
 // int32 global levels + LevelDesc each; per polygon: N, flips, pv[3N], pe[2N],
 // PolyDev record, its levels, gamma; then the device arrays in the order of
 // grid_arrays(); finally the two undefined-leaf counts.  Native byte order.
+}  // extern "C"
+
 namespace {
 
 constexpr char kGridMagic[8] = {'G', 'A', 'H', 'H', 'G', 'G', 'R', '1'};
@@ -439,6 +455,8 @@ void finish_stats(gahhg_ctx* c) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int gahhg_save_grid(gahhg_ctx* c, const char* path) {
   RCK(ctx_ok(c));
@@ -859,8 +877,13 @@ int gahhg_get_stats(gahhg_ctx* c, gahhg_stats* out) {
   return GAHHG_OK;
 }
 
-int gahhg_write_report(gahhg_ctx* c, const char* path, const char* poly_name, const char* query_name) {
-  RCK(ctx_ok(c));
+}  // extern "C"
+
+namespace {
+
+// The report block of GPISP::exportStat2TXT (source/GPISPAHHG.cu:2115-2161).
+int write_report_block(gahhg_ctx* c, const char* path, const char* poly_name, const char* query_name,
+                       int64_t n_vertices, const gahhg_stats& st) {
   FILE* fp = path ? fopen(path, "a") : stdout;
   if (!fp) return set_err(c, GAHHG_E_IO, "Can not open file %s", path);
   time_t t = time(nullptr);
@@ -869,20 +892,52 @@ int gahhg_write_report(gahhg_ctx* c, const char* path, const char* poly_name, co
   fprintf(fp, "\n----------------------------------------%d/%d/%d %d:%d:%d-----------------------------------------\n",
           lt.tm_year + 1900, lt.tm_mon + 1, lt.tm_mday, lt.tm_hour, lt.tm_min, lt.tm_sec);
   fprintf(fp, "\nPolygon file: %s\n", poly_name ? poly_name : "");
-  fprintf(fp, "Number of polygon vertice: %lld\n", (long long)c->n_edges);
+  fprintf(fp, "Number of polygon vertice: %lld\n", (long long)n_vertices);
   fprintf(fp, "Query point file: %s\n", query_name ? query_name : "");
-  fprintf(fp, "Number of query points: %lld\n", (long long)c->st.n_queries);
-  fprintf(fp, "Depth of GAHHG: %d\n", c->st.depth);
+  fprintf(fp, "Number of query points: %lld\n", (long long)st.n_queries);
+  fprintf(fp, "Depth of GAHHG: %d\n", st.depth);
   fprintf(fp, "Time of preprocessing: %f seconds (creat GAHHG: %f seconds, classify GAHHG: %f seconds)\n",
-          c->st.pre_ms * 1e-3, c->st.create_ms * 1e-3, c->st.classify_ms * 1e-3);
-  fprintf(fp, "Storage overhead of GAHHG on CPU: %lld Bytes\n", (long long)c->st.cpu_space_bytes);
-  fprintf(fp, "Storage overhead of GAHHG on GPU: %lld KB\n", (long long)(c->st.gpu_space_bytes / 1000));
-  fprintf(fp, "Total test time for all query points: %f seconds\n", c->st.query_ms * 1e-3);
-  fprintf(fp, "Average test time per query point: %.5e seconds\n", c->st.query_unit_s);
-  fprintf(fp, "Test result: inside %lld, outside %lld\n", (long long)c->st.inside, (long long)c->st.outside);
+          st.pre_ms * 1e-3, st.create_ms * 1e-3, st.classify_ms * 1e-3);
+  fprintf(fp, "Storage overhead of GAHHG on CPU: %lld Bytes\n", (long long)st.cpu_space_bytes);
+  fprintf(fp, "Storage overhead of GAHHG on GPU: %lld KB\n", (long long)(st.gpu_space_bytes / 1000));
+  fprintf(fp, "Total test time for all query points: %f seconds\n", st.query_ms * 1e-3);
+  fprintf(fp, "Average test time per query point: %.5e seconds\n", st.query_unit_s);
+  fprintf(fp, "Test result: inside %lld, outside %lld\n", (long long)st.inside, (long long)st.outside);
   fprintf(fp, "\n---------------------------------------------------------------------------------------------------\n");
   if (path) fclose(fp);
   return GAHHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gahhg_write_report(gahhg_ctx* c, const char* path, const char* poly_name, const char* query_name) {
+  RCK(ctx_ok(c));
+  return write_report_block(c, path, poly_name, query_name, c->n_edges, c->st);
+}
+
+int gahhg_write_polygon_report(gahhg_ctx* c, int32_t p, const char* path, const char* poly_name,
+                               const char* query_name, int64_t n_queries, int64_t inside, double query_ms,
+                               double pre_ms) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "report before build");
+  if (p < 0 || p >= (int32_t)c->polys.size() || n_queries < 0 || inside < 0 || inside > n_queries)
+    return set_err(c, GAHHG_E_ARG, "polygon report: bad arguments");
+  gahhg_stats st;
+  poly_stats(c, p, &st);
+  st.n_queries = n_queries;
+  st.inside = inside;
+  st.outside = n_queries - inside;
+  st.query_ms = query_ms;
+  st.query_unit_s = n_queries > 0 ? query_ms * 1e-3 / (double)n_queries : 0.0;
+  st.pre_ms = pre_ms;
+  st.create_ms = st.classify_ms = 0.0;
+  if (c->polys.size() == 1) {
+    st.create_ms = c->st.create_ms;
+    st.classify_ms = c->st.classify_ms;
+  }
+  return write_report_block(c, path, poly_name, query_name, c->polys[p].N, st);
 }
 
 int gahhg_write_result_csv(const char* path, const int32_t* labels, int64_t n) {

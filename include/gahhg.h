@@ -138,6 +138,11 @@ int gahhg_query_xyz_device(gahhg_ctx* ctx, const double* d_xyz, int64_t n, int32
  * labels equal what a single-polygon context builds for it. */
 int gahhg_set_polygons_lonlat(gahhg_ctx* ctx, const double* lon, const double* lat, const int64_t* ring_off,
                               const int32_t* poly_ring_off, int32_t n_polys);
+/* A batch read from n polygon files, each as gahhg_load_polygon_txt
+ * (GPISPPolygon::loadPolyFromTXT, source/GPISPPolygon.cu:32); polygon k is
+ * paths[k].  Replaces the reference's one process per file of
+ * test/explist.bat with one batch. */
+int gahhg_load_polygons_txt(gahhg_ctx* ctx, const char* const* paths, int32_t n);
 int gahhg_num_polygons(gahhg_ctx* ctx, int32_t* n);
 /* Queries against a batch: poly[i] selects the polygon of point i. */
 int gahhg_query_batch_xyz(gahhg_ctx* ctx, const double* xyz, const int32_t* poly, int64_t n, int32_t* labels);
@@ -255,6 +260,12 @@ int gahhg_get_stats(gahhg_ctx* ctx, gahhg_stats* out);
  * report block; names are the polygon / query file names to print. */
 int gahhg_write_report(gahhg_ctx* ctx, const char* path, const char* poly_name,
                        const char* query_name);
+/* The same report block for polygon p of a batch, with the caller's query
+ * figures for it (points, points inside, query and build milliseconds; a
+ * batch shares its build and query launches, so callers pass their share). */
+int gahhg_write_polygon_report(gahhg_ctx* ctx, int32_t p, const char* path, const char* poly_name,
+                               const char* query_name, int64_t n_queries, int64_t inside, double query_ms,
+                               double pre_ms);
 /* GPISPTP::exportResult2CSV (source/GPISPTP.cu:81-99): "1" for IN else "0". */
 int gahhg_write_result_csv(const char* path, const int32_t* labels, int64_t n);
 
