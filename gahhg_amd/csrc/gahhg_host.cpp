@@ -4,7 +4,8 @@
 // Orientation rule: source/GPISPPolygon.cu:280-313.
 // LL -> XYZ (y up): source/GPISPPolygon.cuh:21-30; query points go through
 // the FakeLLVertex round trip of source/GPISPPolygon.cuh:72,82-92.
-// Transcendentals run here on the host (glibc); the GPU never evaluates one.
+// Lon/lat conversions use correctly rounded sin and cos (gahhg_trig.h, shared
+// with the device loader); the level-0 grid's constants use the C library.
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -16,15 +17,21 @@
 #include <vector>
 
 #include "gahhg_internal.h"
+#include "gahhg_trig.h"
 
 #pragma clang fp contract(off)
 
 namespace gahhg {
 
+// Correctly rounded sin and cos (gahhg_trig.h), the same function as the
+// device conversion, so both give identical bits.
 static V3 ll_to_xyz(double lo, double la) {
   double a = la * kPi / 180.0;
   double o = lo * kPi / 180.0;
-  return v3(cos(a) * sin(o), sin(a), cos(a) * cos(o));
+  double sa, ca, so, co;
+  trig::sincos_cr(a, &sa, &ca);
+  trig::sincos_cr(o, &so, &co);
+  return v3(ca * so, sa, ca * co);
 }
 
 void query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz) {

@@ -27,6 +27,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <quadmath.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -138,14 +139,48 @@ const char* og_last_error(og_ctx* c) { return c ? c->err : "null context"; }
 /* ---------------------------------------------------------------------- */
 /* Lon/lat conversions.                                                    */
 /* ---------------------------------------------------------------------- */
+/* sin and cos correctly rounded to double.  The reference calls the
+ * platform's sin and cos, whose last bit depends on the C library; this
+ * oracle (and the product) use the correctly rounded values, a function of
+ * the argument alone.  Here: sinl/cosl (64-bit significand; within 1.3 units
+ * of their last place on |x| <= 2 pi, measured against the quad routines)
+ * decide the rounding unless their value lies within 8 such units of a
+ * midpoint between two doubles; then sinq/cosq (113-bit significand) decide. */
+static int near_double_midpoint(long double v) {
+    double d = (double)v;
+    if ((long double)d == v) return 0;
+    double n = nextafter(d, v > (long double)d ? INFINITY : -INFINITY);
+    long double mid = ((long double)d + (long double)n) * 0.5L;  /* exact */
+    long double unit = ldexpl(1.0L, ilogbl(v) - 63);
+    return fabsl(v - mid) <= 8.0L * unit;
+}
+
+static double cr_sin(double x) {
+    long double v = sinl((long double)x);
+    return near_double_midpoint(v) ? (double)sinq((__float128)x) : (double)v;
+}
+
+static double cr_cos(double x) {
+    long double v = cosl((long double)x);
+    return near_double_midpoint(v) ? (double)cosq((__float128)x) : (double)v;
+}
+
+void og_sincos_cr(const double* x, int64_t n, double* s, double* c) {
+    for (int64_t i = 0; i < n; i++) {
+        s[i] = cr_sin(x[i]);
+        c[i] = cr_cos(x[i]);
+    }
+}
+
 /* LLVertex::LL2XYZ, source/GPISPPolygon.cuh:21-30 (y is up). */
 static inline v3 ll2xyz(double lo, double la) {
     double a = la * OG_PI / 180.0;
     double o = lo * OG_PI / 180.0;
+    double ca = cr_cos(a);
     v3 p;
-    p.x = cos(a) * sin(o);
-    p.y = sin(a);
-    p.z = cos(a) * cos(o);
+    p.x = ca * cr_sin(o);
+    p.y = cr_sin(a);
+    p.z = ca * cr_cos(o);
     return p;
 }
 
@@ -153,13 +188,7 @@ static inline v3 ll2xyz(double lo, double la) {
 static inline v3 fakell2xyz(double lo, double la) {
     double fakela = la + 90;
     double realla = fakela > 180 ? 270 - fakela : fakela - 90;
-    double a = realla * OG_PI / 180.0;
-    double o = lo * OG_PI / 180.0;
-    v3 p;
-    p.x = cos(a) * sin(o);
-    p.y = sin(a);
-    p.z = cos(a) * cos(o);
-    return p;
+    return ll2xyz(lo, realla);
 }
 
 void og_ll2xyz(const double* lon, const double* lat, int64_t n, double* xyz) {

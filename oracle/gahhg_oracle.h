@@ -71,6 +71,9 @@ int32_t og_orientation_flips(og_ctx* c);  /* bit r set: ring r was reversed */
 void og_get_polygon(og_ctx* c, double* pv /*3N*/, int32_t* pe /*2N*/);
 
 /* Point conversions: source/GPISPPolygon.cuh:21-30 and :72,82-92. */
+/* sin and cos of x[i], correctly rounded (certified with long double and,
+ * near a rounding midpoint, quad precision). */
+void og_sincos_cr(const double* x, int64_t n, double* s, double* c);
 void og_ll2xyz(const double* lon, const double* lat, int64_t n, double* xyz);
 void og_query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n,
                             double* xyz);

@@ -61,6 +61,7 @@ def lib():
         L.og_get_polygon.argtypes = [P, dp, ip]
         L.og_ll2xyz.argtypes = [dp, dp, ctypes.c_int64, dp]
         L.og_query_lonlat_to_xyz.argtypes = [dp, dp, ctypes.c_int64, dp]
+        L.og_sincos_cr.argtypes = [dp, ctypes.c_int64, dp, dp]
         L.og_load_query_csv.argtypes = [P, ctypes.c_char_p, P, P, ctypes.c_int64]
         L.og_load_query_csv.restype = ctypes.c_int64
         L.og_grid0.argtypes = [dp, ip, dp, ip]
@@ -85,6 +86,15 @@ def lib():
         L.og_kat_edge_border_inter.restype = ctypes.c_int32
         _lib = L
     return _lib
+
+
+def sincos_cr(x):
+    """Correctly rounded sin and cos of every element of x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().og_sincos_cr(x, x.size, s, c)
+    return s, c
 
 
 def grid0():

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import brute_force as bf
+import exact_trig
 import oracle_lib as ol
 from gahhg_amd import datasets
 
@@ -70,7 +71,8 @@ def test_ll2xyz_closed_form():
     for i in range(len(lon)):
         a = lat[i] * PI / 180.0
         o = lon[i] * PI / 180.0
-        exp = (math.cos(a) * math.sin(o), math.sin(a), math.cos(a) * math.cos(o))
+        ca = exact_trig.cr_cos(a)
+        exp = (ca * exact_trig.cr_sin(o), exact_trig.cr_sin(a), ca * exact_trig.cr_cos(o))
         assert tuple(got[i]) == exp
     # y is up, x east of the lon=0 meridian at lon=90
     assert got[0].tolist() == [0.0, 0.0, 1.0]
