@@ -78,7 +78,8 @@ EXPORTED = [
     "gahhg_trace_query_device", "gahhg_build_trace", "gahhg_mlp_forward", "gahhg_mlp_backward",
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
-    "gahhg_load_polygons_txt",
+    "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
+    "gahhg_load_polygon_txt_device",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -151,6 +152,9 @@ def lib():
         "gahhg_save_grid": ([P, ctypes.c_char_p], ctypes.c_int),
         "gahhg_load_grid": ([P, ctypes.c_char_p], ctypes.c_int),
         "gahhg_load_polygons_txt": ([P, ctypes.POINTER(ctypes.c_char_p), i32], ctypes.c_int),
+        "gahhg_load_query_csv_device": ([P, ctypes.c_char_p, P, P, P], ctypes.c_int),
+        "gahhg_parse_query_csv_device": ([P, ctypes.c_char_p, i64, P, P, P], ctypes.c_int),
+        "gahhg_load_polygon_txt_device": ([P, ctypes.c_char_p, P], ctypes.c_int),
         "gahhg_write_polygon_report": ([P, i32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64, i64,
                                         ctypes.c_double, ctypes.c_double], ctypes.c_int),
     }
@@ -201,6 +205,29 @@ def write_result_csv(path, labels):
     rc = lib().gahhg_write_result_csv(str(path).encode(), labels, labels.size)
     if rc:
         raise GahhgError(rc, f"write {path}")
+
+
+class LoadInfo(ctypes.Structure):
+    _fields_ = [("lines", ctypes.c_int64), ("bytes", ctypes.c_int64), ("host_lines", ctypes.c_int64),
+                ("read_ms", ctypes.c_double), ("parse_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return dict((f, getattr(self, f)) for f, _ in self._fields_)
+
+
+class DeviceView:
+    """Device memory owned by the context (not freed by the caller), e.g. the
+    points of a device-side load; valid until the context's next load."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx = ctx
+        self.ptr = ctypes.c_void_p(ptr)
+        self.nbytes = int(nbytes)
+
+    def download(self, arr):
+        assert arr.flags["C_CONTIGUOUS"] and arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.L.gahhg_memcpy_d2h(self.ctx.h, arr.ctypes.data, self.ptr, arr.nbytes))
+        return arr
 
 
 class DeviceBuffer:
@@ -270,6 +297,37 @@ class GPISP:
         return self
 
     loadPolyFromTXT = load_polygon_txt
+
+    def load_polygon_txt_device(self, path):
+        """load_polygon_txt with the vertex lines parsed on the GPU; the load
+        figures are kept in self.load_info."""
+        info = LoadInfo()
+        self._check(self.L.gahhg_load_polygon_txt_device(self.h, str(path).encode(), ctypes.byref(info)))
+        self.load_info = info.as_dict()
+        return self
+
+    # -- device-side query loading -------------------------------------------
+    def load_query_csv_device(self, path):
+        """Query CSV parsed and converted on the GPU.  Returns (DeviceView of
+        n x 3 doubles, n); the load figures are kept in self.load_info."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        info = LoadInfo()
+        self._check(self.L.gahhg_load_query_csv_device(self.h, str(path).encode(), ctypes.byref(p),
+                                                       ctypes.byref(n), ctypes.byref(info)))
+        self.load_info = info.as_dict()
+        return DeviceView(self, p.value, 24 * n.value), n.value
+
+    def parse_query_csv_device(self, text):
+        """As load_query_csv_device, from the file's bytes in host memory."""
+        text = bytes(text)
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        info = LoadInfo()
+        self._check(self.L.gahhg_parse_query_csv_device(self.h, text, len(text), ctypes.byref(p),
+                                                        ctypes.byref(n), ctypes.byref(info)))
+        self.load_info = info.as_dict()
+        return DeviceView(self, p.value, 24 * n.value), n.value
 
     def set_polygon_lonlat(self, lon, lat, ring_off):
         self._check(self.L.gahhg_set_polygon_lonlat(

@@ -294,6 +294,7 @@ void gahhg_destroy(gahhg_ctx* c) {
   dfree(c->qsort.buf);
   dfree(c->d_mlp_part);
   learn_free(c);
+  free_text_load(c);
   dfree(c->d_pv);
   dfree(c->d_pe);
   dfree(c->d_edge);
@@ -314,6 +315,49 @@ int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
   RCK(select_device(c));
   Polygon p;
   int rc = load_polygon_txt(path, &p, &c->err);
+  if (rc) return rc;
+  c->polys.clear();
+  c->polys.push_back(std::move(p));
+  return upload_polygons(c);
+}
+
+int gahhg_load_query_csv_device(gahhg_ctx* c, const char* path, const double** d_xyz, int64_t* n,
+                                gahhg_load_info* info) {
+  RCK(ctx_ok(c));
+  if (!d_xyz || !n) return set_err(c, GAHHG_E_ARG, "null output");
+  RCK(select_device(c));
+  if (info) memset(info, 0, sizeof(*info));
+  RCK(load_query_csv_device(c, path, n, info));
+  *d_xyz = c->tl.xyz;
+  return GAHHG_OK;
+}
+
+int gahhg_parse_query_csv_device(gahhg_ctx* c, const char* text, int64_t bytes, const double** d_xyz,
+                                 int64_t* n, gahhg_load_info* info) {
+  RCK(ctx_ok(c));
+  if (!d_xyz || !n || bytes < 0 || (bytes > 0 && !text)) return set_err(c, GAHHG_E_ARG, "bad text arguments");
+  RCK(select_device(c));
+  if (info) memset(info, 0, sizeof(*info));
+  RCK(parse_query_csv_device(c, text, bytes, n, info));
+  *d_xyz = c->tl.xyz;
+  return GAHHG_OK;
+}
+
+int gahhg_load_polygon_txt_device(gahhg_ctx* c, const char* path, gahhg_load_info* info) {
+  RCK(ctx_ok(c));
+  if (!path) return set_err(c, GAHHG_E_ARG, "null path");
+  RCK(select_device(c));
+  if (info) memset(info, 0, sizeof(*info));
+  Polygon p;
+  int rc = load_polygon_txt_device(c, path, &p, info);
+  if (rc == GAHHG_E_FORMAT) {
+    // a malformed file: the host loader reads it in file order and names
+    // the first fault, as the reference does
+    Polygon q;
+    const int rc2 = load_polygon_txt(path, &q, &c->err);
+    if (rc2) return rc2;
+    return set_err(c, GAHHG_E_STATE, "device and host polygon loaders disagree on %s", path);
+  }
   if (rc) return rc;
   c->polys.clear();
   c->polys.push_back(std::move(p));

@@ -69,7 +69,9 @@ struct Ring {
   int32_t n;  // unique vertices
 };
 
-static int assemble(const double* lo, const double* la, const std::vector<Ring>& rings,
+// xyz, when given, holds every vertex's point (parallel to lo/la), as the
+// device loader computed it with the same conversion.
+static int assemble(const double* lo, const double* la, const double* xyz, const std::vector<Ring>& rings,
                     Polygon* out, std::string* err) {
   int64_t total = 0;
   for (const Ring& r : rings) total += r.n;
@@ -82,8 +84,16 @@ static int assemble(const double* lo, const double* la, const std::vector<Ring>&
   out->pv.assign(3 * total, 0.0);
   out->pe.assign(2 * total, 0);
   int64_t k = 0;
-  for (const Ring& r : rings)
-    for (int j = 0; j < r.n; j++) store3(&out->pv[3 * k++], ll_to_xyz(lo[r.start + j], la[r.start + j]));
+  for (const Ring& r : rings) {
+    double* pv = &out->pv[3 * k];
+    if (xyz) {
+      memcpy(pv, xyz + 3 * r.start, 24 * (size_t)r.n);
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int j = 0; j < r.n; j++) store3(pv + 3 * j, ll_to_xyz(lo[r.start + j], la[r.start + j]));
+    }
+    k += r.n;
+  }
   // Edge list: forward rings link i -> i+1; reversed rings walk the indices
   // downwards; the last edge closes the ring.
   int32_t idx = 0;
@@ -105,8 +115,17 @@ static int assemble(const double* lo, const double* la, const std::vector<Ring>&
   return GAHHG_OK;
 }
 
+void lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz) {
+  for (int64_t i = 0; i < n; i++) store3(xyz + 3 * i, ll_to_xyz(lon[i], lat[i]));
+}
+
 int polygon_from_lonlat(const double* lon, const double* lat, const int64_t* ring_off,
                         int32_t n_rings, Polygon* out, std::string* err) {
+  return polygon_from_lonlat_xyz(lon, lat, nullptr, ring_off, n_rings, out, err);
+}
+
+int polygon_from_lonlat_xyz(const double* lon, const double* lat, const double* xyz, const int64_t* ring_off,
+                            int32_t n_rings, Polygon* out, std::string* err) {
   if (n_rings < 1 || !lon || !lat || !ring_off) {
     *err = "polygon: need at least one ring";
     return GAHHG_E_ARG;
@@ -120,7 +139,7 @@ int polygon_from_lonlat(const double* lon, const double* lat, const int64_t* rin
     }
     rings.push_back(Ring{ring_off[r], (int32_t)(cnt - 1)});
   }
-  return assemble(lon, lat, rings, out, err);
+  return assemble(lon, lat, xyz, rings, out, err);
 }
 
 namespace {
@@ -181,7 +200,7 @@ int load_polygon_txt(const char* path, Polygon* out, std::string* err) {
     }
   }
   fclose(f);
-  return assemble(lo.data(), la.data(), rings, out, err);
+  return assemble(lo.data(), la.data(), nullptr, rings, out, err);
 }
 
 int read_query_csv(const char* path, std::vector<double>* lon, std::vector<double>* lat,

@@ -251,11 +251,42 @@ struct Polygon {
 int load_polygon_txt(const char* path, Polygon* out, std::string* err);
 int polygon_from_lonlat(const double* lon, const double* lat, const int64_t* ring_off,
                         int32_t n_rings, Polygon* out, std::string* err);
+// As polygon_from_lonlat with each vertex's point already converted (xyz
+// parallel to lon/lat, closing vertices included; null = convert here).
+int polygon_from_lonlat_xyz(const double* lon, const double* lat, const double* xyz, const int64_t* ring_off,
+                            int32_t n_rings, Polygon* out, std::string* err);
+// LLVertex::LL2XYZ of n points (correctly rounded sin and cos).
+void lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
 void build_grid0(double* v80, int32_t* bi240x6, double* bn240x3, int32_t* ci42x7);
 void query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
 int read_query_csv(const char* path, std::vector<double>* lon, std::vector<double>* lat,
                    std::string* err);
 
+}  // namespace gahhg
+
+namespace gahhg {
+// Device text loading (gahhg_load.hip): the file's bytes, line offsets,
+// parsed output, listed-line indices, per-block counts, pinned staging.
+struct TextLoad {
+  uint8_t* text = nullptr;
+  int64_t text_cap = 0;
+  int64_t* starts = nullptr;
+  int64_t starts_cap = 0;
+  double* xyz = nullptr;
+  int64_t xyz_cap = 0;
+  int32_t* flags = nullptr;
+  int64_t flags_cap = 0;
+  int32_t* counts = nullptr;
+  int64_t counts_cap = 0;
+  int32_t* misc = nullptr;
+  int64_t misc_cap = 0;
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+};
+int load_query_csv_device(gahhg_ctx* c, const char* path, int64_t* n, gahhg_load_info* info);
+int parse_query_csv_device(gahhg_ctx* c, const char* text, int64_t bytes, int64_t* n, gahhg_load_info* info);
+int load_polygon_txt_device(gahhg_ctx* c, const char* path, Polygon* out, gahhg_load_info* info);
+void free_text_load(gahhg_ctx* c);
 }  // namespace gahhg
 
 struct gahhg_ctx {
@@ -303,6 +334,7 @@ struct gahhg_ctx {
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
   gahhg::LearnState learn;
+  gahhg::TextLoad tl;
   float* d_mlp_part = nullptr;       // per-wave partial gradients (grow-only)
   int64_t mlp_part_cap = 0;
   int64_t und_snap = 0, und_final = 0;

@@ -28,8 +28,10 @@
 
 #ifdef __HIPCC__
 #define GAHHG_TRIG_FN __host__ __device__ inline
+#define GAHHG_TRIG_UNROLL _Pragma("unroll")
 #else
 #define GAHHG_TRIG_FN inline
+#define GAHHG_TRIG_UNROLL
 #endif
 
 namespace gahhg {
@@ -132,22 +134,24 @@ GAHHG_TRIG_FN void sincos_cr(double x, double* s, double* c) {
   const DD z = dd_mul(r, r);
   // sin r = r + r * sum_{i>=1} kSinC[i-1] z^i; terms i >= 8 in double
   double a = kSinC[13][0];
-#pragma unroll
+GAHHG_TRIG_UNROLL
   for (int i = 12; i >= 7; i--) a = kSinC[i][0] + z.hi * a;
   DD A = {a, 0.0};
-#pragma unroll
+GAHHG_TRIG_UNROLL
   for (int i = 6; i >= 0; i--) A = dd_add(DD{kSinC[i][0], kSinC[i][1]}, dd_mul(z, A));
   const DD sr = dd_add(r, dd_mul(r, dd_mul(z, A)));
   // cos r = 1 + sum_{i>=1} kCosC[i-1] z^i; terms i >= 9 in double
   double b = kCosC[14][0];
-#pragma unroll
+GAHHG_TRIG_UNROLL
   for (int i = 13; i >= 8; i--) b = kCosC[i][0] + z.hi * b;
   DD B = {b, 0.0};
-#pragma unroll
+GAHHG_TRIG_UNROLL
   for (int i = 7; i >= 0; i--) B = dd_add(DD{kCosC[i][0], kCosC[i][1]}, dd_mul(z, B));
   const DD cr = dd_add(DD{1.0, 0.0}, dd_mul(z, B));
-  const long long k = (long long)kd;
-  switch ((int)(k & 3)) {
+  // quadrant kd mod 4, exact for any integer-valued kd (kd * 0.25, floor
+  // and the products are exact; beyond 2^54 kd is a multiple of 4)
+  const int k = (int)(kd - 4.0 * floor(kd * 0.25));
+  switch (k) {
     case 0: *s = sr.hi; *c = cr.hi; break;
     case 1: *s = cr.hi; *c = -sr.hi; break;
     case 2: *s = -sr.hi; *c = -cr.hi; break;

@@ -107,6 +107,42 @@ def write_query_csv(path, lon, lat):
             f.write(f"{a:.17g},{b:.17g}\n")
 
 
+def _fixed_columns(v, decimals):
+    """v as fixed-point text, one row of bytes per value, right-aligned with
+    0 bytes as padding."""
+    scale = 10 ** decimals
+    a = np.round(np.abs(v) * scale).astype(np.int64)
+    ip, fp = a // scale, a % scale
+    ndig = np.ones(len(v), dtype=np.int64)
+    for k in range(1, 19):
+        ndig += ip >= 10 ** k
+    maxd = int(ndig.max()) if len(v) else 1
+    width = 1 + maxd + 1 + decimals
+    m = np.zeros((len(v), width), dtype=np.uint8)
+    for k in range(decimals):
+        m[:, width - 1 - k] = ord("0") + (fp // 10 ** k) % 10
+    m[:, width - 1 - decimals] = ord(".")
+    for t in range(maxd):
+        col = width - 2 - decimals - t
+        m[:, col] = np.where(t < ndig, ord("0") + (ip // 10 ** t) % 10, 0)
+    neg = np.signbit(v)
+    rows = np.nonzero(neg)[0]
+    m[rows, width - 2 - decimals - ndig[rows]] = ord("-")
+    return m
+
+
+def query_csv_bytes(lon, lat, decimals=6):
+    """A query CSV ("lon,lat" per line, fixed-point with `decimals` digits,
+    as the reference's query files are written), built with array
+    operations for large point counts."""
+    a = _fixed_columns(np.asarray(lon, dtype=np.float64), decimals)
+    b = _fixed_columns(np.asarray(lat, dtype=np.float64), decimals)
+    n = len(a)
+    m = np.concatenate([a, np.full((n, 1), ord(","), np.uint8), b, np.full((n, 1), ord("\n"), np.uint8)], axis=1)
+    flat = m.reshape(-1)
+    return flat[flat != 0].tobytes()
+
+
 def uniform_queries(rng, n):
     lon = rng.uniform(-180.0, 180.0, n)
     lat = np.degrees(np.arcsin(rng.uniform(-1.0, 1.0, n)))

@@ -131,6 +131,34 @@ int gahhg_query_xyz(gahhg_ctx* ctx, const double* xyz, int64_t n, int32_t* label
  * and waits for it.  This is GPISP::test's timed region. */
 int gahhg_query_xyz_device(gahhg_ctx* ctx, const double* d_xyz, int64_t n, int32_t* d_labels);
 
+/* Text input parsed on the GPU (GPISPTP::loadTPFromCSV, source/GPISPTP.cu:14-77,
+ * and GPISPPolygon::loadPolyFromTXT, source/GPISPPolygon.cu:32-278): the
+ * file's bytes are copied to device memory once; lines are split, numbers
+ * parsed and points converted on the device.  Values equal the host
+ * loaders' on every line (lines the device does not convert exactly are
+ * parsed on the host with the reference's sscanf formats); a malformed file
+ * gives the host loader's error. */
+typedef struct gahhg_load_info {
+  int64_t lines;       /* lines in the file */
+  int64_t bytes;       /* file size */
+  int64_t host_lines;  /* lines parsed on the host */
+  double read_ms;      /* file read + copy to the device */
+  double parse_ms;     /* line split + parse + conversion on the device */
+  double total_ms;     /* the whole call */
+} gahhg_load_info;
+
+/* Query CSV ("%lf,%lf" = longitude, latitude per line) -> *d_xyz, n points
+ * (FakeLL2XYZ) in device memory owned by the context, valid until the next
+ * device load or gahhg_destroy; pass it to gahhg_query_xyz_device.  info may
+ * be null. */
+int gahhg_load_query_csv_device(gahhg_ctx* ctx, const char* path, const double** d_xyz, int64_t* n,
+                                gahhg_load_info* info);
+/* The same from text already in host memory (bytes long). */
+int gahhg_parse_query_csv_device(gahhg_ctx* ctx, const char* text, int64_t bytes, const double** d_xyz,
+                                 int64_t* n, gahhg_load_info* info);
+/* gahhg_load_polygon_txt with the vertex lines parsed on the device. */
+int gahhg_load_polygon_txt_device(gahhg_ctx* ctx, const char* path, gahhg_load_info* info);
+
 /* Batches (no counterpart in the reference, which handles one polygon per
  * process): n_polys polygons whose rings are ring_off[poly_ring_off[k] ..
  * poly_ring_off[k+1]] (each ring as in gahhg_set_polygon_lonlat).  All are
