@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--no-series", action="store_true")
+    ap.add_argument("--no-load", action="store_true")
+    ap.add_argument("--load-lines", type=int, default=20_000_000)
     ap.add_argument("--rule", action="store_true", help="density-rule depth only (no learned model)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size reference for "
@@ -274,7 +276,75 @@ def main():
         out["batch_256"] = bench_batch(args, oracle_lib)
     if not args.no_series:
         out["series_mars"] = bench_series(oracle_lib)
+    if not args.no_load:
+        out["load_path"] = bench_load(args)
     print(json.dumps(out), flush=True)
+
+
+def bench_load(args):
+    """A query CSV of --load-lines points ("lon,lat" with 6 decimals, as the
+    reference's query files) from the file to device-resident xyz: the host
+    loader (sscanf per line, conversion on the host threads, copy to the
+    device) against the device loader (file bytes copied once, split, parse
+    and conversion on the GPU).  Both give the same bits."""
+    import shutil
+    import tempfile
+
+    import gahhg_amd
+    from gahhg_amd import datasets
+
+    rng = np.random.default_rng(17)
+    n = args.load_lines
+    t0 = time.time()
+    lon = rng.uniform(-180.0, 180.0, n)
+    lat = np.degrees(np.arcsin(rng.uniform(-1.0, 1.0, n)))
+    text = datasets.query_csv_bytes(lon, lat, decimals=6)
+    del lon, lat
+    tmp = tempfile.mkdtemp(prefix="gahhg_load_")
+    path = os.path.join(tmp, "queries.csv")
+    with open(path, "wb") as f:
+        f.write(text)
+    nbytes = len(text)
+    del text
+    log(f"load_path: {n} lines, {nbytes / 1e6:.1f} MB of text written in {time.time() - t0:.1f}s")
+    g = gahhg_amd.GPISP(device=0)
+    try:
+        # host loader
+        t = time.perf_counter()
+        hlon, hlat = gahhg_amd.read_query_csv(path)
+        t_read = time.perf_counter()
+        hxyz = gahhg_amd.query_lonlat_to_xyz(hlon, hlat)
+        t_conv = time.perf_counter()
+        d = g.alloc(hxyz.nbytes)
+        d.upload(hxyz)
+        t_up = time.perf_counter()
+        host = {"read_parse_ms": round((t_read - t) * 1e3, 1), "convert_ms": round((t_conv - t_read) * 1e3, 1),
+                "upload_ms": round((t_up - t_conv) * 1e3, 1), "total_ms": round((t_up - t) * 1e3, 1),
+                "convert_threads": os.cpu_count()}
+        d.free()
+        del hlon, hlat
+        # device loader
+        infos = []
+        for _ in range(3):
+            v, m = g.load_query_csv_device(path)
+            infos.append(g.load_info)
+        dxyz = v.download(np.empty((m, 3)))
+        same = bool(m == len(hxyz) and np.array_equal(dxyz.view(np.int64), hxyz.view(np.int64)))
+        del dxyz, hxyz
+        tot = float(np.median([i["total_ms"] for i in infos]))
+        par = float(np.median([i["parse_ms"] for i in infos]))
+        rd = float(np.median([i["read_ms"] for i in infos]))
+        res = {"lines": n, "text_bytes": nbytes, "host_loader": host,
+               "device_loader": {"total_ms": round(tot, 2), "read_copy_ms": round(rd, 2), "parse_ms": round(par, 3),
+                                 "host_lines": int(infos[-1]["host_lines"]),
+                                 "parse_lines_per_s": round(n / (par * 1e-3), 1),
+                                 "parse_text_GB_per_s": round(nbytes / (par * 1e-3) / 1e9, 2)},
+               "speedup_total": round(host["total_ms"] / tot, 2), "bit_identical": same}
+    finally:
+        g.close()
+        shutil.rmtree(tmp, ignore_errors=True)
+    log(f"load_path: {res}")
+    return res
 
 
 def bench_series(oracle_lib):
