@@ -28,6 +28,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+def xcd_kw(args):
+    return {} if args.query_xcd is None else {"query_xcd": args.query_xcd}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -108,6 +112,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bins", type=int, default=256)
+    ap.add_argument("--query-xcd", type=int, default=None,
+                    help="blocks per XCD run in the query kernel (0 = plain order; default: the library's)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--no-series", action="store_true")
@@ -135,7 +141,7 @@ def main():
     model = load_depth_model(args)
     # the density-rule build of the accelerated path (the learned build is
     # compared against it)
-    g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **xcd_kw(args))
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
     rule_pre, rule_create, rule_classify = median_build(g, args.builds)
     rule_depth = g.stats()["depth"]
@@ -408,7 +414,7 @@ def bench_batch(args, oracle_lib):
     n_edges = int(sum(len(p[0]) - (len(p[2]) - 1) for p in items))
     log(f"batch_256: {len(items)} polygons, {n_edges} edges, {Q} queries generated in {time.time() - t0:.1f}s")
     model = load_depth_model(args)
-    g = gahhg_amd.GPISP(device=0, query_bins=args.bins)
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **xcd_kw(args))
     g.set_polygons_lonlat(items)
     rule_build = median_build(g, 3)[0]
     rule_depths = [g.polygon_stats(k)["depth"] for k in range(len(items))]

@@ -965,6 +965,7 @@ int build_hierarchy(gahhg_ctx* c) {
   CK(fill_i32(c, c->d_loc_prev, Nt, 0));
 
   GridView g = view(c);
+  c->pmarks.assign(c->pd.size(), 0);
   int lv = 0;
   for (;; lv++) {
     const LevelDesc GL = c->glv[lv];
@@ -995,6 +996,7 @@ int build_hierarchy(gahhg_ctx* c) {
       break;
     }
     if (c->prm.verbose) printf("Subdivide level %d on GPU.\n", lv);
+    for (size_t p = 0; p < c->pd.size(); p++) c->pmarks[p] += c->pd[p].refining ? 1 : 0;
     CK(ensure_flags(c, (int64_t)GL.cnum + 1));
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
                        lv == 0 ? 1 : 0);

@@ -129,6 +129,7 @@ void poly_stats(gahhg_ctx* c, int p, gahhg_stats* o) {
   // + 96), GPISPAHHG.cu:2100.
   o->cpu_space_bytes = 792 + 28LL * depth;
   o->orientation_flipped = c->polys[p].flips;
+  o->mark_passes = p < (int)c->pmarks.size() ? c->pmarks[p] : 0;
 }
 
 // Segment map from global to one polygon's local indices.
@@ -243,6 +244,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_sub = 1;
   p->query_sub_budget = 4;
   p->query_sort = 1;
+  p->query_xcd = 0;
   return GAHHG_OK;
 }
 
@@ -260,7 +262,7 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       c->prm.query_bins > 4096 || c->prm.query_group < 0 || c->prm.query_group > 64 ||
       (c->prm.query_group & (c->prm.query_group - 1)) != 0 || c->prm.query_sub < 0 || c->prm.query_sub > 256 ||
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
-      c->prm.query_sort > 2) {
+      c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -432,6 +434,7 @@ int gahhg_build(gahhg_ctx* c) {
       tot.gpu_space_bytes += ps.gpu_space_bytes;
       tot.cpu_space_bytes += ps.cpu_space_bytes;
       tot.depth = std::max(tot.depth, ps.depth);
+      tot.mark_passes = std::max(tot.mark_passes, ps.mark_passes);
     }
     tot.n_edges = c->n_edges;
     c->st = tot;

@@ -76,6 +76,26 @@ __device__ __forceinline__ void wave_add(T* counter, int key, T v) {
 }
 #endif
 
+#ifdef __HIPCC__
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
+// (consecutive blocks go to consecutive XCDs), and each XCD has its own L2.
+// With chunk K > 0 this returns a logical block index under which each XCD
+// runs K consecutive logical blocks at a time: within every run of 8K
+// blocks, XCD j (the blocks b with b % 8 == j) takes logical blocks
+// [j*K, j*K + K); blocks past the last full run keep their index (a
+// bijection).  Neighbouring work then shares one L2, while the runs still
+// interleave over the launch so that no XCD gets a whole expensive region.
+__device__ __forceinline__ int64_t xcd_block(int K) {
+  const int64_t b = blockIdx.x;
+  if (K <= 0) return b;
+  const int64_t super = 8 * (int64_t)K;
+  const int64_t full = ((int64_t)gridDim.x / super) * super;
+  if (b >= full) return b;
+  const int64_t s = b / super, w = b - s * super;
+  return s * super + (w & 7) * K + (w >> 3);
+}
+#endif
+
 // Fixed-point scale of the per top-level cell edge sums (2^-40 rad).
 constexpr double kFix = 1099511627776.0;
 
@@ -308,6 +328,7 @@ struct gahhg_ctx {
   std::vector<std::vector<gahhg::LevelDesc>> plv;  // per polygon: global segments per level
   std::vector<float> pgamma;                        // per polygon density ratio
   std::vector<int32_t> depth_override;              // per polygon fixed depth (0 = rule)
+  std::vector<int32_t> pmarks;                      // per polygon levels marked in the last build
   std::vector<gahhg::LevelDesc> glv;                // per level: global ranges over all polygons
   int64_t tot_v = 0, tot_b = 0, tot_c = 0, tot_r = 0;
 

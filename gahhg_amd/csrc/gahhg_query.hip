@@ -609,10 +609,11 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
                                                    const int32_t* __restrict__ qpoly,
                                                    const int32_t* __restrict__ order, int64_t n,
                                                    int32_t* __restrict__ labels, LevelTab lt,
-                                                   int32_t* __restrict__ trace, int G_override) {
+                                                   int32_t* __restrict__ trace, int G_override, int xcd) {
   // thread t handles point order[t] (points grouped by polygon and start
-  // bin, so that a wave's gathers share cache lines), or point t
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // bin, so that a wave's gathers share cache lines), or point t; with xcd,
+  // each XCD takes a contiguous run of t
+  const int64_t t = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const int64_t q = (order && t < n) ? (int64_t)order[t] : t;
   const int lane = threadIdx.x & 63;
   V3 p = v3(0.0, 0.0, 0.0);
@@ -917,7 +918,7 @@ static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
                      (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
                      L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const double*)L.sv,
                      (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_labels, level_tab(c),
-                     d_trace, c->prm.query_group);
+                     d_trace, c->prm.query_group, c->prm.query_xcd);
 }
 
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {

@@ -61,6 +61,11 @@ typedef struct {
   int32_t query_sort;      /* accelerated query: group points by polygon and
                               start bin before the kernel (0 never, 1 from
                               2^20 points, 2 always); labels are unchanged  */
+  int32_t query_xcd;       /* accelerated query: K > 0 runs K consecutive
+                              blocks of grouped points on one XCD at a time
+                              (workgroups reach the 8 XCDs round-robin, each
+                              with its own L2); 0 the plain block order;
+                              labels are unchanged                          */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by
@@ -84,7 +89,8 @@ typedef struct {
   int64_t outside;        /* n_queries - inside                                */
   int64_t unlocated;      /* points that reached no leaf                       */
   int32_t orientation_flipped; /* bit r: ring r reversed on load               */
-  int32_t mark_passes;    /* marking walks run                                 */
+  int32_t mark_passes;    /* levels at which the polygon's edges were marked
+                             (a batch: the largest)                           */
   int64_t walk_steps_mark;
   int64_t walk_steps_dispatch;
   int64_t walk_partner_missing;
