@@ -1136,8 +1136,9 @@ static int parity_type(const og_ctx* c, int from, int mb) {
     return (cnt % 2 == 0) ? c->c[from].type : 3 - c->c[from].type;
 }
 
-/* kernel_Classify_setIECell (:1366-1392) repeated to a fixpoint; each round
- * reads a snapshot of the types.  Returns UNDEFINED leaves after round 1. */
+/* kernel_Classify_setIECell (:1366-1392) repeated to a fixpoint; each pass
+ * reads a snapshot of the types.  Returns the UNDEFINED leaves left after
+ * the first pass. */
 static int64_t ie_fixpoint(og_ctx* c, int lv, int32_t* snap) {
     og_level* G = &c->g[lv];
     int64_t after_first = -1;
