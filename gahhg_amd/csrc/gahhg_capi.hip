@@ -244,6 +244,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_sub = 1;
   p->query_sub_budget = 4;
   p->query_sort = 1;
+  p->query_sort_bits = 14;
   p->query_xcd = 0;
   return GAHHG_OK;
 }
@@ -262,7 +263,8 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       c->prm.query_bins > 4096 || c->prm.query_group < 0 || c->prm.query_group > 64 ||
       (c->prm.query_group & (c->prm.query_group - 1)) != 0 || c->prm.query_sub < 0 || c->prm.query_sub > 256 ||
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
-      c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096) {
+      c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096 ||
+      c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14) {
     delete c;
     return GAHHG_E_ARG;
   }

@@ -380,7 +380,8 @@ int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
-int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, const int32_t** order);
+int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                 const int32_t** order, int64_t* m);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);

@@ -845,19 +845,36 @@ int pack_query_layout(gahhg_ctx* c) {
   }
   return rc;
 }
-// Sort key of a point: its polygon, then its start-table bin split 4 x 4
-// (a cube map of 256 bins per side when there is no table); value = index.
-__global__ void k_sort_key(const double* __restrict__ xyz, const int32_t* __restrict__ qpoly, int64_t n, int B,
-                           int bin_bits, uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+// Spreads the low 16 bits of v to the even bit positions.
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Sort key of a point: its polygon, then its cube-map face, then the
+// Z-order (Morton) code of its face coordinates quantised to r bits per
+// axis, so that points close on the sphere get close keys and a wave's
+// lanes walk the same cells and leaves; value = index.
+__global__ void k_sort_key(const double* __restrict__ xyz, const int32_t* __restrict__ qpoly, int64_t n, int r,
+                           uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const V3 p = load3(xyz + 3 * t);
   double fx = 0.0, fy = 0.0;
-  const int bin = bin_of(p, B, fx, fy);
-  const uint32_t si = (uint32_t)min(max((int)(fx * 4.0), 0), 3), sj = (uint32_t)min(max((int)(fy * 4.0), 0), 3);
-  const uint32_t b = bin < 0 ? 0u : (((uint32_t)bin * 4u + si) * 4u + sj);
+  const int R = 1 << r;
+  const int cell = bin_of(p, R, fx, fy);  // (face * R + i) * R + j
+  uint32_t code = 0;
+  if (cell >= 0) {
+    const uint32_t face = (uint32_t)(cell >> (2 * r));
+    const uint32_t i = (uint32_t)((cell >> r) & (R - 1)), j = (uint32_t)(cell & (R - 1));
+    code = (face << (2 * r)) | (spread_bits(i) << 1) | spread_bits(j);
+  }
   const uint32_t poly = qpoly ? (uint32_t)qpoly[t] : 0u;
-  key[t] = (poly << bin_bits) | b;
+  key[t] = (poly << (3 + 2 * r)) | code;
   val[t] = (uint32_t)t;
 }
 
@@ -870,42 +887,158 @@ static int ceil_log2(int64_t v) {
 // Groups the points by (polygon, start bin) with a device radix sort
 // (rocPRIM through hipCUB) over the key bits in use; *order receives the
 // point indices in key order.  Buffers are kept by the context and grow.
-int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, const int32_t** order) {
+// The grouping key of k_sort_key for one point.
+__device__ __forceinline__ uint32_t group_key(V3 p, uint32_t poly, int r) {
+  double fx = 0.0, fy = 0.0;
+  const int R = 1 << r;
+  const int cell = bin_of(p, R, fx, fy);
+  uint32_t code = 0;
+  if (cell >= 0) {
+    const uint32_t face = (uint32_t)(cell >> (2 * r));
+    const uint32_t i = (uint32_t)((cell >> r) & (R - 1)), j = (uint32_t)(cell & (R - 1));
+    code = (face << (2 * r)) | (spread_bits(i) << 1) | spread_bits(j);
+  }
+  return (poly << (3 + 2 * r)) | code;
+}
+
+// First pass of a grouped query: the start table answers the points it
+// resolves (label written here); every other point gets its grouping key,
+// the rest kNoKey (no real key has it: the face field never exceeds 5).
+// counts[block] = the block's points left for the walk.
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+__global__ void __launch_bounds__(256) k_query_table(const PolyDev* __restrict__ pd, const int32_t* __restrict__ table,
+                                                     const int32_t* __restrict__ sub, int B,
+                                                     const double* __restrict__ xyz,
+                                                     const int32_t* __restrict__ qpoly, int64_t n, int r,
+                                                     int32_t* __restrict__ labels, uint32_t* __restrict__ key,
+                                                     int32_t* __restrict__ counts) {
+  __shared__ int part[4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  if (t < n) {
+    const V3 p = load3(xyz + 3 * t);
+    const uint32_t poly = qpoly ? (uint32_t)qpoly[t] : 0u;
+    const PolyDev& P = pd[poly];
+    double fx = 0.0, fy = 0.0;
+    const int bin = bin_of(p, B, fx, fy);
+    int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
+    if (e <= -kSubCode) {
+      const int S = P.tsub;
+      const int si = min(max((int)(fx * S), 0), S - 1), sj = min(max((int)(fy * S), 0), S - 1);
+      e = sub[(int64_t)(-e - kSubCode) + si * S + sj];
+    }
+    if (e <= -2) {
+      labels[t] = -e - 2;
+      key[t] = kNoKey;
+    } else {
+      keep = true;
+      key[t] = group_key(p, poly, r);
+    }
+  }
+  const int c = __popcll(__ballot(keep));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Second pass: the kept points, in point order, to [offs[block], ...).
+__global__ void __launch_bounds__(256) k_query_compact(const uint32_t* __restrict__ key, int64_t n,
+                                                       const int32_t* __restrict__ offs, uint32_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout) {
+  __shared__ int part[4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t k = t < n ? key[t] : kNoKey;
+  const bool keep = k != kNoKey;
+  const unsigned long long m = __ballot(keep);
+  if (lane == 0) part[wid] = __popcll(m);
+  __syncthreads();
+  int before = offs[blockIdx.x];
+  for (int w = 0; w < wid; w++) before += part[w];
+  if (keep) {
+    const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
+    kout[pos] = k;
+    vout[pos] = (uint32_t)t;
+  }
+}
+
+// Groups the points by (polygon, position on the sphere) with a device radix
+// sort (rocPRIM through hipCUB) over the key bits in use; *order receives
+// the point indices in key order and *m their number.  With a start table,
+// the points it resolves are answered first (labels written) and only the
+// rest are compacted (block counts, a scan, a stable scatter) and sorted.
+// Buffers are kept by the context and grow.
+int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                 const int32_t** order, int64_t* m) {
   if (n > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "sort: more than 2^31 points");
-  const int B = c->q.table ? c->q.bins : 256;
-  const int bin_bits = ceil_log2(6LL * B * B * 16);
+  // face coordinates at r bits per axis, r as large as the 32-bit key
+  // leaves room for after the polygon and face bits (at most query_sort_bits)
   const int poly_bits = ceil_log2((int64_t)c->polys.size());
-  const int end_bit = std::min(32, bin_bits + poly_bits);
+  const int r = std::max(1, std::min(c->prm.query_sort_bits, (32 - 3 - poly_bits) / 2));
+  const int end_bit = std::min(32, 3 + 2 * r + poly_bits);
   QuerySort& s = c->qsort;
   size_t temp = 0;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0,
                                                     end_bit, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sort size");
+  const int64_t nb = (n + 255) / 256;  // blocks of the table passes
+  // bytes of the block counts, offsets and total for cap points
+  auto count_bytes = [](int64_t cap) { return ((8 * ((cap + 255) / 256) + 4) + 255) / 256 * 256; };
   if (s.cap < n || s.temp_bytes < (int64_t)temp) {
+    (void)hipStreamSynchronize(c->stream);
     dfree(s.buf);
     s.buf = nullptr;
     s.cap = 0;
-    const int64_t cap = std::max<int64_t>(n, s.cap);
+    const int64_t cap = n;
     const int64_t tb = ((int64_t)temp + 255) / 256 * 256;
-    if (dalloc(c, (void**)&s.buf, 16 * cap + tb)) return GAHHG_E_NOMEM;
+    if (dalloc(c, (void**)&s.buf, count_bytes(cap) + 16 * cap + tb)) return GAHHG_E_NOMEM;
     s.cap = cap;
     s.temp_bytes = tb;
   }
-  uint32_t* k0 = (uint32_t*)s.buf;
+  // buffer: [block counts | block offsets | total] [k0 | k1 | v0 | v1] [sort temp]
+  const int64_t cnt_cap = count_bytes(s.cap);
+  int32_t* bcount = (int32_t*)s.buf;
+  int32_t* boffs = bcount + nb;
+  int32_t* btotal = boffs + nb;
+  uint32_t* k0 = (uint32_t*)((char*)s.buf + cnt_cap);
   uint32_t* k1 = k0 + s.cap;
   uint32_t* v0 = k1 + s.cap;
   uint32_t* v1 = v0 + s.cap;
   void* tmp = (void*)(v1 + s.cap);
-  hipLaunchKernelGGL(k_sort_key, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_xyz, d_poly, n, B,
-                     bin_bits, k0, v0);
-  int rc = hip_check(c, hipGetLastError(), "sort keys");
-  if (rc) return rc;
+  const unsigned blocks = (unsigned)nb;
+  int64_t items = n;
+  if (c->q.table) {
+    const QueryLayout& L = c->q;
+    hipLaunchKernelGGL(k_query_table, dim3(blocks), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
+                       (const int32_t*)L.table, (const int32_t*)L.sub, L.bins, d_xyz, d_poly, n, r, d_labels, k1,
+                       bcount);
+    int rc = hip_check(c, hipGetLastError(), "table pass");
+    if (rc) return rc;
+    rc = scan_i32(c, bcount, boffs, nb, nullptr, btotal, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_query_compact, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t*)k1, n,
+                       (const int32_t*)boffs, k0, v0);
+    rc = hip_check(c, hipGetLastError(), "compaction");
+    if (rc) return rc;
+    int32_t h = 0;
+    rc = hip_check(c, hipMemcpyAsync(&h, btotal, 4, hipMemcpyDeviceToHost, c->stream), "table pass count");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "table pass count");
+    if (rc) return rc;
+    items = (int64_t)h;
+  } else {
+    hipLaunchKernelGGL(k_sort_key, dim3(blocks), dim3(256), 0, c->stream, d_xyz, d_poly, n, r, k0, v0);
+    int rc = hip_check(c, hipGetLastError(), "sort keys");
+    if (rc) return rc;
+  }
+  *m = items;
+  *order = (const int32_t*)v1;
+  if (items == 0) return GAHHG_OK;
   size_t tb = (size_t)s.temp_bytes;
-  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t*)k0, k1, (const uint32_t*)v0, v1, (int)n, 0,
+  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t*)k0, k1, (const uint32_t*)v0, v1, (int)items, 0,
                                          end_bit, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sort");
-  *order = (const int32_t*)v1;
   return GAHHG_OK;
 }
 
@@ -933,12 +1066,13 @@ int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t 
     (void)hipEventRecord(c->ev[3], c->stream);
   } else {
     const int32_t* order = nullptr;
+    int64_t m = n;
     if (c->prm.query_sort == 2 || (c->prm.query_sort == 1 && n >= kSortMin)) {
-      const int rc = sort_queries(c, d_xyz, d_poly, n, &order);
+      const int rc = sort_queries(c, d_xyz, d_poly, n, d_labels, &order, &m);
       if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[2], c->stream);
-    launch_query<false>(c, d_xyz, d_poly, order, n, d_labels, nullptr);
+    if (m > 0) launch_query<false>(c, d_xyz, d_poly, order, m, d_labels, nullptr);
     (void)hipEventRecord(c->ev[3], c->stream);
   }
   return hip_check(c, hipGetLastError(), "query kernel");

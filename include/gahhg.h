@@ -58,9 +58,15 @@ typedef struct {
   int32_t query_sub;       /* largest sub-table side S of the start table
                               (power of two <= 256; 0 or 1 = none, default) */
   int32_t query_sub_budget;/* sub-table entries allowed per polygon edge     */
-  int32_t query_sort;      /* accelerated query: group points by polygon and
-                              start bin before the kernel (0 never, 1 from
-                              2^20 points, 2 always); labels are unchanged  */
+  int32_t query_sort;      /* accelerated query: answer what the start table
+                              resolves, then group the other points by
+                              polygon and position before the walk (0 never,
+                              1 from 2^20 points, 2 always); labels are
+                              unchanged                                     */
+  int32_t query_sort_bits; /* grouping key: Z-order of the cube-map face
+                              coordinates at up to this many bits per axis
+                              (1..14, default 14; fewer when a batch's
+                              polygon ids need the key's bits)              */
   int32_t query_xcd;       /* accelerated query: K > 0 runs K consecutive
                               blocks of grouped points on one XCD at a time
                               (workgroups reach the 8 XCDs round-robin, each
