@@ -21,9 +21,11 @@
 // host loader does.  The values therefore equal the host loader's on every
 // line.
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "gahhg_decimal.h"
@@ -301,9 +303,37 @@ int listed_lines(gahhg_ctx* c, int n_flag, int64_t nl, int64_t bytes, int64_t fi
   return GAHHG_OK;
 }
 
+// Reads [off, off + n) of the file into dst with `threads` parallel preads
+// (copies out of the page cache run at one core's memory bandwidth each).
+bool read_parallel(int fd, uint8_t* dst, int64_t off, int64_t n, int threads) {
+  const int64_t slice = std::max<int64_t>((n + threads - 1) / threads, 1 << 20);
+  std::vector<std::thread> pool;
+  std::vector<char> ok((size_t)threads, 1);
+  for (int t = 0; t < threads; t++) {
+    const int64_t a = (int64_t)t * slice, b = std::min<int64_t>(n, a + slice);
+    if (a >= b) break;
+    pool.emplace_back([=, &ok]() {
+      int64_t done = a;
+      while (done < b) {
+        const ssize_t r = pread(fd, dst + done, (size_t)(b - done), (off_t)(off + done));
+        if (r <= 0) {
+          ok[(size_t)t] = 0;
+          return;
+        }
+        done += r;
+      }
+    });
+  }
+  for (std::thread& th : pool) th.join();
+  for (char v : ok)
+    if (!v) return false;
+  return true;
+}
+
 // Copies `bytes` bytes of an open file to the device text buffer through
-// two pinned staging buffers (the read of one chunk overlaps the copy of
-// the previous).  *last is the file's last byte (0 if empty).
+// two pinned staging buffers: while one chunk is copied to the device the
+// next is read (by several threads) into the other.  *last is the file's
+// last byte (0 if empty).
 int upload_file(gahhg_ctx* c, FILE* f, int64_t bytes, uint8_t* last) {
   TextLoad& L = c->tl;
   for (int k = 0; k < 2; k++) {
@@ -314,6 +344,8 @@ int upload_file(gahhg_ctx* c, FILE* f, int64_t bytes, uint8_t* last) {
       if (rc) return rc;
     }
   }
+  const int fd = fileno(f);
+  const int threads = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
   *last = 0;
   int64_t off = 0;
   int k = 0;
@@ -321,7 +353,7 @@ int upload_file(gahhg_ctx* c, FILE* f, int64_t bytes, uint8_t* last) {
     const int64_t n = std::min<int64_t>(kStageBytes, bytes - off);
     int rc = hip_check(c, hipEventSynchronize(L.stage_ev[k]), "staging wait");
     if (rc) return rc;
-    if ((int64_t)fread(L.stage[k], 1, (size_t)n, f) != n) return set_err(c, GAHHG_E_IO, "short read");
+    if (!read_parallel(fd, (uint8_t*)L.stage[k], off, n, threads)) return set_err(c, GAHHG_E_IO, "short read");
     *last = ((uint8_t*)L.stage[k])[n - 1];
     rc = hip_check(c, hipMemcpyAsync(L.text + off, L.stage[k], (size_t)n, hipMemcpyHostToDevice, c->stream),
                    "text upload");
