@@ -46,6 +46,8 @@ void free_grid(gahhg_ctx* c) {
   dfree(q.pcnt);
   dfree(q.qc);
   dfree(q.hdr);
+  dfree(q.ib);
+  dfree(q.idx);
   dfree(q.sv);
   dfree(q.sn);
   dfree(q.sf);
@@ -245,6 +247,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_sub_budget = 4;
   p->query_sort = 1;
   p->query_sort_bits = 14;
+  p->query_index = 1;
   p->query_xcd = 0;
   return GAHHG_OK;
 }
@@ -264,7 +267,8 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       (c->prm.query_group & (c->prm.query_group - 1)) != 0 || c->prm.query_sub < 0 || c->prm.query_sub > 256 ||
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
       c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096 ||
-      c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14) {
+      c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14 || c->prm.query_index < 0 ||
+      c->prm.query_index > 1) {
     delete c;
     return GAHHG_E_ARG;
   }

@@ -208,6 +208,9 @@ struct QueryLayout {
   double* sn = nullptr;
   uint8_t* sf = nullptr;
   int64_t nrec = 0;
+  int32_t* ib = nullptr;   // per cell: angular index offset in idx, -1 = none
+  int32_t* idx = nullptr;  // angular indexes of the leaves' edge streams
+  int64_t nidx = 0;
   int64_t leaves = 0;  // bottom cells with refs
   int32_t* pcnt = nullptr;  // per polygon: edge-stream records, leaves with refs
   int64_t bytes = 0;

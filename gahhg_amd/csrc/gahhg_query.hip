@@ -546,14 +546,252 @@ __global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* _
   }
 }
 
+// ---- angular index of a leaf's edge stream --------------------------------
+// The crossing test of record k (edge a -> b of the leaf's stream) against
+// the arc from the leaf centre c to a point p fails at its first condition
+// when a and b lie strictly on one side of the arc's plane: dot(nq, a) and
+// dot(nq, b) both below -eps or both above eps, nq = arc_normal(c, p).  nq
+// is perpendicular to c, so it is fixed by one angle psi in the tangent
+// plane of c: nq = cos(psi) u + sin(psi) v, and dot(nq, x) =
+// R_x cos(psi - phi_x) for x = R_x (cos phi_x u + sin phi_x v) + (c.x) c.
+// The signs of dot(nq, a) and dot(nq, b) can only differ, or either come
+// within eps of 0, for psi (mod pi) on the arc from phi_a + pi/2 to
+// phi_b + pi/2, widened by asin(2 eps / R) + 1e-7 rad at each end (this
+// absorbs the rounding of nq, of the dot products and of the angles).  A
+// leaf with kIdxMin or more records gets kIdxB buckets over psi in [0, pi),
+// each listing the records whose widened arc meets it; a point then tests
+// only the records of its psi's bucket.  The records left out fail the
+// first condition for every psi of the bucket, so the crossing count, and
+// the label, are those of the full stream.
+constexpr int kIdxB = 16;
+constexpr int kIdxMin = 16;
+constexpr double kPiD = 3.141592653589793;  // pi rounded to double (angles only)
+
+// Tangent basis (u, v) at c, the same in the build and the query.
+__device__ __forceinline__ void leaf_basis(V3 c, V3& u, V3& v) {
+  const V3 axis = fabs(c.y) < 0.9 ? v3(0.0, 1.0, 0.0) : v3(1.0, 0.0, 0.0);
+  u = unit(cross(c, axis));
+  v = cross(c, u);
+}
+
+// Bucket of psi = angle of (nq.u, nq.v) mod pi, by a binary search over
+// the bucket boundaries j pi / kIdxB (no transcendental in the query; a psi
+// within rounding of a boundary may land on either side, which the 1e-7 rad
+// widening of the records' arcs covers).
+__device__ __forceinline__ int psi_bucket(V3 nq, V3 u, V3 v) {
+  double x = dot(nq, u), y = dot(nq, v);
+  if (y < 0.0 || (y == 0.0 && x < 0.0)) {
+    x = -x;
+    y = -y;
+  }
+  static_assert(kIdxB == 16, "boundary tables are for 16 buckets");
+  // boundary directions (cos, sin) of j pi / 16, j = 0..15
+  constexpr double bc[16] = {0x1.0000000000000p+0, 0x1.f6297cff75cb0p-1, 0x1.d906bcf328d46p-1, 0x1.a9b66290ea1a3p-1, 0x1.6a09e667f3bcdp-1, 0x1.1c73b39ae68c9p-1, 0x1.87de2a6aea964p-2, 0x1.8f8b83c69a60dp-3, 0x1.1a62633145c07p-54, -0x1.8f8b83c69a608p-3, -0x1.87de2a6aea962p-2, -0x1.1c73b39ae68c6p-1, -0x1.6a09e667f3bccp-1, -0x1.a9b66290ea1a4p-1, -0x1.d906bcf328d46p-1, -0x1.f6297cff75cb0p-1};
+  constexpr double bs[16] = {0x0.0p+0, 0x1.8f8b83c69a60ap-3, 0x1.87de2a6aea963p-2, 0x1.1c73b39ae68c8p-1, 0x1.6a09e667f3bccp-1, 0x1.a9b66290ea1a3p-1, 0x1.d906bcf328d46p-1, 0x1.f6297cff75cb0p-1, 0x1.0000000000000p+0, 0x1.f6297cff75cb0p-1, 0x1.d906bcf328d46p-1, 0x1.a9b66290ea1a5p-1, 0x1.6a09e667f3bcdp-1, 0x1.1c73b39ae68c8p-1, 0x1.87de2a6aea965p-2, 0x1.8f8b83c69a617p-3};
+  int b = 0;
+#pragma unroll
+  for (int step = 8; step > 0; step >>= 1) {
+    const int j = b + step;
+    if (bc[j] * y - bs[j] * x >= 0.0) b = j;  // psi at or past boundary j
+  }
+  return b;
+}
+
+// Buckets met by the widened arc of edge a -> b: first bucket and count
+// (count kIdxB = all).
+__device__ __forceinline__ void edge_buckets(V3 a, V3 b, V3 u, V3 v, int& b0, int& nb) {
+  const double ua = dot(u, a), va = dot(v, a), ub = dot(u, b), vb = dot(v, b);
+  const double Ra = sqrt(ua * ua + va * va), Rb = sqrt(ub * ub + vb * vb);
+  const double Rm = fmin(Ra, Rb);
+  if (!(Rm > 1e-6)) {
+    b0 = 0;
+    nb = kIdxB;
+    return;
+  }
+  const double pa = atan2(va, ua), pb = atan2(vb, ub);
+  double d = pb - pa;  // to (-pi, pi]
+  if (d > kPiD) d -= 2.0 * kPiD;
+  if (d <= -kPiD) d += 2.0 * kPiD;
+  const double w = asin(fmin(1.0, 2.0 * kEps / Rm)) + 1e-7;
+  const double len = fabs(d) + 2.0 * w;
+  if (len >= kPiD - 1e-6) {
+    b0 = 0;
+    nb = kIdxB;
+    return;
+  }
+  double start = (d >= 0.0 ? pa : pb) + 0.5 * kPiD - w;
+  start = fmod(start, kPiD);
+  if (start < 0.0) start += kPiD;
+  const double scale = kIdxB / kPiD;
+  const int s = min(max((int)(start * scale), 0), kIdxB - 1);
+  const int e = (int)((start + len) * scale);  // may pass kIdxB: wraps
+  b0 = s;
+  nb = min(e - s + 1, kIdxB);
+}
+
+// Index sizes: one wave per cell; isize[cell] = kIdxB + 1 offsets plus the
+// candidate entries of a leaf with >= kIdxMin records, 0 otherwise.
+// Leaves to index (>= kIdxMin records): flag per cell; every cell starts
+// with no index (ib = -1, isize = 0).
+__global__ void k_index_flag(const int32_t* __restrict__ hdr, int cnum, int32_t* __restrict__ flag,
+                             int32_t* __restrict__ isize, int32_t* __restrict__ ib) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= cnum) return;
+  flag[cell] = hdr[2 * cell + 1] >= kIdxMin ? 1 : 0;
+  isize[cell] = 0;
+  ib[cell] = -1;
+}
+
+__global__ void k_index_list(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int cnum,
+                             int32_t* __restrict__ list) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell < cnum && flag[cell]) list[pos[cell]] = cell;
+}
+
+// One wave per listed leaf.
+__global__ void __launch_bounds__(256) k_index_size(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                                    const double* __restrict__ sv, const uint8_t* __restrict__ sf,
+                                                    const int32_t* __restrict__ list, int nlist,
+                                                    int32_t* __restrict__ isize) {
+  const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= nlist) return;
+  const int cell = list[j];
+  const int r0 = hdr[2 * cell], m = hdr[2 * cell + 1];
+  V3 u, v;
+  leaf_basis(load3(qc + 3 * (int64_t)cell), u, v);
+  int tot = 0;
+  for (int base = 0; base < m - 1; base += 64) {
+    const int k = base + lane;
+    int nb = 0;
+    if (k < m - 1 && (sf[r0 + k] & 4)) {
+      int b0;
+      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o, 64);
+    tot += nb;
+  }
+  if (lane == 0) isize[cell] = kIdxB + 1 + tot;
+}
+
+// Per polygon (one block each, over its bottom cells): pstat[2p] = records
+// of its indexed leaves, pstat[2p+1] = their index entries (to size the
+// query's lane groups).
+__global__ void __launch_bounds__(256) k_index_poly_sums(const PolyDev* __restrict__ pd,
+                                                         const int32_t* __restrict__ hdr,
+                                                         const int32_t* __restrict__ isize,
+                                                         unsigned long long* __restrict__ pstat) {
+  __shared__ unsigned long long part[2][4];
+  const PolyDev& P = pd[blockIdx.x];
+  unsigned long long sm = 0, st = 0;
+  for (int i = threadIdx.x; i < P.bcn; i += blockDim.x) {
+    const int cell = P.bcs + i;
+    const int sz = isize[cell];
+    if (sz > 0) {
+      sm += (unsigned long long)hdr[2 * cell + 1];
+      st += (unsigned long long)(sz - (kIdxB + 1));
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sm += __shfl_xor(sm, o, 64);
+    st += __shfl_xor(st, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = sm;
+    part[1][threadIdx.x >> 6] = st;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pstat[2 * blockIdx.x] = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    pstat[2 * blockIdx.x + 1] = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+  }
+}
+
+// Index contents at ioff[cell] (leaves sized by k_index_size): kIdxB + 1
+// offsets (relative to the entries), then per bucket its records in
+// ascending order; ib[cell] = ioff[cell].
+__global__ void __launch_bounds__(256) k_index_fill(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                                    const double* __restrict__ sv, const uint8_t* __restrict__ sf,
+                                                    const int32_t* __restrict__ list, int nlist,
+                                                    const int32_t* __restrict__ ioff, int32_t* __restrict__ ib,
+                                                    int32_t* __restrict__ idx) {
+  const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= nlist) return;
+  const int cell = list[j];
+  const int r0 = hdr[2 * cell], m = hdr[2 * cell + 1];
+  const int base = ioff[cell];
+  int32_t* off = idx + base;
+  int32_t* ent = off + kIdxB + 1;
+  V3 u, v;
+  leaf_basis(load3(qc + 3 * (int64_t)cell), u, v);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  // pass 1: entries per bucket
+  int cnt[kIdxB];
+#pragma unroll
+  for (int b = 0; b < kIdxB; b++) cnt[b] = 0;
+  for (int c0 = 0; c0 < m - 1; c0 += 64) {
+    const int k = c0 + lane;
+    int b0 = 0, nb = 0;
+    if (k < m - 1 && (sf[r0 + k] & 4))
+      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+#pragma unroll
+    for (int b = 0; b < kIdxB; b++) {
+      const bool in = nb > 0 && ((b - b0 + kIdxB) % kIdxB) < nb;
+      cnt[b] += __popcll(__ballot(in));
+    }
+  }
+  int start[kIdxB];
+  int acc = 0;
+#pragma unroll
+  for (int b = 0; b < kIdxB; b++) {
+    start[b] = acc;
+    acc += cnt[b];
+  }
+  if (lane == 0) {
+    for (int b = 0; b < kIdxB; b++) off[b] = start[b];
+    off[kIdxB] = acc;
+    ib[cell] = base;
+  }
+  // pass 2: entries, in record order within each bucket
+  for (int c0 = 0; c0 < m - 1; c0 += 64) {
+    const int k = c0 + lane;
+    int b0 = 0, nb = 0;
+    if (k < m - 1 && (sf[r0 + k] & 4))
+      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+#pragma unroll
+    for (int b = 0; b < kIdxB; b++) {
+      const bool in = nb > 0 && ((b - b0 + kIdxB) % kIdxB) < nb;
+      const unsigned long long bal = __ballot(in);
+      if (in) ent[start[b] + __popcll(bal & below)] = k;
+      start[b] += __popcll(bal);
+    }
+  }
+}
+
 // Edge tests of a wave's boundary points, by groups of G lanes: each group
 // resolves the points of its own lanes one at a time, its G lanes splitting
-// the leaf's edge stream (coalesced loads), and the crossing count is the
-// popcount of the group's part of a ballot.  Every condition below is
-// uniform within a group, so a group's lanes stay active together.
+// the leaf's edge stream (coalesced loads) or, for an indexed leaf, the
+// records of the point's psi bucket, and the crossing count is the popcount
+// of the group's part of a ballot.  Every condition below is uniform within
+// a group, so a group's lanes stay active together.
+__device__ __forceinline__ bool record_cross(int rk, V3 nq, V3 pp, const double* __restrict__ sv,
+                                             const double* __restrict__ sn, const uint8_t* __restrict__ sf) {
+  const int fl = sf[rk];
+  if (!(fl & 4)) return false;
+  const double ra = dot(nq, load3(sv + 3 * (int64_t)rk));
+  const double rb = dot(nq, load3(sv + 3 * (int64_t)(rk + 1)));
+  if (same_side(ra, rb)) return false;
+  const double r3 = dot(load3(sn + 3 * (int64_t)rk), pp);
+  return !(((fl & 1) && lt0(r3)) || ((fl & 2) && gt0(r3)));
+}
+
 template <int G>
 __device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, int64_t q,
                                            const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                           const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
                                            const double* __restrict__ sv, const double* __restrict__ sn,
                                            const uint8_t* __restrict__ sf, int32_t* __restrict__ labels) {
   const int gbase = lane & ~(G - 1);
@@ -567,23 +805,26 @@ __device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, i
     const V3 c = load3(qc + 3 * (int64_t)lt);
     const V3 nq = arc_normal(c, pp);
     const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
+    const int base = ib ? ib[lt] : -1;
     int cnt = 0;
-    for (int base = 0; base < m - 1; base += G) {
-      const int k = base + (lane - gbase);
-      bool cross = false;
-      if (k < m - 1) {
-        const int rk = r0 + k;
-        const int fl = sf[rk];
-        if (fl & 4) {
-          const double ra = dot(nq, load3(sv + 3 * (int64_t)rk));
-          const double rb = dot(nq, load3(sv + 3 * (int64_t)(rk + 1)));
-          if (!same_side(ra, rb)) {
-            const double r3 = dot(load3(sn + 3 * (int64_t)rk), pp);
-            cross = !(((fl & 1) && lt0(r3)) || ((fl & 2) && gt0(r3)));
-          }
-        }
+    if (base < 0) {
+      for (int b0 = 0; b0 < m - 1; b0 += G) {
+        const int k = b0 + (lane - gbase);
+        const bool cross = k < m - 1 && record_cross(r0 + k, nq, pp, sv, sn, sf);
+        cnt += __popcll(__ballot(cross) & gmask);
       }
-      cnt += __popcll(__ballot(cross) & gmask);
+    } else {
+      V3 u, v;
+      leaf_basis(c, u, v);
+      const int b = psi_bucket(nq, u, v);
+      const int* off = idx + base;
+      const int e0 = off[b], e1 = off[b + 1];
+      const int* ent = off + kIdxB + 1;
+      for (int i0 = e0; i0 < e1; i0 += G) {
+        const int i = i0 + (lane - gbase);
+        const bool cross = i < e1 && record_cross(r0 + ent[i], nq, pp, sv, sn, sf);
+        cnt += __popcll(__ballot(cross) & gmask);
+      }
     }
     if (lane == src) labels[q] = (cnt & 1) ? 3 - type : type;
   }
@@ -604,6 +845,7 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
                                                    const int32_t* __restrict__ table,
                                                    const int32_t* __restrict__ sub, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                                   const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
                                                    const double* __restrict__ sv, const double* __restrict__ sn,
                                                    const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
                                                    const int32_t* __restrict__ qpoly,
@@ -682,13 +924,13 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) gw = min(gw, __shfl_xor(gw, o, 64));
   switch (gw) {
-    case 1: leaf_tests<1>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    case 2: leaf_tests<2>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    case 4: leaf_tests<4>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    case 8: leaf_tests<8>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    case 16: leaf_tests<16>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    case 32: leaf_tests<32>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
-    default: leaf_tests<64>(lane, leaf, p, type, q, qc, hdr, sv, sn, sf, labels); break;
+    case 1: leaf_tests<1>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    case 2: leaf_tests<2>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    case 4: leaf_tests<4>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    case 8: leaf_tests<8>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    case 16: leaf_tests<16>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    case 32: leaf_tests<32>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+    default: leaf_tests<64>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
   }
 }
 
@@ -764,6 +1006,100 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
   return hip_check(c, hipGetLastError(), "sub link");
 }
 
+// Angular indexes of the leaves with kIdxMin or more records (see above).
+static int build_leaf_index(gahhg_ctx* c) {
+  QueryLayout& q = c->q;
+  const int64_t nc = c->tot_c;
+  if (nc <= 0 || c->prm.query_index == 0) return GAHHG_OK;
+  const int np = (int)c->polys.size();
+  int32_t* scratch = nullptr;
+  // sizes, offsets, total (+pad), per-polygon sums, flags, positions, count, list
+  const int64_t words = 2 * nc + 2 + 4 * (int64_t)np + 3 * nc + 2;
+  if (dalloc(c, (void**)&q.ib, nc * 4) || dalloc(c, (void**)&scratch, words * 4)) return GAHHG_E_NOMEM;
+  int32_t* isize = scratch;
+  int32_t* ioff = scratch + nc;
+  int32_t* total = scratch + 2 * nc;
+  unsigned long long* pstat = (unsigned long long*)(scratch + 2 * nc + 2);
+  int32_t* flag = scratch + 2 * nc + 2 + 4 * np;
+  int32_t* pos = flag + nc;
+  int32_t* nlist_d = pos + nc;
+  int32_t* list = nlist_d + 2;
+  const unsigned cb = (unsigned)((nc + 255) / 256);
+  hipLaunchKernelGGL(k_index_flag, dim3(cb), dim3(256), 0, c->stream, (const int32_t*)q.hdr, (int)nc, flag, isize,
+                     q.ib);
+  int rc = hip_check(c, hipGetLastError(), "index flags");
+  if (!rc) rc = scan_i32(c, flag, pos, nc, nullptr, nlist_d, 0);
+  int32_t nlist = 0;
+  if (!rc) {
+    hipLaunchKernelGGL(k_index_list, dim3(cb), dim3(256), 0, c->stream, (const int32_t*)flag, (const int32_t*)pos,
+                       (int)nc, list);
+    rc = hip_check(c, hipMemcpyAsync(&nlist, nlist_d, 4, hipMemcpyDeviceToHost, c->stream), "index list");
+  }
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "index list");
+  const unsigned lb = (unsigned)(((int64_t)std::max(nlist, 1) * 64 + 255) / 256);
+  if (!rc && nlist > 0) {
+    hipLaunchKernelGGL(k_index_size, dim3(lb), dim3(256), 0, c->stream, (const double*)q.qc, (const int32_t*)q.hdr,
+                       (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, isize);
+    rc = hip_check(c, hipGetLastError(), "index size");
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_index_poly_sums, dim3((unsigned)np), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
+                       (const int32_t*)q.hdr, (const int32_t*)isize, pstat);
+    rc = hip_check(c, hipGetLastError(), "index sums");
+  }
+  if (!rc) rc = scan_i32(c, isize, ioff, nc, nullptr, total, 0);
+  int32_t h = 0;
+  std::vector<unsigned long long> ps(2 * (size_t)np);
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(&h, total, 4, hipMemcpyDeviceToHost, c->stream), "index size");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(ps.data(), pstat, 16 * (size_t)np, hipMemcpyDeviceToHost, c->stream),
+                   "index size");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "index size");
+  if (!rc) {
+    // lanes per edge-test group, from the records a point tests on average
+    // (the whole stream of an unindexed leaf, one bucket of an indexed one):
+    // about half of them without an index (measured), an eighth with one,
+    // where more groups resolving points side by side pay more than fewer
+    // chunks per point (fractal_1m, 100M points: G = 4, 8, 16, 32 give
+    // 9.0, 9.5, 10.7, 13.7 ms)
+    std::vector<int32_t> pc((size_t)np * 2);
+    rc = hip_check(c, hipMemcpy(pc.data(), q.pcnt, np * 8, hipMemcpyDeviceToHost), "leaf counts");
+    for (int p = 0; !rc && p < np; p++) {
+      const double leaves = pc[2 * p + 1];
+      const double tested = (double)pc[2 * p] - (double)ps[2 * p] + (double)ps[2 * p + 1] / kIdxB;
+      const double mean = leaves > 0 ? tested / leaves : 1.0;
+      const double share = pc[2 * p] > 0 ? (double)ps[2 * p] / pc[2 * p] : 0.0;  // records in indexed leaves
+      const double per_lane = 2.0 + 6.0 * share;
+      int Gp = 4;
+      while (Gp < 64 && Gp < mean / per_lane) Gp *= 2;
+      c->pd[p].qg = Gp;
+      if (c->prm.verbose && np == 1)
+        printf("Leaf index: %d of %d leaves indexed, %lld records in indexed leaves, %lld entries; %.1f records "
+               "tested per point on average, %d lanes per group.\n",
+               nlist, pc[2 * p + 1], (long long)ps[2 * p], (long long)ps[2 * p + 1], mean, Gp);
+    }
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                       c->stream),
+                     "polygon upload");
+  }
+  if (!rc && dalloc(c, (void**)&q.idx, std::max<int64_t>(h, 1) * 4)) rc = GAHHG_E_NOMEM;
+  if (rc) {
+    dfree(scratch);
+    return rc;
+  }
+  q.nidx = h;
+  q.bytes += nc * 4 + (int64_t)h * 4;
+  if (nlist > 0)
+    hipLaunchKernelGGL(k_index_fill, dim3(lb), dim3(256), 0, c->stream, (const double*)q.qc, (const int32_t*)q.hdr,
+                       (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, (const int32_t*)ioff,
+                       q.ib, q.idx);
+  rc = hip_check(c, hipGetLastError(), "index fill");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "index fill");
+  dfree(scratch);
+  return rc;
+}
+
 int pack_query_layout(gahhg_ctx* c) {
   QueryLayout& q = c->q;
   const int np = (int)c->polys.size();
@@ -823,6 +1159,8 @@ int pack_query_layout(gahhg_ctx* c) {
                      (const double*)c->d_pv, (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
                      (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf, G);
   rc = hip_check(c, hipGetLastError(), "leaf write");
+  if (rc) return rc;
+  rc = build_leaf_index(c);
   if (rc) return rc;
   if (q.bins > 0) {
     const int64_t per = 6LL * q.bins * q.bins;
@@ -1049,7 +1387,8 @@ static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
   const QueryLayout& L = c->q;
   hipLaunchKernelGGL((k_query<TRACE>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                      (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
-                     L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const double*)L.sv,
+                     L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
+                     (const int32_t*)L.idx, (const double*)L.sv,
                      (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_labels, level_tab(c),
                      d_trace, c->prm.query_group, c->prm.query_xcd);
 }

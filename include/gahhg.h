@@ -67,6 +67,11 @@ typedef struct {
                               coordinates at up to this many bits per axis
                               (1..14, default 14; fewer when a batch's
                               polygon ids need the key's bits)              */
+  int32_t query_index;     /* accelerated query: 1 (default) gives leaves
+                              with many edge records an angular index, so a
+                              point tests only the records that can cross
+                              its arc; 0 tests every record; labels are
+                              unchanged                                     */
   int32_t query_xcd;       /* accelerated query: K > 0 runs K consecutive
                               blocks of grouped points on one XCD at a time
                               (workgroups reach the 8 XCDs round-robin, each
