@@ -56,7 +56,8 @@ def measured_traffic(depth, Q, sort_bits):
     if not m:
         return None
     d = json.loads(m.group(1))
-    same = d.get("depth") == depth and d.get("query_sort_bits") == sort_bits and d.get("table_pass")
+    same = (d.get("depth") == depth and d.get("query_sort_bits") == sort_bits and d.get("table_pass")
+            and d.get("leaf_index"))
     return round(float(d["k_query_traffic_bytes"]), 0) if same else None
 
 

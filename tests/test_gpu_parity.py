@@ -334,3 +334,49 @@ def test_refs_beyond_flag_scratch():
     xyz = ol.query_lonlat_to_xyz(lon, lat)
     assert np.count_nonzero(g.query_xyz(xyz) != o.query_xyz(xyz)) == 0
     g.close()
+
+
+def test_leaf_index_agrees():
+    """The angular leaf index (query_index=1, default) skips only records
+    that cannot count: labels equal the full-stream scan (query_index=0) and
+    the oracle on points 1e-6 .. 1e-12 rad either side of the edges, at and
+    around the bottom-cell centres (the arcs' origin), and uniform points."""
+    say("leaf index vs full scan")
+    w = datasets.fractal_100k(n_queries=300_000, n_outer=30_000)
+    g1 = gahhg_amd.GPISP(device=0, query_index=1)
+    g0 = gahhg_amd.GPISP(device=0, query_index=0)
+    for g in (g0, g1):
+        g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    pv, pe = o.polygon()
+    rng = np.random.default_rng(17)
+    pts = [gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])]
+    e = rng.choice(len(pe), 40_000)
+    a, b = pv[pe[e, 0]], pv[pe[e, 1]]
+    t = rng.uniform(0, 1, (len(e), 1))
+    m = a * (1 - t) + b * t
+    n = np.cross(a, b)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    for off in (1e-6, 1e-8, 1e-9, 1e-10, 1e-12):
+        for sgn in (1, -1):
+            q = m + sgn * off * n
+            pts.append(q / np.linalg.norm(q, axis=1)[:, None])
+    gg = g1.grid()
+    cells, lv = gg["cells"], gg["levels"]
+    bot = np.arange(lv[-1][4], lv[-1][4] + lv[-1][5])
+    bot = bot[cells[bot, 3] >= 16][:20_000]
+    cen = np.zeros((len(bot), 3))
+    for k in range(6):
+        has = cells[bot, 1] > k
+        cen[has] += gg["v"][gg["borders"][cells[bot[has], 0] + k, 2]]
+    cen /= np.linalg.norm(cen, axis=1)[:, None]
+    pts.append(cen)
+    for s in (1e-14, 1e-11, 1e-8, 1e-6):
+        q = cen + s * rng.standard_normal(cen.shape)
+        pts.append(q / np.linalg.norm(q, axis=1)[:, None])
+    xyz = np.concatenate(pts)
+    l1, l0 = g1.query_xyz(xyz), g0.query_xyz(xyz)
+    assert np.count_nonzero(l1 != l0) == 0
+    assert np.count_nonzero(l1 != o.query_xyz(xyz)) == 0
+    g0.close()
+    g1.close()
