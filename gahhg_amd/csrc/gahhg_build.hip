@@ -681,6 +681,25 @@ void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+int use_buf(gahhg_ctx* c, DevBuf& b, int64_t bytes, void** active) {
+  bytes = std::max<int64_t>(bytes, 256);
+  if (b.p == nullptr || bytes > b.cap) {
+    (void)hipStreamSynchronize(c->stream);
+    free_buf(b);
+    const int64_t n = bytes + bytes / 8;  // headroom for the next, slightly larger build
+    if (dalloc(c, &b.p, n)) return GAHHG_E_NOMEM;
+    b.cap = n;
+  }
+  *active = b.p;
+  return GAHHG_OK;
+}
+
+void free_buf(DevBuf& b) {
+  dfree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
 int fill_i32(gahhg_ctx* c, int32_t* p, int64_t n, int32_t v) {
   if (n <= 0) return GAHHG_OK;
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, p, n, v);

@@ -40,18 +40,25 @@ void free_grid(gahhg_ctx* c) {
   for (void* p : ptrs) dfree(p);
   g = DeviceGrid();
   QueryLayout& q = c->q;
-  dfree(q.table);
-  dfree(q.sub);
-  dfree(q.tmp);
-  dfree(q.pcnt);
-  dfree(q.qc);
-  dfree(q.hdr);
-  dfree(q.ib);
-  dfree(q.idx);
-  dfree(q.sv);
-  dfree(q.sn);
-  dfree(q.sf);
+  DevBuf* bufs[] = {&q.b_table, &q.b_sub, &q.b_tmp, &q.b_pcnt, &q.b_qc, &q.b_hdr,
+                    &q.b_ib,    &q.b_idx, &q.b_sv,  &q.b_sn,   &q.b_sf, &q.b_iscr};
+  for (DevBuf* b : bufs) free_buf(*b);
   q = QueryLayout();
+  c->built = false;
+  c->gnum = 0;
+}
+
+// Before a rebuild of the same polygons: the grid arrays and the query
+// layout's buffers are kept (a build writes every element it reads), only
+// the layout's parts are marked absent, so a rebuild allocates nothing new
+// unless it needs more room.
+void reset_grid(gahhg_ctx* c) {
+  QueryLayout& q = c->q;
+  QueryLayout kept;
+  kept.b_table = q.b_table, kept.b_sub = q.b_sub, kept.b_tmp = q.b_tmp, kept.b_pcnt = q.b_pcnt;
+  kept.b_qc = q.b_qc, kept.b_hdr = q.b_hdr, kept.b_ib = q.b_ib, kept.b_idx = q.b_idx;
+  kept.b_sv = q.b_sv, kept.b_sn = q.b_sn, kept.b_sf = q.b_sf, kept.b_iscr = q.b_iscr;
+  q = kept;
   c->built = false;
   c->gnum = 0;
 }
@@ -419,7 +426,7 @@ int gahhg_build(gahhg_ctx* c) {
   if (!c->have_poly) return set_err(c, GAHHG_E_STATE, "build: no polygon loaded");
   RCK(select_device(c));
   (void)hipStreamSynchronize(c->stream);
-  free_grid(c);
+  reset_grid(c);
   const double t0 = now_ms();
   RCK(build_hierarchy(c));
   const double t1 = now_ms();

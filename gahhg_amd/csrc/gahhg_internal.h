@@ -195,6 +195,13 @@ struct GridView {
 //    record r to record r+1 exists, sn[r] is its plane normal and bits 0/1
 //    give the side of the leaf centre (ne.c < -eps / ne.c > eps).  hdr holds
 //    (first record, record count) per bottom cell.
+// A device allocation kept across builds: grown (contents dropped) only when
+// a build needs more.
+struct DevBuf {
+  void* p = nullptr;
+  int64_t cap = 0;  // bytes
+};
+
 struct QueryLayout {
   int32_t bins = 0;
   int32_t* table = nullptr;
@@ -214,6 +221,9 @@ struct QueryLayout {
   int64_t leaves = 0;  // bottom cells with refs
   int32_t* pcnt = nullptr;  // per polygon: edge-stream records, leaves with refs
   int64_t bytes = 0;
+  // allocations behind the pointers above (a pointer is null when its part
+  // of the layout is not built)
+  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr;
 };
 
 // Learned refinement depth: model, cost constants, hint sample and the
@@ -407,6 +417,9 @@ int set_err(gahhg_ctx* c, int code, const char* fmt, ...);
 int hip_check(gahhg_ctx* c, int hip_status, const char* what);
 int dalloc(gahhg_ctx* c, void** p, int64_t bytes);
 void dfree(void* p);
+// *active = a buffer of at least `bytes` from b (grown when too small).
+int use_buf(gahhg_ctx* c, DevBuf& b, int64_t bytes, void** active);
+void free_buf(DevBuf& b);
 
 // Fills n int32 with v on the context stream (a kernel, so the call stays
 // graph-capturable).

@@ -944,7 +944,7 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
   const int np = (int)c->polys.size();
   const int B = q.bins;
   const int64_t need = 5 * (nb + 1) + np;
-  if (dalloc(c, (void**)&q.tmp, need * 4)) return GAHHG_E_NOMEM;
+  if (use_buf(c, q.b_tmp, need * 4, (void**)&q.tmp)) return GAHHG_E_NOMEM;
   q.tmpcap = need;
   int32_t* flag = q.tmp;
   int32_t* scan = flag + (nb + 1);
@@ -994,7 +994,7 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
   if (rc) return rc;
   int64_t nbin = 0;
   for (int p = 0; p < np; p++) nbin += c->pd[p].tsub > 1 ? K[p] : 0;
-  if (dalloc(c, (void**)&q.sub, total * 4)) return GAHHG_E_NOMEM;
+  if (use_buf(c, q.b_sub, total * 4, (void**)&q.sub)) return GAHHG_E_NOMEM;
   q.nsub = total;
   q.bytes += total * 4;
   hipLaunchKernelGGL(k_sub_build, dim3((unsigned)nbin), dim3(64), 0, c->stream, g, (const PolyDev*)c->d_pd, B, per,
@@ -1015,7 +1015,8 @@ static int build_leaf_index(gahhg_ctx* c) {
   int32_t* scratch = nullptr;
   // sizes, offsets, total (+pad), per-polygon sums, flags, positions, count, list
   const int64_t words = 2 * nc + 2 + 4 * (int64_t)np + 3 * nc + 2;
-  if (dalloc(c, (void**)&q.ib, nc * 4) || dalloc(c, (void**)&scratch, words * 4)) return GAHHG_E_NOMEM;
+  if (use_buf(c, q.b_ib, nc * 4, (void**)&q.ib) || use_buf(c, q.b_iscr, words * 4, (void**)&scratch))
+    return GAHHG_E_NOMEM;
   int32_t* isize = scratch;
   int32_t* ioff = scratch + nc;
   int32_t* total = scratch + 2 * nc;
@@ -1083,21 +1084,15 @@ static int build_leaf_index(gahhg_ctx* c) {
                                        c->stream),
                      "polygon upload");
   }
-  if (!rc && dalloc(c, (void**)&q.idx, std::max<int64_t>(h, 1) * 4)) rc = GAHHG_E_NOMEM;
-  if (rc) {
-    dfree(scratch);
-    return rc;
-  }
+  if (!rc && use_buf(c, q.b_idx, std::max<int64_t>(h, 1) * 4, (void**)&q.idx)) rc = GAHHG_E_NOMEM;
+  if (rc) return rc;
   q.nidx = h;
   q.bytes += nc * 4 + (int64_t)h * 4;
   if (nlist > 0)
     hipLaunchKernelGGL(k_index_fill, dim3(lb), dim3(256), 0, c->stream, (const double*)q.qc, (const int32_t*)q.hdr,
                        (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, (const int32_t*)ioff,
                        q.ib, q.idx);
-  rc = hip_check(c, hipGetLastError(), "index fill");
-  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "index fill");
-  dfree(scratch);
-  return rc;
+  return hip_check(c, hipGetLastError(), "index fill");
 }
 
 int pack_query_layout(gahhg_ctx* c) {
@@ -1119,7 +1114,7 @@ int pack_query_layout(gahhg_ctx* c) {
   int G = 1;
   while (G < 64 && G < (bottom > 0 ? 0.25 * refs / bottom : 1.0)) G *= 2;
   const unsigned wb = (unsigned)((nc * G + 255) / 256);
-  if (dalloc(c, (void**)&q.pcnt, (int64_t)np * 8)) return GAHHG_E_NOMEM;
+  if (use_buf(c, q.b_pcnt, (int64_t)np * 8, (void**)&q.pcnt)) return GAHHG_E_NOMEM;
   int rc = fill_i32(c, q.pcnt, (int64_t)np * 2, 0);
   if (rc) return rc;
   hipLaunchKernelGGL(k_leaf_count, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
@@ -1150,9 +1145,9 @@ int pack_query_layout(gahhg_ctx* c) {
                    "polygon upload");
     if (rc) return rc;
   }
-  if (dalloc(c, (void**)&q.qc, nc * 24) || dalloc(c, (void**)&q.hdr, nc * 8) ||
-      dalloc(c, (void**)&q.sv, q.nrec * 24) || dalloc(c, (void**)&q.sn, q.nrec * 24) ||
-      dalloc(c, (void**)&q.sf, q.nrec))
+  if (use_buf(c, q.b_qc, nc * 24, (void**)&q.qc) || use_buf(c, q.b_hdr, nc * 8, (void**)&q.hdr) ||
+      use_buf(c, q.b_sv, q.nrec * 24, (void**)&q.sv) || use_buf(c, q.b_sn, q.nrec * 24, (void**)&q.sn) ||
+      use_buf(c, q.b_sf, q.nrec, (void**)&q.sf))
     return GAHHG_E_NOMEM;
   q.bytes = nc * 32 + q.nrec * 49;
   hipLaunchKernelGGL(k_leaf_write, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
@@ -1165,7 +1160,7 @@ int pack_query_layout(gahhg_ctx* c) {
   if (q.bins > 0) {
     const int64_t per = 6LL * q.bins * q.bins;
     const int64_t nb = per * np;
-    if (dalloc(c, (void**)&q.table, nb * 4)) return GAHHG_E_NOMEM;
+    if (use_buf(c, q.b_table, nb * 4, (void**)&q.table)) return GAHHG_E_NOMEM;
     q.bytes += nb * 4;
     for (int p = 0; p < np; p++) {
       c->pd[p].tbl = (int32_t)(per * p);
