@@ -273,19 +273,24 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const double* __res
       if (MODE == kCount) cnt_out[e] = (int32_t)d.steps;
     }
   }
-  unsigned long long s = wave_sum_u64(d.steps);
-  unsigned long long pm = wave_sum_u64(d.partner_missing);
-  unsigned long long ch = wave_sum_u64(d.cap_hits);
-  unsigned long long un = wave_sum_u64(d.unlocated);
-  unsigned long long bi = wave_sum_u64(d.bad_index);
-  if ((threadIdx.x & 63) == 0) {
-    if (MODE == kMark && s) atomicAdd(&st->steps_mark, s);
-    if (MODE == kCount && s) atomicAdd(&st->steps_dispatch, s);
+  // counters: summed per wave, then per block, one atomic per block and
+  // counter (per-wave atomics on one address serialise across the launch)
+  __shared__ unsigned long long part[kBlock / 64][5];
+  unsigned long long v[5] = {wave_sum_u64(d.steps), wave_sum_u64(d.partner_missing), wave_sum_u64(d.cap_hits),
+                             wave_sum_u64(d.unlocated), wave_sum_u64(d.bad_index)};
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 5; k++) part[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+      for (int k = 0; k < 5; k++) v[k] += part[w][k];
+    if (MODE == kMark && v[0]) atomicAdd(&st->steps_mark, v[0]);
+    if (MODE == kCount && v[0]) atomicAdd(&st->steps_dispatch, v[0]);
     if (MODE != kWrite) {  // the write walk repeats the count walk exactly
-      if (pm) atomicAdd(&st->partner_missing, pm);
-      if (ch) atomicAdd(&st->cap_hits, ch);
-      if (un) atomicAdd(&st->unlocated, un);
-      if (bi) atomicAdd(&st->bad_index, bi);
+      if (v[1]) atomicAdd(&st->partner_missing, v[1]);
+      if (v[2]) atomicAdd(&st->cap_hits, v[2]);
+      if (v[3]) atomicAdd(&st->unlocated, v[3]);
+      if (v[4]) atomicAdd(&st->bad_index, v[4]);
     }
   }
 }

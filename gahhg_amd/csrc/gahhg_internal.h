@@ -125,7 +125,6 @@ struct DevState {
   int32_t R;        // bottom-level refs
   int32_t changed;  // classification round changed something
   int32_t nrec;     // records of the query edge stream
-  int32_t nleaf;    // bottom cells with refs
   unsigned long long steps_mark, steps_dispatch, partner_missing, cap_hits, unlocated, bad_index;
   unsigned long long zero_ref_edges, undefined_leaves;
   unsigned long long q_in, q_undef;  // label census of the last query

@@ -462,9 +462,7 @@ __device__ __forceinline__ unsigned long long group_mask(int lane, int G) {
 
 __global__ void __launch_bounds__(256) k_leaf_count(GridView g, const PolyDev* __restrict__ pd,
                                                     const int32_t* __restrict__ pe, int cnum,
-                                                    int32_t* __restrict__ cnt, DevState* st,
-                                                    int32_t* __restrict__ pcnt, int G) {
-  // every lane reaches the wave-aggregated adds at the end
+                                                    int32_t* __restrict__ cnt, int G) {
   const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   const int lane = threadIdx.x & 63, gl = lane & (G - 1);
   const unsigned long long gm = group_mask(lane, G);
@@ -481,12 +479,41 @@ __global__ void __launch_bounds__(256) k_leaf_count(GridView g, const PolyDev* _
     }
     if (gl == 0) cnt[t] = rn > 0 ? rn + chains : 0;
   }
-  const bool leaf = t < cnum && gl == 0 && rn > 0;
-  const int key = leaf ? 2 * g.c_poly[t] : -1;
-  wave_add<int32_t>(pcnt, key, leaf ? rn + chains : 0);
-  wave_add<int32_t>(pcnt, leaf ? key + 1 : -1, 1);
-  const unsigned long long w = __ballot(leaf);
-  if (lane == 0 && w) atomicAdd(&st->nleaf, (int32_t)__popcll(w));
+}
+
+// Per polygon sums over its bottom cells, kSumChunk cells per block (grid:
+// chunks x polygons), one atomic per block and sum into zeroed totals.
+constexpr int kSumChunk = 8192;
+
+// pcnt[2p] += edge-stream records, pcnt[2p+1] += leaves with refs.
+__global__ void __launch_bounds__(256) k_leaf_poly_sums(const PolyDev* __restrict__ pd,
+                                                        const int32_t* __restrict__ cnt,
+                                                        int32_t* __restrict__ pcnt) {
+  __shared__ int part[2][4];
+  const PolyDev& P = pd[blockIdx.y];
+  const int i0 = blockIdx.x * kSumChunk, i1 = min(P.bcn, i0 + kSumChunk);
+  int rec = 0, leaves = 0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const int k = cnt[P.bcs + i];
+    rec += k;
+    leaves += k > 0 ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rec += __shfl_xor(rec, o, 64);
+    leaves += __shfl_xor(leaves, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = rec;
+    part[1][threadIdx.x >> 6] = leaves;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int r = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    const int l = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    if (r) atomicAdd(&pcnt[2 * blockIdx.y], r);
+    if (l) atomicAdd(&pcnt[2 * blockIdx.y + 1], l);
+  }
 }
 
 // Centre and edge stream of every bottom cell with refs (G lanes per cell):
@@ -675,17 +702,18 @@ __global__ void __launch_bounds__(256) k_index_size(const double* __restrict__ q
   if (lane == 0) isize[cell] = kIdxB + 1 + tot;
 }
 
-// Per polygon (one block each, over its bottom cells): pstat[2p] = records
-// of its indexed leaves, pstat[2p+1] = their index entries (to size the
-// query's lane groups).
+// Per polygon over its bottom cells (grid: chunks x polygons, as
+// k_leaf_poly_sums): pstat[2p] += records of its indexed leaves,
+// pstat[2p+1] += their index entries (to size the query's lane groups).
 __global__ void __launch_bounds__(256) k_index_poly_sums(const PolyDev* __restrict__ pd,
                                                          const int32_t* __restrict__ hdr,
                                                          const int32_t* __restrict__ isize,
                                                          unsigned long long* __restrict__ pstat) {
   __shared__ unsigned long long part[2][4];
-  const PolyDev& P = pd[blockIdx.x];
+  const PolyDev& P = pd[blockIdx.y];
+  const int i0 = blockIdx.x * kSumChunk, i1 = min(P.bcn, i0 + kSumChunk);
   unsigned long long sm = 0, st = 0;
-  for (int i = threadIdx.x; i < P.bcn; i += blockDim.x) {
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
     const int cell = P.bcs + i;
     const int sz = isize[cell];
     if (sz > 0) {
@@ -704,8 +732,10 @@ __global__ void __launch_bounds__(256) k_index_poly_sums(const PolyDev* __restri
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    pstat[2 * blockIdx.x] = part[0][0] + part[0][1] + part[0][2] + part[0][3];
-    pstat[2 * blockIdx.x + 1] = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    const unsigned long long a = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    const unsigned long long b = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    if (a) atomicAdd(&pstat[2 * blockIdx.y], a);
+    if (b) atomicAdd(&pstat[2 * blockIdx.y + 1], b);
   }
 }
 
@@ -1006,6 +1036,14 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
   return hip_check(c, hipGetLastError(), "sub link");
 }
 
+// Grid of the per-polygon sum kernels: chunks of kSumChunk bottom cells x
+// polygons.
+static dim3 sum_grid(const gahhg_ctx* c) {
+  int chunks = 1;
+  for (const PolyDev& P : c->pd) chunks = std::max(chunks, (P.bcn + kSumChunk - 1) / kSumChunk);
+  return dim3((unsigned)chunks, (unsigned)c->pd.size());
+}
+
 // Angular indexes of the leaves with kIdxMin or more records (see above).
 static int build_leaf_index(gahhg_ctx* c) {
   QueryLayout& q = c->q;
@@ -1043,8 +1081,9 @@ static int build_leaf_index(gahhg_ctx* c) {
                        (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, isize);
     rc = hip_check(c, hipGetLastError(), "index size");
   }
+  if (!rc) rc = fill_i32(c, (int32_t*)pstat, 4 * (int64_t)np, 0);
   if (!rc) {
-    hipLaunchKernelGGL(k_index_poly_sums, dim3((unsigned)np), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
+    hipLaunchKernelGGL(k_index_poly_sums, sum_grid(c), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
                        (const int32_t*)q.hdr, (const int32_t*)isize, pstat);
     rc = hip_check(c, hipGetLastError(), "index sums");
   }
@@ -1115,10 +1154,12 @@ int pack_query_layout(gahhg_ctx* c) {
   while (G < 64 && G < (bottom > 0 ? 0.25 * refs / bottom : 1.0)) G *= 2;
   const unsigned wb = (unsigned)((nc * G + 255) / 256);
   if (use_buf(c, q.b_pcnt, (int64_t)np * 8, (void**)&q.pcnt)) return GAHHG_E_NOMEM;
-  int rc = fill_i32(c, q.pcnt, (int64_t)np * 2, 0);
-  if (rc) return rc;
   hipLaunchKernelGGL(k_leaf_count, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
-                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, c->d_st, q.pcnt, G);
+                     (const int32_t*)c->d_pe, (int)c->tot_c, c->d_f0, G);
+  int rc = fill_i32(c, q.pcnt, 2 * (int64_t)np, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_leaf_poly_sums, sum_grid(c), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
+                     (const int32_t*)c->d_f0, q.pcnt);
   rc = hip_check(c, hipGetLastError(), "leaf count");
   if (rc) return rc;
   rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
@@ -1127,13 +1168,14 @@ int pack_query_layout(gahhg_ctx* c) {
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   if (rc) return rc;
   q.nrec = std::max<int32_t>(c->h_st->nrec, 1);
-  q.leaves = c->h_st->nleaf;
   // lanes per edge-test group of each polygon's queries: the smallest power
   // of two >= half its mean edge-stream records per leaf, in [4, 64]
   {
     std::vector<int32_t> pc((size_t)np * 2);
     rc = hip_check(c, hipMemcpy(pc.data(), q.pcnt, np * 8, hipMemcpyDeviceToHost), "leaf counts");
     if (rc) return rc;
+    q.leaves = 0;
+    for (int p = 0; p < np; p++) q.leaves += pc[2 * p + 1];
     for (int p = 0; p < np; p++) {
       const double mean = pc[2 * p + 1] > 0 ? (double)pc[2 * p] / pc[2 * p + 1] : 1.0;
       int Gp = 4;
