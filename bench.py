@@ -43,11 +43,11 @@ def per_query_flops(qc, n):
     return (5 * qc["plane_tests"] + 3 * qc["bnum_sum"] + 14 * qc["boundary_hits"] + 64 * qc["ref_tests"]) / n
 
 
-def measured_traffic(depth, Q, sort_bits):
+def measured_traffic(depth, Q, prm):
     """HBM bytes per k_query launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a copy of known
-    size), when the hierarchy, the point count and the query path match;
-    else None."""
+    size), when the hierarchy, the point count and the query path (prm: the
+    context's parameters) match; else None."""
     import re
     path = os.path.join(ROOT, "profiles", "query_fractal_1m_traffic.md")
     if not os.path.exists(path) or Q != 100_000_000:
@@ -56,8 +56,9 @@ def measured_traffic(depth, Q, sort_bits):
     if not m:
         return None
     d = json.loads(m.group(1))
-    same = (d.get("depth") == depth and d.get("query_sort_bits") == sort_bits and d.get("table_pass")
-            and d.get("leaf_index"))
+    same = (d.get("depth") == depth and d.get("query_sort_bits") == prm.query_sort_bits and d.get("table_pass")
+            and d.get("leaf_index") == bool(prm.query_index) and d.get("query_sub") == prm.query_sub
+            and d.get("query_bins") == prm.query_bins)
     return round(float(d["k_query_traffic_bytes"]), 0) if same else None
 
 
@@ -250,7 +251,7 @@ def main():
                 "frac": round(frac_hbm, 4), "traffic": None}
     roof["kernel"] = "k_query"
     roof["kernel_ms"] = round(kms, 4)
-    roof["traffic"] = measured_traffic(int(st["depth"]), Q, gahhg_amd.default_params().query_sort_bits)
+    roof["traffic"] = measured_traffic(int(st["depth"]), Q, gahhg_amd.default_params(query_bins=args.bins))
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,

@@ -250,7 +250,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_layout = 0;
   p->query_bins = 256;
   p->query_group = 0;
-  p->query_sub = 1;
+  p->query_sub = 8;
   p->query_sub_budget = 4;
   p->query_sort = 1;
   p->query_sort_bits = 14;
