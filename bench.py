@@ -28,8 +28,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def xcd_kw(args):
-    return {} if args.query_xcd is None else {"query_xcd": args.query_xcd}
+def query_kw(args):
+    kw = {} if args.query_xcd is None else {"query_xcd": args.query_xcd}
+    if args.table_budget is not None:
+        kw["query_table_budget"] = args.table_budget
+    return kw
 
 
 def log(*a):
@@ -118,6 +121,8 @@ def main():
     ap.add_argument("--bins", type=int, default=256)
     ap.add_argument("--query-xcd", type=int, default=None,
                     help="blocks per XCD run in the query kernel (0 = plain order; default: the library's)")
+    ap.add_argument("--table-budget", type=int, default=None,
+                    help="batches: start-table entries per polygon edge (0 = full tables)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--no-series", action="store_true")
@@ -145,7 +150,7 @@ def main():
     model = load_depth_model(args)
     # the density-rule build of the accelerated path (the learned build is
     # compared against it)
-    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **xcd_kw(args))
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **query_kw(args))
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
     rule_pre, rule_create, rule_classify = median_build(g, args.builds)
     rule_depth = g.stats()["depth"]
@@ -285,7 +290,7 @@ def main():
     if not args.no_batch:
         out["batch_256"] = bench_batch(args, oracle_lib)
     if not args.no_series:
-        out["series_mars"] = bench_series(oracle_lib)
+        out["series_mars"] = bench_series(args, oracle_lib)
     if not args.no_load:
         out["load_path"] = bench_load(args)
     print(json.dumps(out), flush=True)
@@ -357,7 +362,7 @@ def bench_load(args):
     return res
 
 
-def bench_series(oracle_lib):
+def bench_series(args, oracle_lib):
     """The reference's Mars series (624 polygon files x the 10000 points of
     Mars_query.csv): one batch (file reads, build, query, label download) against
     one context per polygon in turn (the reference starts one process per file;
@@ -368,10 +373,10 @@ def bench_series(oracle_lib):
     gold = os.path.join(ROOT, "tests", "golden")
     files = sweep.series_files(os.path.join(gold, "mars"))
     qcsv = os.path.join(gold, "reference", "Mars_query.csv")
-    sweep.run(files[:8], qcsv)  # warm the library and device
+    sweep.run(files[:8], qcsv, **query_kw(args))  # warm the library and device
     walls = []
     for _ in range(3):
-        r = sweep.run(files, qcsv)
+        r = sweep.run(files, qcsv, **query_kw(args))
         walls.append(r["wall_ms"])
     lab = r["labels"]
     lon, lat = gahhg_amd.read_query_csv(qcsv)
@@ -418,7 +423,7 @@ def bench_batch(args, oracle_lib):
     n_edges = int(sum(len(p[0]) - (len(p[2]) - 1) for p in items))
     log(f"batch_256: {len(items)} polygons, {n_edges} edges, {Q} queries generated in {time.time() - t0:.1f}s")
     model = load_depth_model(args)
-    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **xcd_kw(args))
+    g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **query_kw(args))
     g.set_polygons_lonlat(items)
     rule_build = median_build(g, 3)[0]
     rule_depths = [g.polygon_stats(k)["depth"] for k in range(len(items))]

@@ -41,7 +41,8 @@ class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
         "device", "verbose", "query_layout", "query_bins", "query_group", "query_sub",
-    "query_sub_budget", "query_sort", "query_sort_bits", "query_index", "query_xcd")]
+    "query_sub_budget", "query_sort", "query_sort_bits", "query_index", "query_xcd",
+    "query_table_budget")]
 
 
 _STAT_FIELDS = [
@@ -261,13 +262,14 @@ class GPISP:
 
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
                  query_layout=0, query_bins=256, query_group=0, query_sub=8, query_sub_budget=4,
-                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=0):
+                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=0, query_table_budget=8):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
                              verbose=1 if verbose else 0, block_query=block_query,
                              query_layout=query_layout, query_bins=query_bins, query_group=query_group,
                              query_sub=query_sub, query_sub_budget=query_sub_budget, query_sort=query_sort,
-                             query_sort_bits=query_sort_bits, query_index=query_index, query_xcd=query_xcd)
+                             query_sort_bits=query_sort_bits, query_index=query_index, query_xcd=query_xcd,
+                             query_table_budget=query_table_budget)
         h = ctypes.c_void_p()
         rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
         if rc:

@@ -965,6 +965,7 @@ int build_hierarchy(gahhg_ctx* c) {
     P.bn = kGrid0Borders;
     P.lbs = 0;
     P.tbl = -1;
+    P.tbins = 0;
     P.depth = p < (int)c->depth_override.size() ? c->depth_override[p] : 0;
     if (P.depth == 1) P.refining = 0;  // level 0 is the bottom; nothing is marked
     c->plv[p].push_back(LevelDesc{kGrid0Verts * p, kGrid0Verts, kGrid0Borders * p, kGrid0Borders, kGrid0Cells * p,

@@ -4,9 +4,11 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gahhg_internal.h"
@@ -40,8 +42,8 @@ void free_grid(gahhg_ctx* c) {
   for (void* p : ptrs) dfree(p);
   g = DeviceGrid();
   QueryLayout& q = c->q;
-  DevBuf* bufs[] = {&q.b_table, &q.b_sub, &q.b_tmp, &q.b_pcnt, &q.b_qc, &q.b_hdr,
-                    &q.b_ib,    &q.b_idx, &q.b_sv,  &q.b_sn,   &q.b_sf, &q.b_iscr};
+  DevBuf* bufs[] = {&q.b_table, &q.b_sub, &q.b_tmp, &q.b_pcnt, &q.b_qc,   &q.b_hdr, &q.b_ib,
+                    &q.b_idx,   &q.b_sv,  &q.b_sn,  &q.b_sf,   &q.b_iscr, &q.b_tstart};
   for (DevBuf* b : bufs) free_buf(*b);
   q = QueryLayout();
   c->built = false;
@@ -58,6 +60,7 @@ void reset_grid(gahhg_ctx* c) {
   kept.b_table = q.b_table, kept.b_sub = q.b_sub, kept.b_tmp = q.b_tmp, kept.b_pcnt = q.b_pcnt;
   kept.b_qc = q.b_qc, kept.b_hdr = q.b_hdr, kept.b_ib = q.b_ib, kept.b_idx = q.b_idx;
   kept.b_sv = q.b_sv, kept.b_sn = q.b_sn, kept.b_sf = q.b_sf, kept.b_iscr = q.b_iscr;
+  kept.b_tstart = q.b_tstart;
   q = kept;
   c->built = false;
   c->gnum = 0;
@@ -256,6 +259,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_sort_bits = 14;
   p->query_index = 1;
   p->query_xcd = 0;
+  p->query_table_budget = 8;
   return GAHHG_OK;
 }
 
@@ -275,7 +279,7 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
       c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096 ||
       c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14 || c->prm.query_index < 0 ||
-      c->prm.query_index > 1) {
+      c->prm.query_index > 1 || c->prm.query_table_budget < 0) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -383,12 +387,27 @@ int gahhg_load_polygons_txt(gahhg_ctx* c, const char* const* paths, int32_t n) {
   RCK(ctx_ok(c));
   if (n < 1 || !paths) return set_err(c, GAHHG_E_ARG, "batch: need polygon files");
   RCK(select_device(c));
-  std::vector<Polygon> ps((size_t)n);
-  for (int k = 0; k < n; k++) {
+  for (int k = 0; k < n; k++)
     if (!paths[k]) return set_err(c, GAHHG_E_ARG, "batch: null path %d", k);
-    const int rc = load_polygon_txt(paths[k], &ps[k], &c->err);
-    if (rc) return rc;
-  }
+  // files parsed on up to 16 host threads; the first failing file (in list
+  // order) gives the error, as a sequential load would
+  std::vector<Polygon> ps((size_t)n);
+  std::vector<int> rcs((size_t)n, GAHHG_OK);
+  std::vector<std::string> errs((size_t)n);
+  std::atomic<int> next(0);
+  const int threads = std::max(1, std::min({16, n, (int)std::thread::hardware_concurrency()}));
+  auto work = [&]() {
+    for (int k; (k = next.fetch_add(1)) < n;) rcs[(size_t)k] = load_polygon_txt(paths[k], &ps[(size_t)k], &errs[(size_t)k]);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (std::thread& th : pool) th.join();
+  for (int k = 0; k < n; k++)
+    if (rcs[(size_t)k]) {
+      c->err = errs[(size_t)k];
+      return rcs[(size_t)k];
+    }
   c->polys = std::move(ps);
   return upload_polygons(c);
 }
@@ -476,7 +495,7 @@ int gahhg_build(gahhg_ctx* c) {
 namespace {
 
 constexpr char kGridMagic[8] = {'G', 'A', 'H', 'H', 'G', 'G', 'R', '1'};
-constexpr int32_t kGridVersion = 1;
+constexpr int32_t kGridVersion = 2;
 
 struct Arr {
   void* p;
@@ -635,7 +654,7 @@ int gahhg_load_grid(gahhg_ctx* c, const char* path) {
   c->tot_r = cnt[3];
   c->und_snap = und[0];
   c->und_final = und[1];
-  for (PolyDev& P : c->pd) P.tbl = -1;
+  for (PolyDev& P : c->pd) P.tbl = -1, P.tbins = 0;
   memset(c->h_st, 0, sizeof(DevState));
   RCK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state"));
   RCK(hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,

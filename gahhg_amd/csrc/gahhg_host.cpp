@@ -89,7 +89,7 @@ static int assemble(const double* lo, const double* la, const double* xyz, const
     if (xyz) {
       memcpy(pv, xyz + 3 * r.start, 24 * (size_t)r.n);
     } else {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (r.n > 65536)
       for (int j = 0; j < r.n; j++) store3(pv + 3 * j, ll_to_xyz(lo[r.start + j], la[r.start + j]));
     }
     k += r.n;

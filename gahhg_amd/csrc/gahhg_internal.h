@@ -49,6 +49,7 @@ struct PolyDev {
   int32_t U, nv, nbc, nbb;  // refined cells, their corners, border children, their borders
   int32_t nvb, nbb0, ncb;   // next level: first vertex, border, cell
   int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
+  int32_t tbins;        // bins per cube-face side of that table (0 = none)
   int32_t depth;        // fixed depth for this polygon (0 = density rule)
   int32_t tsub;         // sub-table side S of this polygon's start table (1 = none)
   int32_t qg;           // lanes per edge-test group of this polygon's queries
@@ -222,7 +223,8 @@ struct QueryLayout {
   int64_t bytes = 0;
   // allocations behind the pointers above (a pointer is null when its part
   // of the layout is not built)
-  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr;
+  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart;
+  int32_t* tstart = nullptr;  // per polygon: first table entry (np + 1 values, ascending)
 };
 
 // Learned refinement depth: model, cost constants, hint sample and the

@@ -355,13 +355,21 @@ __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, dou
 }
 
 // Level-1 entries: one per bin of every polygon's cube map.
-__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly, int B,
-                              int32_t* __restrict__ table_all) {
-  const int64_t per = 6LL * B * B;
+// Entry tt of all polygons' tables: the polygon owning it is found by a
+// binary search over the tables' first entries (tstart, np + 1 values).
+__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly,
+                              const int32_t* __restrict__ tstart, int32_t* __restrict__ table_all) {
   const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tt >= per * npoly) return;
-  const int poly = (int)(tt / per);
-  const int64_t t = tt - per * poly;
+  if (tt >= tstart[npoly]) return;
+  int lo = 0, hi = npoly - 1;  // last polygon with tstart <= tt
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tstart[mid] <= tt) lo = mid;
+    else hi = mid - 1;
+  }
+  const int poly = lo;
+  const int B = pd[poly].tbins;
+  const int64_t t = tt - tstart[poly];
   const int f = (int)(t / ((int64_t)B * B));
   const int i = (int)((t / B) % B);
   const int j = (int)(t % B);
@@ -897,9 +905,9 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
     const PolyDev& P = pd[qpoly ? qpoly[q] : 0];
     int cur = -1;
     bool done = false;
-    if (B > 0) {
+    if (B > 0 && P.tbins > 0) {
       double fx = 0.0, fy = 0.0;
-      const int bin = bin_of(p, B, fx, fy);
+      const int bin = bin_of(p, P.tbins, fx, fy);
       int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
       if (e <= -kSubCode) {
         const int S = P.tsub;
@@ -1201,21 +1209,43 @@ int pack_query_layout(gahhg_ctx* c) {
   rc = build_leaf_index(c);
   if (rc) return rc;
   if (q.bins > 0) {
-    const int64_t per = 6LL * q.bins * q.bins;
-    const int64_t nb = per * np;
-    if (use_buf(c, q.b_table, nb * 4, (void**)&q.table)) return GAHHG_E_NOMEM;
-    q.bytes += nb * 4;
+    // bins per face side: the requested ones for one polygon; in a batch
+    // each polygon gets at most query_table_budget entries per edge (at
+    // least 16 and at most q.bins bins per side), so that many small
+    // polygons do not each pay for a whole-sphere table (624 polygons of
+    // ~500 edges: 30 ms of table build at 128 bins)
+    const int64_t tb = c->prm.query_table_budget;
+    std::vector<int32_t> ts((size_t)np + 1, 0);
     for (int p = 0; p < np; p++) {
-      c->pd[p].tbl = (int32_t)(per * p);
+      int Bp = q.bins;
+      if (np > 1 && tb > 0) {
+        Bp = 16;
+        while (2 * Bp <= q.bins && 6LL * (2 * Bp) * (2 * Bp) <= tb * c->pd[p].N) Bp *= 2;
+        Bp = std::min(Bp, q.bins);
+      }
+      const int64_t next = (int64_t)ts[p] + 6LL * Bp * Bp;
+      if (next > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
+      c->pd[p].tbl = ts[p];
+      c->pd[p].tbins = Bp;
       c->pd[p].tsub = 1;
+      ts[p + 1] = (int32_t)next;
     }
-    if (nb > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
-    rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
-                                     c->stream),
-                   "polygon upload");
+    const int64_t nb = ts[np];
+    if (use_buf(c, q.b_table, nb * 4, (void**)&q.table) ||
+        use_buf(c, q.b_tstart, ((int64_t)np + 1) * 4, (void**)&q.tstart))
+      return GAHHG_E_NOMEM;
+    q.bytes += nb * 4;
+    rc = hip_check(c, hipMemcpyAsync(q.tstart, ts.data(), ((size_t)np + 1) * 4, hipMemcpyHostToDevice, c->stream),
+                   "table starts");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                       c->stream),
+                     "polygon upload");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "table starts");  // ts leaves scope
     if (rc) return rc;
+    const int64_t per = 6LL * q.bins * q.bins;
     hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
-                       (const PolyDev*)c->d_pd, np, q.bins, q.table);
+                       (const PolyDev*)c->d_pd, np, (const int32_t*)q.tstart, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
     // sub-tables for one polygon only: on the 256-polygon batch they cost
     // 11.5 ms of build for 0.9 ms less per 64M points
@@ -1299,7 +1329,7 @@ __global__ void __launch_bounds__(256) k_query_table(const PolyDev* __restrict__
     const uint32_t poly = qpoly ? (uint32_t)qpoly[t] : 0u;
     const PolyDev& P = pd[poly];
     double fx = 0.0, fy = 0.0;
-    const int bin = bin_of(p, B, fx, fy);
+    const int bin = P.tbins > 0 ? bin_of(p, P.tbins, fx, fy) : -1;
     int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
     if (e <= -kSubCode) {
       const int S = P.tsub;

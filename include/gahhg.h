@@ -80,6 +80,13 @@ typedef struct {
                               (workgroups reach the 8 XCDs round-robin, each
                               with its own L2); 0 the plain block order;
                               labels are unchanged                          */
+  int32_t query_table_budget;/* batches: start-table entries allowed per
+                              polygon edge (bins per face side a power of
+                              two in [16, query_bins]; default 8: least
+                              build + one query pass on 256 polygons with
+                              64M points and on 624 polygons with 10k);
+                              0 = every polygon gets query_bins; single
+                              polygons always do                           */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by

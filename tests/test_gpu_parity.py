@@ -314,6 +314,13 @@ def test_batch_matches_single_polygon_builds():
     gs.set_polygons_lonlat(items).build()
     assert (gs.query_batch_xyz(xyz, poly) == got).all()
     gs.close()
+    # start tables of one size for all polygons (budget 0) and of sizes
+    # between the default's and the full one (budget 40)
+    for budget in (0, 40):
+        gt = gahhg_amd.GPISP(device=0, query_table_budget=budget)
+        gt.set_polygons_lonlat(items).build()
+        assert (gt.query_batch_xyz(xyz, poly) == got).all(), budget
+        gt.close()
     g.close()
     gp.close()
 
