@@ -32,6 +32,8 @@ def query_kw(args):
     kw = {} if args.query_xcd is None else {"query_xcd": args.query_xcd}
     if args.table_budget is not None:
         kw["query_table_budget"] = args.table_budget
+    if args.query_graph:
+        kw["query_graph"] = 1
     return kw
 
 
@@ -123,6 +125,8 @@ def main():
                     help="blocks per XCD run in the query kernel (0 = plain order; default: the library's)")
     ap.add_argument("--table-budget", type=int, default=None,
                     help="batches: start-table entries per polygon edge (0 = full tables)")
+    ap.add_argument("--query-graph", action="store_true",
+                    help="replay each query pass as a recorded HIP graph")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--no-series", action="store_true")

@@ -42,7 +42,7 @@ class Params(ctypes.Structure):
         "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
         "device", "verbose", "query_layout", "query_bins", "query_group", "query_sub",
     "query_sub_budget", "query_sort", "query_sort_bits", "query_index", "query_xcd",
-    "query_table_budget")]
+    "query_table_budget", "query_graph")]
 
 
 _STAT_FIELDS = [
@@ -80,7 +80,7 @@ EXPORTED = [
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
-    "gahhg_load_polygon_txt_device",
+    "gahhg_load_polygon_txt_device", "gahhg_sort_pairs",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -133,6 +133,7 @@ def lib():
         "gahhg_grid_counts": ([P, lp], ctypes.c_int),
         "gahhg_export_grid": ([P, P, P, P, P, P, P], ctypes.c_int),
         "gahhg_kat_device": ([P, i32, dp, i64, dp], ctypes.c_int),
+        "gahhg_sort_pairs": ([P, P, P, i64, i64, i32], ctypes.c_int),
         "gahhg_set_polygons_lonlat": ([P, dp, dp, lp, ip, i32], ctypes.c_int),
         "gahhg_num_polygons": ([P, ctypes.POINTER(i32)], ctypes.c_int),
         "gahhg_query_batch_xyz": ([P, dp, ip, i64, ip], ctypes.c_int),
@@ -262,14 +263,15 @@ class GPISP:
 
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
                  query_layout=0, query_bins=256, query_group=0, query_sub=8, query_sub_budget=4,
-                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=0, query_table_budget=8):
+                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=0, query_table_budget=8,
+                 query_graph=0):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
                              verbose=1 if verbose else 0, block_query=block_query,
                              query_layout=query_layout, query_bins=query_bins, query_group=query_group,
                              query_sub=query_sub, query_sub_budget=query_sub_budget, query_sort=query_sort,
                              query_sort_bits=query_sort_bits, query_index=query_index, query_xcd=query_xcd,
-                             query_table_budget=query_table_budget)
+                             query_table_budget=query_table_budget, query_graph=query_graph)
         h = ctypes.c_void_p()
         rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
         if rc:
@@ -543,6 +545,15 @@ class GPISP:
                                              normals.ctypes.data, refs.ctypes.data, levels.ctypes.data))
         return dict(depth=d, v=v.reshape(-1, 3), cells=cells.reshape(-1, 7), borders=borders.reshape(-1, 6),
                     normals=normals.reshape(-1, 3), refs=refs[:R], levels=levels.reshape(-1, 7))
+
+    def sort_pairs(self, keys, vals, end_bit=32, n_bound=None):
+        """The grouping sort on the device: (keys, vals) sorted stably by
+        key bits [0, end_bit), with launches sized for n_bound points."""
+        k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+        v = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+        nb = k.size if n_bound is None else int(n_bound)
+        self._check(self.L.gahhg_sort_pairs(self.h, k.ctypes.data, v.ctypes.data, k.size, nb, end_bit))
+        return k, v
 
     def kat(self, op, inputs):
         inputs = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 12)

@@ -46,6 +46,8 @@ void free_grid(gahhg_ctx* c) {
                     &q.b_idx,   &q.b_sv,  &q.b_sn,  &q.b_sf,   &q.b_iscr, &q.b_tstart};
   for (DevBuf* b : bufs) free_buf(*b);
   q = QueryLayout();
+  drop_query_graph(c);
+  c->qepoch++;
   c->built = false;
   c->gnum = 0;
 }
@@ -62,11 +64,13 @@ void reset_grid(gahhg_ctx* c) {
   kept.b_sv = q.b_sv, kept.b_sn = q.b_sn, kept.b_sf = q.b_sf, kept.b_iscr = q.b_iscr;
   kept.b_tstart = q.b_tstart;
   q = kept;
+  c->qepoch++;
   c->built = false;
   c->gnum = 0;
 }
 
 void free_scratch(gahhg_ctx* c) {
+  c->qepoch++;
   void* ptrs[] = {c->d_f0, c->d_f1, c->d_o0, c->d_o1, c->d_f2, c->d_f3, c->d_o2, c->d_o3, c->d_scan,
                   c->d_ecnt, c->d_rcell, c->d_rflag, c->d_redge, c->d_rscan, c->d_loc_cur, c->d_loc_prev};
   for (void* p : ptrs) dfree(p);
@@ -260,6 +264,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_index = 1;
   p->query_xcd = 0;
   p->query_table_budget = 8;
+  p->query_graph = 0;
   return GAHHG_OK;
 }
 
@@ -279,7 +284,8 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       (c->prm.query_sub & (c->prm.query_sub - 1)) != 0 || c->prm.query_sub_budget < 0 || c->prm.query_sort < 0 ||
       c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096 ||
       c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14 || c->prm.query_index < 0 ||
-      c->prm.query_index > 1 || c->prm.query_table_budget < 0) {
+      c->prm.query_index > 1 || c->prm.query_table_budget < 0 || c->prm.query_graph < 0 ||
+      c->prm.query_graph > 1) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -296,7 +302,9 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
   }
   for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
   if (hipMalloc((void**)&c->d_st, sizeof(DevState)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cnt, 64, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_cnt, hipEventDisableTiming) != hipSuccess) {
     delete c;
     return GAHHG_E_HIP;
   }
@@ -320,6 +328,8 @@ void gahhg_destroy(gahhg_ctx* c) {
   dfree(c->d_st);
   for (void* p : c->user_allocs) dfree(p);
   if (c->h_st) (void)hipHostFree(c->h_st);
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->ev_cnt) (void)hipEventDestroy(c->ev_cnt);
   for (int i = 0; i < 4; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -723,12 +733,14 @@ int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
     for (int G : groups) {
       c->prm.block_query = b;
       c->prm.query_group = G;
+      c->qepoch++;  // a recorded query pass holds the old launch shape
       float ms = 0.f;
       for (int rep = 0; rep < 3; rep++) {  // first run warms up; keep the faster of two
         RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
         const int rc = run_query(c, d_xyz, d_poly, n, d_labels);
         if (rc) {
           c->prm = keep;
+          c->qepoch++;
           return rc;
         }
         RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
@@ -745,6 +757,7 @@ int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
     }
   c->prm.query_group = bg;
   c->prm.block_query = bb;
+  c->qepoch++;
   if (out3) {
     out3[0] = bg;
     out3[1] = bb;
@@ -1101,6 +1114,33 @@ int gahhg_get_polygon_stats(gahhg_ctx* c, int32_t p, gahhg_stats* out) {
   if (p < 0 || p >= (int32_t)c->polys.size()) return set_err(c, GAHHG_E_ARG, "bad polygon index");
   poly_stats(c, p, out);
   return GAHHG_OK;
+}
+
+int gahhg_sort_pairs(gahhg_ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int64_t n_bound, int32_t end_bit) {
+  RCK(ctx_ok(c));
+  if (n < 0 || n_bound < n || n_bound > INT32_MAX || end_bit < 1 || end_bit > 32 || (n > 0 && (!keys || !vals)))
+    return set_err(c, GAHHG_E_ARG, "sort_pairs: bad arguments");
+  if (n_bound == 0) return GAHHG_OK;
+  RCK(select_device(c));
+  void* buf = nullptr;
+  const int64_t hist = radix_hist_entries(n_bound);
+  RCK(dalloc(c, &buf, 16 * n_bound + 4 * hist + 256));
+  uint32_t* k0 = (uint32_t*)buf;
+  uint32_t* k1 = k0 + n_bound;
+  uint32_t* v0 = k1 + n_bound;
+  uint32_t* v1 = v0 + n_bound;
+  int32_t* h = (int32_t*)(v1 + n_bound);
+  int32_t* d_m = h + hist;
+  uint32_t *kr = nullptr, *vr = nullptr;
+  int rc = hip_check(c, hipMemcpy(k0, keys, 4 * n, hipMemcpyHostToDevice), "sort H2D");
+  if (!rc) rc = hip_check(c, hipMemcpy(v0, vals, 4 * n, hipMemcpyHostToDevice), "sort H2D");
+  if (!rc) rc = fill_i32(c, d_m, 1, (int32_t)n);
+  if (!rc) rc = radix_sort_pairs(c, k0, v0, k1, v1, d_m, n_bound, end_bit, h, &kr, &vr);
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sort sync");
+  if (!rc) rc = hip_check(c, hipMemcpy(keys, kr, 4 * n, hipMemcpyDeviceToHost), "sort D2H");
+  if (!rc) rc = hip_check(c, hipMemcpy(vals, vr, 4 * n, hipMemcpyDeviceToHost), "sort D2H");
+  dfree(buf);
+  return rc;
 }
 
 int gahhg_kat_device(gahhg_ctx* c, int32_t op, const double* in12, int64_t n, double* out3) {

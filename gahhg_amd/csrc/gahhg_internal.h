@@ -15,6 +15,8 @@
 #else
 typedef struct ihipStream_t* hipStream_t;
 typedef struct ihipEvent_t* hipEvent_t;
+typedef struct ihipGraph* hipGraph_t;
+typedef struct hipGraphExec* hipGraphExec_t;
 #endif
 
 namespace gahhg {
@@ -268,7 +270,19 @@ constexpr double kLearnMargin = 2.0;
 struct QuerySort {
   void* buf = nullptr;
   int64_t cap = 0;
-  int64_t temp_bytes = 0;
+};
+
+// A grouped query pass recorded as a HIP graph (query_graph = 1), replayed
+// while the points, polygon ids, labels, count and every buffer the pass
+// touches stay where they were at capture (epoch unchanged).
+struct QueryGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* xyz = nullptr;
+  const void* poly = nullptr;
+  void* labels = nullptr;
+  int64_t n = 0;
+  uint64_t epoch = 0;
 };
 
 // Queries below this count are not sorted (query_sort = 1).
@@ -368,6 +382,10 @@ struct gahhg_ctx {
 
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
+  gahhg::QueryGraph qgraph;
+  uint64_t qepoch = 1;              // bumped when a buffer a query pass reads or writes moves
+  int32_t* h_cnt = nullptr;         // pinned: points the start table leaves open
+  hipEvent_t ev_cnt = nullptr;      // h_cnt written
   gahhg::LearnState learn;
   gahhg::TextLoad tl;
   float* d_mlp_part = nullptr;       // per-wave partial gradients (grow-only)
@@ -388,14 +406,29 @@ namespace gahhg {
 // sum instead.
 int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, const int32_t* d_n,
              int32_t* d_total, int inclusive);
+// Grows the scan scratch for n_bound elements now (a later scan of at most
+// that many then neither allocates nor synchronises).
+int scan_reserve(gahhg_ctx* c, int64_t n_bound);
 
 // ---- build / classify / query (gahhg_build.hip etc.) ----
 int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
+// Groups the points for the walk; *order = point indices in group order.
+// Their number goes to *m, or, with host_count false, stays on the device
+// at *d_count (*m = -1) so that nothing waits for it.
 int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
-                 const int32_t** order, int64_t* m);
+                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count);
+void drop_query_graph(gahhg_ctx* c);
+
+// ---- radix sort (gahhg_sort.hip) ----
+// Stable sort of *d_m (<= n_max) pairs (k0, v0) by key bits [0, end_bit);
+// (k1, v1) is the second buffer; *kres / *vres = where the result is
+// (one of the two).  hist holds radix_hist_entries(n_max) int32.
+int64_t radix_hist_entries(int64_t n_max);
+int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const int32_t* d_m,
+                     int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);

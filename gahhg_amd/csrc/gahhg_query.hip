@@ -12,7 +12,6 @@
 
 #include <algorithm>
 
-#include <hipcub/hipcub.hpp>
 
 #include "gahhg_internal.h"
 
@@ -888,12 +887,19 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
                                                    const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
                                                    const int32_t* __restrict__ qpoly,
                                                    const int32_t* __restrict__ order, int64_t n,
+                                                   const int32_t* __restrict__ d_n,
                                                    int32_t* __restrict__ labels, LevelTab lt,
                                                    int32_t* __restrict__ trace, int G_override, int xcd) {
   // thread t handles point order[t] (points grouped by polygon and start
   // bin, so that a wave's gathers share cache lines), or point t; with xcd,
-  // each XCD takes a contiguous run of t
-  const int64_t t = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  // each XCD takes a contiguous run of t.  With d_n the count is *d_n (at
+  // most n, the launch bound): blocks past it leave at once.
+  const int64_t blk = xcd_block(xcd);
+  if (d_n) {
+    n = *d_n;
+    if (blk * blockDim.x >= n) return;
+  }
+  const int64_t t = blk * blockDim.x + threadIdx.x;
   const int64_t q = (order && t < n) ? (int64_t)order[t] : t;
   const int lane = threadIdx.x & 63;
   V3 p = v3(0.0, 0.0, 0.0);
@@ -1144,6 +1150,7 @@ static int build_leaf_index(gahhg_ctx* c) {
 
 int pack_query_layout(gahhg_ctx* c) {
   QueryLayout& q = c->q;
+  c->qepoch++;
   const int np = (int)c->polys.size();
   // start tables: the requested resolution for one polygon; batches keep
   // each polygon's table to at most 128 bins per face side (393 KB) and
@@ -1371,14 +1378,33 @@ __global__ void __launch_bounds__(256) k_query_compact(const uint32_t* __restric
   }
 }
 
-// Groups the points by (polygon, position on the sphere) with a device radix
-// sort (rocPRIM through hipCUB) over the key bits in use; *order receives
-// the point indices in key order and *m their number.  With a start table,
-// the points it resolves are answered first (labels written) and only the
-// rest are compacted (block counts, a scan, a stable scatter) and sorted.
-// Buffers are kept by the context and grow.
+// Bytes of the block counts, offsets and total of a grouping of cap points.
+static int64_t count_bytes(int64_t cap) { return ((8 * ((cap + 255) / 256) + 4) + 255) / 256 * 256; }
+
+// Grows the grouping buffer to n points (contents not kept).
+static int sort_buffers(gahhg_ctx* c, int64_t n) {
+  QuerySort& s = c->qsort;
+  if (s.cap >= n) return GAHHG_OK;
+  (void)hipStreamSynchronize(c->stream);
+  dfree(s.buf);
+  s.buf = nullptr;
+  s.cap = 0;
+  c->qepoch++;
+  if (dalloc(c, (void**)&s.buf, count_bytes(n) + 16 * n + 4 * radix_hist_entries(n))) return GAHHG_E_NOMEM;
+  s.cap = n;
+  return GAHHG_OK;
+}
+
+// Groups the points by (polygon, position on the sphere) with the device
+// radix sort of gahhg_sort.hip over the key bits in use.  With a start
+// table, the points it resolves are answered first (labels written) and
+// only the rest are compacted (block counts, a scan, a stable scatter) and
+// sorted; their number stays on the device (the sort reads it there).  With
+// host_count it is also copied to pinned memory as soon as the compaction
+// has written it, and read after the sort is queued, so the GPU does not
+// idle while the host waits.  Buffers are kept by the context and grow.
 int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
-                 const int32_t** order, int64_t* m) {
+                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count) {
   if (n > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "sort: more than 2^31 points");
   // face coordinates at r bits per axis, r as large as the 32-bit key
   // leaves room for after the polygon and face bits (at most query_sort_bits)
@@ -1386,26 +1412,10 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
   const int r = std::max(1, std::min(c->prm.query_sort_bits, (32 - 3 - poly_bits) / 2));
   const int end_bit = std::min(32, 3 + 2 * r + poly_bits);
   QuerySort& s = c->qsort;
-  size_t temp = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                    (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0,
-                                                    end_bit, c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "sort size");
   const int64_t nb = (n + 255) / 256;  // blocks of the table passes
-  // bytes of the block counts, offsets and total for cap points
-  auto count_bytes = [](int64_t cap) { return ((8 * ((cap + 255) / 256) + 4) + 255) / 256 * 256; };
-  if (s.cap < n || s.temp_bytes < (int64_t)temp) {
-    (void)hipStreamSynchronize(c->stream);
-    dfree(s.buf);
-    s.buf = nullptr;
-    s.cap = 0;
-    const int64_t cap = n;
-    const int64_t tb = ((int64_t)temp + 255) / 256 * 256;
-    if (dalloc(c, (void**)&s.buf, count_bytes(cap) + 16 * cap + tb)) return GAHHG_E_NOMEM;
-    s.cap = cap;
-    s.temp_bytes = tb;
-  }
-  // buffer: [block counts | block offsets | total] [k0 | k1 | v0 | v1] [sort temp]
+  int rc = sort_buffers(c, n);
+  if (rc) return rc;
+  // buffer: [block counts | block offsets | total] [k0 | k1 | v0 | v1] [radix histograms]
   const int64_t cnt_cap = count_bytes(s.cap);
   int32_t* bcount = (int32_t*)s.buf;
   int32_t* boffs = bcount + nb;
@@ -1414,15 +1424,14 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
   uint32_t* k1 = k0 + s.cap;
   uint32_t* v0 = k1 + s.cap;
   uint32_t* v1 = v0 + s.cap;
-  void* tmp = (void*)(v1 + s.cap);
+  int32_t* hist = (int32_t*)(v1 + s.cap);
   const unsigned blocks = (unsigned)nb;
-  int64_t items = n;
   if (c->q.table) {
     const QueryLayout& L = c->q;
     hipLaunchKernelGGL(k_query_table, dim3(blocks), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
                        (const int32_t*)L.table, (const int32_t*)L.sub, L.bins, d_xyz, d_poly, n, r, d_labels, k1,
                        bcount);
-    int rc = hip_check(c, hipGetLastError(), "table pass");
+    rc = hip_check(c, hipGetLastError(), "table pass");
     if (rc) return rc;
     rc = scan_i32(c, bcount, boffs, nb, nullptr, btotal, 0);
     if (rc) return rc;
@@ -1430,37 +1439,110 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
                        (const int32_t*)boffs, k0, v0);
     rc = hip_check(c, hipGetLastError(), "compaction");
     if (rc) return rc;
-    int32_t h = 0;
-    rc = hip_check(c, hipMemcpyAsync(&h, btotal, 4, hipMemcpyDeviceToHost, c->stream), "table pass count");
-    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "table pass count");
-    if (rc) return rc;
-    items = (int64_t)h;
   } else {
     hipLaunchKernelGGL(k_sort_key, dim3(blocks), dim3(256), 0, c->stream, d_xyz, d_poly, n, r, k0, v0);
-    int rc = hip_check(c, hipGetLastError(), "sort keys");
+    rc = hip_check(c, hipGetLastError(), "sort keys");
+    if (!rc) rc = fill_i32(c, btotal, 1, (int32_t)n);
     if (rc) return rc;
   }
-  *m = items;
-  *order = (const int32_t*)v1;
-  if (items == 0) return GAHHG_OK;
-  size_t tb = (size_t)s.temp_bytes;
-  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t*)k0, k1, (const uint32_t*)v0, v1, (int)items, 0,
-                                         end_bit, c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "sort");
+  if (host_count) {
+    rc = hip_check(c, hipMemcpyAsync(c->h_cnt, btotal, 4, hipMemcpyDeviceToHost, c->stream), "point count");
+    if (!rc) rc = hip_check(c, hipEventRecord(c->ev_cnt, c->stream), "point count");
+    if (rc) return rc;
+  }
+  uint32_t *kres = nullptr, *vres = nullptr;
+  rc = radix_sort_pairs(c, k0, v0, k1, v1, btotal, n, end_bit, hist, &kres, &vres);
+  if (rc) return rc;
+  *order = (const int32_t*)vres;
+  *d_count = btotal;
+  *m = -1;
+  if (host_count) {
+    rc = hip_check(c, hipEventSynchronize(c->ev_cnt), "point count");
+    if (rc) return rc;
+    *m = *c->h_cnt;
+  }
   return GAHHG_OK;
+}
+
+// Scratch a grouped pass over n points needs, allocated now (a captured
+// pass must neither allocate nor synchronise).
+static int reserve_query(gahhg_ctx* c, int64_t n) {
+  const int64_t nb = (n + 255) / 256;
+  const int rc = scan_reserve(c, std::max<int64_t>(nb, radix_hist_entries(n)));
+  return rc ? rc : sort_buffers(c, n);
 }
 
 template <bool TRACE>
 static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, const int32_t* d_order,
-                         int64_t n, int32_t* d_labels, int32_t* d_trace) {
+                         int64_t n, const int32_t* d_n, int32_t* d_labels, int32_t* d_trace) {
   const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
   const QueryLayout& L = c->q;
   hipLaunchKernelGGL((k_query<TRACE>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                      (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
                      L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
                      (const int32_t*)L.idx, (const double*)L.sv,
-                     (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_labels, level_tab(c),
-                     d_trace, c->prm.query_group, c->prm.query_xcd);
+                     (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_n, d_labels,
+                     level_tab(c), d_trace, c->prm.query_group, c->prm.query_xcd);
+}
+
+// The work of one accelerated query pass, enqueued with no host wait when
+// host_count is false (then it can be recorded into a graph).
+static int enqueue_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                         bool host_count) {
+  const int32_t* order = nullptr;
+  const int32_t* d_m = nullptr;
+  int64_t m = n;
+  if (c->prm.query_sort == 2 || (c->prm.query_sort == 1 && n >= kSortMin)) {
+    const int rc = sort_queries(c, d_xyz, d_poly, n, d_labels, &order, &m, &d_m, host_count);
+    if (rc) return rc;
+  }
+  if (host_count) (void)hipEventRecord(c->ev[2], c->stream);
+  if (m < 0)
+    launch_query<false>(c, d_xyz, d_poly, order, n, d_m, d_labels, nullptr);
+  else if (m > 0)
+    launch_query<false>(c, d_xyz, d_poly, order, m, nullptr, d_labels, nullptr);
+  if (host_count) (void)hipEventRecord(c->ev[3], c->stream);
+  return hip_check(c, hipGetLastError(), "query kernel");
+}
+
+void drop_query_graph(gahhg_ctx* c) {
+  QueryGraph& G = c->qgraph;
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G = QueryGraph();
+}
+
+// query_graph = 1: the pass is recorded once per (points, polygon ids,
+// labels, count) and buffer epoch, then replayed with one launch.
+static int graph_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {
+  QueryGraph& G = c->qgraph;
+  if (!(G.exec && G.xyz == d_xyz && G.poly == d_poly && G.labels == d_labels && G.n == n && G.epoch == c->qepoch)) {
+    drop_query_graph(c);
+    int rc = reserve_query(c, n);
+    if (rc) return rc;
+    rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    if (rc) return rc;
+    const int qrc = enqueue_query(c, d_xyz, d_poly, n, d_labels, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (qrc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return qrc;
+    }
+    rc = hip_check(c, e, "graph capture");
+    if (rc) return rc;
+    G.graph = graph;
+    rc = hip_check(c, hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    if (rc) {
+      drop_query_graph(c);
+      return rc;
+    }
+    G.xyz = d_xyz, G.poly = d_poly, G.labels = d_labels, G.n = n, G.epoch = c->qepoch;
+  }
+  (void)hipEventRecord(c->ev[2], c->stream);
+  const int rc = hip_check(c, hipGraphLaunch(G.exec, c->stream), "graph launch");
+  (void)hipEventRecord(c->ev[3], c->stream);
+  return rc;
 }
 
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {
@@ -1473,16 +1555,10 @@ int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t 
     hipLaunchKernelGGL(k_query_soa, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
                        (const PolyDev*)c->d_pd, (const double*)c->d_edge, d_xyz, d_poly, n, d_labels);
     (void)hipEventRecord(c->ev[3], c->stream);
+  } else if (c->prm.query_graph) {
+    return graph_query(c, d_xyz, d_poly, n, d_labels);
   } else {
-    const int32_t* order = nullptr;
-    int64_t m = n;
-    if (c->prm.query_sort == 2 || (c->prm.query_sort == 1 && n >= kSortMin)) {
-      const int rc = sort_queries(c, d_xyz, d_poly, n, d_labels, &order, &m);
-      if (rc) return rc;
-    }
-    (void)hipEventRecord(c->ev[2], c->stream);
-    if (m > 0) launch_query<false>(c, d_xyz, d_poly, order, m, d_labels, nullptr);
-    (void)hipEventRecord(c->ev[3], c->stream);
+    return enqueue_query(c, d_xyz, d_poly, n, d_labels, true);
   }
   return hip_check(c, hipGetLastError(), "query kernel");
 }
@@ -1500,7 +1576,7 @@ int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, in
                     int32_t* d_trace) {
   if (n <= 0) return GAHHG_OK;
   if (!c->q.hdr) return set_err(c, GAHHG_E_STATE, "trace: the accelerated query layout is not built");
-  launch_query<true>(c, d_xyz, d_poly, nullptr, n, d_labels, d_trace);
+  launch_query<true>(c, d_xyz, d_poly, nullptr, n, nullptr, d_labels, d_trace);
   return hip_check(c, hipGetLastError(), "traced query kernel");
 }
 

@@ -145,17 +145,23 @@ int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, con
     if (d_total) return fill_i32(c, d_total, 1, 0);
     return GAHHG_OK;
   }
-  int64_t need = scratch_need(n_bound);
-  if (need > c->scap) {
-    (void)hipStreamSynchronize(c->stream);
-    dfree(c->d_scan);
-    c->d_scan = nullptr;
-    c->scap = 0;
-    int64_t cap = need * 2;
-    if (dalloc(c, (void**)&c->d_scan, cap * (int64_t)sizeof(int32_t))) return GAHHG_E_NOMEM;
-    c->scap = cap;
-  }
+  const int rc = scan_reserve(c, n_bound);
+  if (rc) return rc;
   return scan_rec(c, in, out, n_bound, d_n, d_total, inclusive, c->d_scan);
+}
+
+int scan_reserve(gahhg_ctx* c, int64_t n_bound) {
+  const int64_t need = scratch_need(std::max<int64_t>(n_bound, 1));
+  if (need <= c->scap) return GAHHG_OK;
+  (void)hipStreamSynchronize(c->stream);
+  dfree(c->d_scan);
+  c->d_scan = nullptr;
+  c->scap = 0;
+  c->qepoch++;
+  const int64_t cap = need * 2;
+  if (dalloc(c, (void**)&c->d_scan, cap * (int64_t)sizeof(int32_t))) return GAHHG_E_NOMEM;
+  c->scap = cap;
+  return GAHHG_OK;
 }
 
 }  // namespace gahhg

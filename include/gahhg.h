@@ -87,6 +87,14 @@ typedef struct {
                               64M points and on 624 polygons with 10k);
                               0 = every polygon gets query_bins; single
                               polygons always do                           */
+  int32_t query_graph;     /* accelerated query: 1 records the pass (table
+                              pass, grouping sort, walk) as a HIP graph on
+                              first use and replays it while the points,
+                              polygon ids, labels and count are the same
+                              buffers and no build came between; the
+                              points the table leaves open are counted on
+                              the device only.  0 (default): launched
+                              kernel by kernel.  Labels are unchanged      */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by
@@ -345,6 +353,13 @@ int gahhg_export_grid(gahhg_ctx* ctx, double* v, int32_t* cells, int32_t* border
  * op 2: isEEInter(a, b, c, d) -> out[0] in {0,1};
  * op 3: isEdgeBorderInter(a, b, c, d) -> out[0] = class, out[1] = reserved. */
 int gahhg_kat_device(gahhg_ctx* ctx, int32_t op, const double* in12, int64_t n, double* out3);
+
+/* Parity hook of the grouping sort (no reference counterpart): sorts the n
+ * host pairs (keys[i], vals[i]) in place, stably, by key bits
+ * [0, end_bit) (1..32), on the device, with launches sized for n_bound >= n
+ * points and the count n read from device memory, as the grouped query
+ * runs it. */
+int gahhg_sort_pairs(gahhg_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int64_t n_bound, int32_t end_bit);
 
 #ifdef __cplusplus
 }
