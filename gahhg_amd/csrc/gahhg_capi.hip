@@ -83,6 +83,7 @@ void free_scratch(gahhg_ctx* c) {
 int upload_polygons(gahhg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->depth_override.clear();  // fixed depths belong to the previous polygons
+  c->learn.estat_valid = c->learn.hist_valid = false;
   free_grid(c);
   free_scratch(c);
   void* ptrs[] = {c->d_pv, c->d_pe, c->d_edge, c->d_epoly, c->d_pd};
@@ -852,6 +853,7 @@ int gahhg_set_query_hint(gahhg_ctx* c, const double* xyz, const int32_t* poly, i
   L.d_hint_xyz = nullptr;
   L.d_hint_poly = nullptr;
   L.hint_n = 0;
+  L.hist_valid = false;
   if (n == 0) return GAHHG_OK;
   RCK(dalloc(c, (void**)&L.d_hint_xyz, n * 24));
   RCK(hip_check(c, hipMemcpy(L.d_hint_xyz, xyz, n * 24, hipMemcpyHostToDevice), "hint upload"));

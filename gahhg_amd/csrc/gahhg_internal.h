@@ -182,6 +182,43 @@ struct GridView {
   int32_t* refs;
 };
 
+#ifdef __HIPCC__
+// The 42 level-0 cells (at most 6 borders each: 12 pentagons, 30 hexagons)
+// staged in LDS, for kernels that locate many points among them.  Level-0
+// geometry is the same for every polygon; the copy is polygon 0's.
+struct Level0Lds {
+  double n[kGrid0Cells][6][3];
+  int bn[kGrid0Cells];
+};
+
+// Block-wide load (every thread of the block calls it).
+__device__ __forceinline__ void level0_load(const GridView& g, int cell0, Level0Lds* s) {
+  for (int t = threadIdx.x; t < kGrid0Cells * 6; t += blockDim.x) {
+    const int i = t / 6, k = t % 6;
+    const int bs = g.c_bstart[cell0 + i], bn = min(g.c_bnum[cell0 + i], 6);
+    if (k == 0) s->bn[i] = bn;
+    if (k < bn) {
+      const V3 v = load3(g.b_n + 3 * (int64_t)(bs + k));
+      s->n[i][k][0] = v.x, s->n[i][k][1] = v.y, s->n[i][k][2] = v.z;
+    }
+  }
+  __syncthreads();
+}
+
+// First level-0 cell (index order) whose border planes all pass the
+// reference's inside test (n.q >= -eps in border order), or -1.
+__device__ __forceinline__ int level0_cell(const Level0Lds* s, V3 q) {
+  for (int i = 0; i < kGrid0Cells; i++) {
+    bool in = true;
+    for (int k = 0; k < s->bn[i] && in; k++)
+      in = !lt0(dot(v3(s->n[i][k][0], s->n[i][k][1], s->n[i][k][2]), q));
+    if (in) return i;
+  }
+  return -1;
+}
+#endif
+
+
 // Read-only query accelerators built after classification.
 //  table: one entry per cube-map bin (6 faces x bins x bins).  For every
 //    point inside the bin's bounding cap, the reference's first-match
@@ -246,8 +283,13 @@ struct LearnState {
   uint32_t* d_hist = nullptr;             // np x 42 hint points
   uint32_t* d_band = nullptr;             // np x 42 hint points in crossed cells
   int32_t* d_hloc = nullptr;              // per hint point: its cell at hloc_level (-1 = done)
+  int32_t* d_hloc0 = nullptr;             // per hint point: its top-level cell
   int64_t hloc_cap = 0;
   int hloc_level = 0;
+  // d_estat, and d_hist with d_hloc0, depend only on the polygons (and the
+  // hint): kept across rebuilds until those change
+  bool estat_valid = false, hist_valid = false;
+  uint32_t* d_stripes = nullptr;          // kStripes x np x 42 partial counts
   float* d_w = nullptr;                   // np x 42 expected points
   int32_t* d_stat = nullptr;              // np x 42 x 2 cells, crossed cells
   float* d_x = nullptr;                   // np x 42 x 2 x 12 features
@@ -261,6 +303,10 @@ struct LearnState {
 // Window of the learned depth decision (crossed cells / N below which a
 // polygon always refines; levels past the density rule's stop it may add).
 constexpr float kLearnFloor = 0.02f;
+// Copies of the per (polygon, top-level cell) counters that the hint kernels
+// add to (block b adds to copy b % kStripes): few blocks add to any one
+// address, whose atomics serialise.
+constexpr int kStripes = 64;
 constexpr int kLearnAbove = 2;
 // Predicted gain factor the model needs to depart from the density rule.
 constexpr double kLearnMargin = 2.0;

@@ -37,29 +37,31 @@ __device__ __forceinline__ bool inside(const GridView& g, V3 q, int cid) {
   return true;
 }
 
-// Points per (polygon, top-level cell) of the hint sample; the top-level
+// Points per (polygon, top-level cell) of the hint sample, added to the
+// block's copy of the counters (kStripes copies of npoly x 42); the top-level
 // cell is the first level-0 cell (index order) whose planes all pass, the
 // same test the query's level-0 scan applies (level-0 geometry is the same
-// for every polygon: polygon 0's cells are used).
+// for every polygon: polygon 0's cells are used, staged in LDS).
 __global__ void k_hint_hist(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ xyz,
                             const int32_t* __restrict__ poly, int64_t n, int npoly, uint32_t* __restrict__ hist,
                             int32_t* __restrict__ loc) {
+  __shared__ Level0Lds l0;
+  level0_load(g, pd[0].cell0, &l0);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   if (t < n) {
     int cell = -1;
     const int p = poly ? poly[t] : 0;
     if (p >= 0 && p < npoly) {
-      const V3 q = load3(xyz + 3 * t);
-      for (int i = 0; i < kGrid0Cells && key < 0; i++)
-        if (inside(g, q, i)) {
-          key = p * kGrid0Cells + i;
-          cell = pd[p].cell0 + i;
-        }
+      const int i = level0_cell(&l0, load3(xyz + 3 * t));
+      if (i >= 0) {
+        key = p * kGrid0Cells + i;
+        cell = pd[p].cell0 + i;
+      }
     }
     loc[t] = cell;
   }
-  wave_add<uint32_t>(hist, key, 1u);
+  wave_add<uint32_t>(hist + (int64_t)(blockIdx.x % kStripes) * npoly * kGrid0Cells, key, 1u);
 }
 
 // Hint points in the boundary band: per (polygon, top-level cell), the hint
@@ -68,7 +70,7 @@ __global__ void k_hint_hist(GridView g, const PolyDev* __restrict__ pd, const do
 // the point's cell `steps` levels above (or -1 once it ended in a leaf above
 // the band, where it stays); the kernel descends it and stores the cell.
 __global__ void k_hint_band(GridView g, const double* __restrict__ xyz, int32_t* __restrict__ loc, int64_t n,
-                            int steps, int marked, uint32_t* __restrict__ band) {
+                            int steps, int marked, int64_t nbins, uint32_t* __restrict__ band) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   if (t < n) {
@@ -96,7 +98,16 @@ __global__ void k_hint_band(GridView g, const double* __restrict__ xyz, int32_t*
         key = g.c_poly[cur] * kGrid0Cells + g.c_root[cur];
     }
   }
-  wave_add<uint32_t>(band, key, 1u);
+  wave_add<uint32_t>(band + (blockIdx.x % kStripes) * nbins, key, 1u);
+}
+
+// out[b] = the sum of the kStripes copies of counter b.
+__global__ void k_sum_stripes(const uint32_t* __restrict__ stripes, int64_t nbins, uint32_t* __restrict__ out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  uint32_t s = 0;
+  for (int k = 0; k < kStripes; k++) s += stripes[k * nbins + b];
+  out[b] = s;
 }
 
 // Expected points per (polygon, top-level cell): the hint histogram times
@@ -215,8 +226,12 @@ int ensure_learn(gahhg_ctx* c) {
   L.d_y = nullptr;
   L.d_dec = nullptr;
   L.cap = 0;
+  L.estat_valid = L.hist_valid = false;
+  dfree(L.d_stripes);
+  L.d_stripes = nullptr;
   const int64_t pr = np * kGrid0Cells;
   if (dalloc(c, (void**)&L.d_estat, pr * 3 * 8) || dalloc(c, (void**)&L.d_hist, pr * 4) ||
+      dalloc(c, (void**)&L.d_stripes, pr * kStripes * 4) ||
       dalloc(c, (void**)&L.d_band, pr * 4) ||
       dalloc(c, (void**)&L.d_w, pr * 4) || dalloc(c, (void**)&L.d_stat, pr * 2 * 4) ||
       dalloc(c, (void**)&L.d_x, pr * 2 * kMlpIn * 4) || dalloc(c, (void**)&L.d_y, pr * 2 * 4) ||
@@ -237,25 +252,41 @@ int learn_prepare(gahhg_ctx* c) {
   int rc = ensure_learn(c);
   if (rc) return rc;
   const int np = (int)c->polys.size();
-  rc = edge_stats(c, L.d_estat);
-  if (rc) return rc;
+  if (!L.estat_valid) {
+    rc = edge_stats(c, L.d_estat);
+    if (rc) return rc;
+    L.estat_valid = true;
+  }
   const uint32_t* hist = nullptr;
   if (L.d_hint_xyz && L.hint_n > 0) {
-    rc = fill_i32(c, (int32_t*)L.d_hist, (int64_t)np * kGrid0Cells, 0);
-    if (rc) return rc;
+    const int64_t nbins = (int64_t)np * kGrid0Cells;
     if (L.hloc_cap < L.hint_n) {
       dfree(L.d_hloc);
-      L.d_hloc = nullptr;
+      dfree(L.d_hloc0);
+      L.d_hloc = L.d_hloc0 = nullptr;
       L.hloc_cap = 0;
-      if (dalloc(c, (void**)&L.d_hloc, L.hint_n * 4)) return GAHHG_E_NOMEM;
+      L.hist_valid = false;
+      if (dalloc(c, (void**)&L.d_hloc, L.hint_n * 4) || dalloc(c, (void**)&L.d_hloc0, L.hint_n * 4))
+        return GAHHG_E_NOMEM;
       L.hloc_cap = L.hint_n;
     }
-    L.hloc_level = 0;
-    hipLaunchKernelGGL(k_hint_hist, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
-                       (const PolyDev*)c->d_pd, (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly, L.hint_n,
-                       np, L.d_hist, L.d_hloc);
-    rc = hip_check(c, hipGetLastError(), "hint histogram");
+    if (!L.hist_valid) {
+      rc = fill_i32(c, (int32_t*)L.d_stripes, nbins * kStripes, 0);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_hint_hist, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
+                         (const PolyDev*)c->d_pd, (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly,
+                         L.hint_n, np, L.d_stripes, L.d_hloc0);
+      rc = hip_check(c, hipGetLastError(), "hint histogram");
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_sum_stripes, dim3(blocks(nbins, 256)), dim3(256), 0, c->stream,
+                         (const uint32_t*)L.d_stripes, nbins, L.d_hist);
+      rc = hip_check(c, hipGetLastError(), "hint histogram sum");
+      if (rc) return rc;
+      L.hist_valid = true;
+    }
+    rc = copy_i32(c, L.d_hloc, L.d_hloc0, L.hint_n);  // the band descent starts at the top-level cells
     if (rc) return rc;
+    L.hloc_level = 0;
     hist = L.d_hist;
   }
   const double q_default = np > 0 ? L.cost[0] / np : 0.0;
@@ -279,13 +310,19 @@ static int learn_features(gahhg_ctx* c, int level, int lo, int n, int marked, in
   }
   const bool hint = L.d_hint_xyz && L.hint_n > 0;
   if (hint) {
-    rc = fill_i32(c, (int32_t*)L.d_band, (int64_t)np * kGrid0Cells, 0);
+    const int64_t nbins = (int64_t)np * kGrid0Cells;
+    rc = fill_i32(c, (int32_t*)L.d_stripes, nbins * kStripes, 0);
     if (rc) return rc;
     if (level < L.hloc_level) return set_err(c, GAHHG_E_STATE, "features: levels must not decrease");
     hipLaunchKernelGGL(k_hint_band, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
-                       (const double*)L.d_hint_xyz, L.d_hloc, L.hint_n, level - L.hloc_level, marked, L.d_band);
+                       (const double*)L.d_hint_xyz, L.d_hloc, L.hint_n, level - L.hloc_level, marked, nbins,
+                       L.d_stripes);
     L.hloc_level = level;
     rc = hip_check(c, hipGetLastError(), "hint band");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_stripes, dim3(blocks(nbins, 256)), dim3(256), 0, c->stream,
+                       (const uint32_t*)L.d_stripes, nbins, L.d_band);
+    rc = hip_check(c, hipGetLastError(), "hint band sum");
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_features, dim3(blocks((int64_t)np * kGrid0Cells * 2, 256)), dim3(256), 0, c->stream, np,
@@ -334,7 +371,7 @@ int learn_features_after_build(gahhg_ctx* c, int level, float* h_x) {
 void learn_free(gahhg_ctx* c) {
   LearnState& L = c->learn;
   void* ptrs[] = {L.d_estat, L.d_hist, L.d_band, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec, L.d_weights,
-                  L.d_hint_xyz, L.d_hint_poly, L.d_hloc};
+                  L.d_hint_xyz, L.d_hint_poly, L.d_hloc, L.d_hloc0, L.d_stripes};
   for (void* q : ptrs) dfree(q);
   L = LearnState();
 }

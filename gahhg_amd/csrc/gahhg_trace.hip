@@ -56,21 +56,15 @@ __global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const d
                              const int32_t* __restrict__ epoly, const int32_t* __restrict__ pe, int64_t Nt,
                              unsigned long long* __restrict__ out) {
   // every lane reaches the wave-aggregated adds (invalid lanes add to key -1)
+  __shared__ Level0Lds l0;  // level-0 geometry is the same for every polygon
+  level0_load(g, pd[0].cell0, &l0);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   unsigned long long len_fx = 0, turn_fx = 0;
   if (e < Nt) {
     const int p = epoly[e];
-    const PolyDev& P = pd[p];
     const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
-    int root = -1;
-    for (int i = 0; i < kGrid0Cells && root < 0; i++) {
-      const int cid = P.cell0 + i;
-      const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
-      bool in = true;
-      for (int k = 0; k < bn && in; k++) in = !lt0(dot(load3(g.b_n + 3 * (int64_t)(bs + k)), p0));
-      if (in) root = i;
-    }
+    const int root = level0_cell(&l0, p0);
     if (root >= 0) {
       const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
       double turn = 0.0;
