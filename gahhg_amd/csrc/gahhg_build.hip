@@ -248,9 +248,11 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
-// MODE kMark walks the edges of refining polygons at `level`; kCount and
-// kWrite walk every edge at its polygon's bottom level, starting from the
-// level above it (buf_prev) when refinement stopped by the density rule.
+// MODE kMark walks the edges of refining polygons at `level` (and keeps
+// each edge's cell count in cnt_out); kCount and kWrite walk every edge at
+// its polygon's bottom level, starting from the level above it (buf_prev)
+// when refinement stopped by the density rule or the model after marking
+// that level (kCount then takes the count the mark walk kept).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const double* __restrict__ edge,
                                                  const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd,
@@ -265,12 +267,22 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const double* __res
       if (P.refining) {
         buf_prev[e] = buf[e];
         walk_edge<MODE>(g, edge, e, level, P, buf, nullptr, d);
+        // the count walk of a polygon that stops at this level repeats this
+        // walk exactly (same start, same cells): keep its count, unless
+        // the walk raised a diagnostic (then the count walk runs and
+        // counts it as the reference's dispatch does)
+        const bool clean = d.partner_missing == 0 && d.cap_hits == 0 && d.unlocated == 0 && d.bad_index == 0;
+        cnt_out[e] = clean ? (int32_t)d.steps : -1;
       }
     } else {
       int32_t* b = P.use_prev ? buf_prev : buf;
-      int32_t* oc = (MODE == kWrite) ? rcell + off[e] : nullptr;
-      walk_edge<MODE>(g, edge, e, P.L, P, b, oc, d);
-      if (MODE == kCount) cnt_out[e] = (int32_t)d.steps;
+      if (MODE == kCount && P.use_prev && cnt_out[e] >= 0) {
+        d.steps = cnt_out[e];  // walked by the mark walk of the bottom level
+      } else {
+        int32_t* oc = (MODE == kWrite) ? rcell + off[e] : nullptr;
+        walk_edge<MODE>(g, edge, e, P.L, P, b, oc, d);
+        if (MODE == kCount) cnt_out[e] = (int32_t)d.steps;
+      }
     }
   }
   // counters: summed per wave, then per block, one atomic per block and
@@ -988,6 +1000,12 @@ int build_hierarchy(gahhg_ctx* c) {
   CK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state init"));
   CK(fill_i32(c, c->d_loc_cur, Nt, 0));
   CK(fill_i32(c, c->d_loc_prev, Nt, 0));
+  if (c->ecap < Nt) {  // per-edge counts: written by the mark walks, read by the dispatch
+    dfree(c->d_ecnt);
+    c->d_ecnt = nullptr;
+    CK(dalloc(c, (void**)&c->d_ecnt, Nt * 4));
+    c->ecap = Nt;
+  }
 
   GridView g = view(c);
   c->pmarks.assign(c->pd.size(), 0);
@@ -1028,7 +1046,7 @@ int build_hierarchy(gahhg_ctx* c) {
     LAUNCH_CK("zero");
     hipLaunchKernelGGL(k_walk<kMark>, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const double*)c->d_edge, (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, lv,
-                       c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                       c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr, c->d_ecnt, (int32_t*)nullptr,
                        c->d_st);
     LAUNCH_CK("mark walk");
     hipLaunchKernelGGL(k_plan, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
