@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=64_000_000)
     ap.add_argument("--no-series", action="store_true")
+    ap.add_argument("--no-f512", action="store_true", help="skip the 512-polygon learned-vs-rule A/B")
+    ap.add_argument("--f512-queries", type=int, default=32_000_000)
     ap.add_argument("--no-load", action="store_true")
     ap.add_argument("--load-lines", type=int, default=20_000_000)
     ap.add_argument("--rule", action="store_true", help="density-rule depth only (no learned model)")
@@ -293,6 +295,8 @@ def main():
     del xyz, labels
     if not args.no_batch:
         out["batch_256"] = bench_batch(args, oracle_lib)
+    if not args.no_f512:
+        out["fractal_512"] = bench_fractal512(args, oracle_lib)
     if not args.no_series:
         out["series_mars"] = bench_series(args, oracle_lib)
     if not args.no_load:
@@ -408,6 +412,79 @@ def bench_series(args, oracle_lib):
            "wall_speedup": round(per_wall / float(np.median(walls)), 2),
            "label_differences_vs_per_polygon": mism, "oracle_mismatches": omism}
     log(f"series_mars: {res}")
+    return res
+
+
+def bench_fractal512(args, oracle_lib):
+    """fractal_512 (512 polygons of the fractal_1m generator scaled to 4k-64k
+    outer vertices, 32M points): the learned depth against the density rule
+    as one batch each, on build ms, queries/s and grid bytes; labels of the
+    two must be equal and match the oracle on a sample of polygons."""
+    import gahhg_amd
+    from gahhg_amd import datasets
+
+    t0 = time.time()
+    w = datasets.fractal_512(n_queries=args.f512_queries)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in w["polys"]]
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    qp = w["qpoly"]
+    Q = len(qp)
+    P = len(items)
+    n_edges = int(sum(len(p[0]) - (len(p[2]) - 1) for p in items))
+    log(f"fractal_512: {P} polygons, {n_edges} edges, {Q} queries generated in {time.time() - t0:.1f}s")
+    model = load_depth_model(args)
+    res = {"n_polygons": P, "n_edges": n_edges, "n_queries": Q}
+    labs, depths = {}, {}
+    for mode in ("rule", "learned"):
+        if mode == "learned" and model is None:
+            continue
+        g = gahhg_amd.GPISP(device=0, query_bins=args.bins, **query_kw(args))
+        g.set_polygons_lonlat(items)
+        if mode == "learned":
+            apply_model(g, model, xyz, qp, Q)
+        build_ms = median_build(g, 3)[0]
+        st = [g.polygon_stats(k) for k in range(P)]
+        depths[mode] = [s["depth"] for s in st]
+        grid = int(sum(s["gpu_space_bytes"] for s in st))
+        dx, dp, dl = g.alloc(xyz.nbytes), g.alloc(4 * Q), g.alloc(4 * Q)
+        dx.upload(xyz)
+        dp.upload(qp)
+        g.query_batch_device(dx, dp, Q, dl)
+        qms = []
+        for _ in range(5):
+            g.query_batch_device(dx, dp, Q, dl)
+            qms.append(g.stats()["query_ms"])
+        labs[mode] = dl.download(np.empty(Q, dtype=np.int32))
+        for b in (dx, dp, dl):
+            b.free()
+        g.close()
+        q = float(np.median(qms))
+        res[mode] = {"build_ms": round(build_ms, 3), "query_gpu_ms": round(q, 3),
+                     "queries_per_s": round(Q / (q * 1e-3), 1), "grid_bytes": grid,
+                     "mean_depth": round(float(np.mean(depths[mode])), 3)}
+        log(f"fractal_512 {mode}: {res[mode]}")
+    # oracle on the points of 24 polygons (checker only)
+    rng = np.random.default_rng(7)
+    check = rng.choice(P, min(P, 24), replace=False)
+    final = labs.get("learned", labs["rule"])
+    fd = depths.get("learned", depths["rule"])
+    om = 0
+    on = 0
+    for k in check:
+        sel = np.nonzero(qp == k)[0][:20000]
+        o = oracle_lib.Oracle().set_lonlat(*items[k]).build(max_depth=int(fd[k]), auto_depth=False)
+        om += int(np.count_nonzero(final[sel] != o.query_xyz(np.ascontiguousarray(xyz[sel]))))
+        on += len(sel)
+    res["oracle_sample"] = on
+    res["oracle_mismatches"] = om
+    if "learned" in res:
+        r, l = res["rule"], res["learned"]
+        res["label_differences_learned_vs_rule"] = int(np.count_nonzero(labs["rule"] != labs["learned"]))
+        res["depth_changes"] = int(sum(a != b for a, b in zip(depths["rule"], depths["learned"])))
+        res["learned_vs_rule"] = {"build": round(l["build_ms"] / r["build_ms"], 4),
+                                  "queries_per_s": round(l["queries_per_s"] / r["queries_per_s"], 4),
+                                  "grid_bytes": round(l["grid_bytes"] / r["grid_bytes"], 4)}
+    log(f"fractal_512: {res}")
     return res
 
 

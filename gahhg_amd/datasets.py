@@ -8,6 +8,8 @@ Workloads (sizes in edges N and queries Q):
                encloses the North Pole) + 3 holes of 50k, 100M queries
                (70% uniform, 30% near-edge)
   batch_256    256 polygons, log-uniform vertex counts in [16, 200k]
+  fractal_512  512 polygons of the fractal_1m generator scaled to 4k-64k
+               outer vertices, 32M queries (fractal_1m's mix per polygon)
 
 Rings are produced as lon/lat degrees with the closing vertex repeated, in
 the layout of the reference's polygon TXT files (README.md:41-69); the
@@ -276,4 +278,41 @@ def batch_256(n_queries=64_000_000, seed=4, n_poly=256):
     return dict(name="batch_256", polys=polys, qlon=qlon, qlat=qlat, qpoly=pid)
 
 
-WORKLOADS = {"star_1k": star_1k, "fractal_100k": fractal_100k, "fractal_1m": fractal_1m, "batch_256": batch_256}
+def fractal_512(n_queries=32_000_000, seed=6, n_poly=512, n_min=4096, n_max=65536):
+    """512 polygons of the fractal_1m generator (fractal outer ring on a
+    64-gon, 3 star holes at 0.375 R of radius 0.1 R with n/20 vertices),
+    scaled down to outer vertex counts log-uniform in [n_min, n_max] (the
+    full 1M vertices would make 512M), centres uniform on the sphere, radii
+    U(10, 40) deg.  Each polygon gets n_queries / n_poly points, fractal_1m's
+    mix: 70% uniform in its cap of radius 1.3 R, 30% near its edges."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    for k in range(n_poly):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        lon0 = float(rng.uniform(-180.0, 180.0))
+        lat0 = float(np.degrees(np.arcsin(rng.uniform(-1.0, 1.0))))
+        R = float(rng.uniform(10.0, 40.0))
+        outer = fractal_ring(rng, lon0, lat0, n, R)
+        holes = []
+        for j in range(3):
+            hl, ha = destination(lon0, lat0, 2.0 * np.pi * j / 3 + np.pi / 3, 0.375 * R * DEG)
+            holes.append(star_ring(rng, float(hl), float(ha), max(16, n // 20), 0.1 * R))
+        lon, lat, off = assemble([outer] + holes)
+        polys.append(dict(lon=lon, lat=lat, ring_off=off, centre=(lon0, lat0), radius=R))
+    per = n_queries // n_poly
+    qrng = np.random.default_rng(seed + 1000)
+    qlon, qlat, qpoly = [], [], []
+    for k, c in enumerate(polys):
+        nu = int(round(0.7 * per))
+        ul, ua = cap_queries(qrng, c["centre"][0], c["centre"][1], 1.3 * c["radius"], nu)
+        el, ea = near_edge_queries(qrng, c["lon"], c["lat"], c["ring_off"], per - nu)
+        qlon += [ul, el]
+        qlat += [ua, ea]
+        qpoly.append(np.full(per, k, dtype=np.int32))
+    perm = qrng.permutation(per * n_poly)
+    return dict(name="fractal_512", polys=polys, qlon=np.concatenate(qlon)[perm], qlat=np.concatenate(qlat)[perm],
+                qpoly=np.concatenate(qpoly)[perm])
+
+
+WORKLOADS = {"star_1k": star_1k, "fractal_100k": fractal_100k, "fractal_1m": fractal_1m, "batch_256": batch_256,
+             "fractal_512": fractal_512}
