@@ -80,7 +80,7 @@ EXPORTED = [
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
-    "gahhg_load_polygon_txt_device", "gahhg_sort_pairs",
+    "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -138,6 +138,7 @@ def lib():
         "gahhg_num_polygons": ([P, ctypes.POINTER(i32)], ctypes.c_int),
         "gahhg_query_batch_xyz": ([P, dp, ip, i64, ip], ctypes.c_int),
         "gahhg_query_batch_xyz_device": ([P, P, P, i64, P], ctypes.c_int),
+        "gahhg_query_pairs_xyz": ([P, dp, i64, ip], ctypes.c_int),
         "gahhg_get_polygon_stats": ([P, i32, ctypes.POINTER(Stats)], ctypes.c_int),
         "gahhg_polygon_grid_counts": ([P, i32, lp], ctypes.c_int),
         "gahhg_export_polygon_grid": ([P, i32, P, P, P, P, P, P], ctypes.c_int),
@@ -411,6 +412,14 @@ class GPISP:
         poly = np.ascontiguousarray(poly, dtype=np.int32)
         labels = np.zeros(poly.size, dtype=np.int32)
         self._check(self.L.gahhg_query_batch_xyz(self.h, xyz, poly, poly.size, labels))
+        return labels
+
+    def query_pairs_xyz(self, xyz):
+        """Every polygon of the batch against the same points: labels of
+        shape (polygons, points)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        labels = np.zeros((self.n_polygons, xyz.shape[0]), dtype=np.int32)
+        self._check(self.L.gahhg_query_pairs_xyz(self.h, xyz.reshape(-1), xyz.shape[0], labels.reshape(-1)))
         return labels
 
     def query_batch_device(self, d_xyz, d_poly, n, d_labels):

@@ -690,6 +690,7 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   if (!d_poly && c->polys.size() != 1)
     return set_err(c, GAHHG_E_ARG, "query: polygon ids required for a batch of %d", (int)c->polys.size());
   RCK(select_device(c));
+  RCK(check_poly_ids(c, d_poly, n));
   RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
   RCK(run_query(c, d_xyz, d_poly, n, d_labels));
   RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
@@ -725,6 +726,7 @@ int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
     return set_err(c, GAHHG_E_ARG, "autotune: polygon ids required for a batch of %d", (int)c->polys.size());
   if (c->prm.query_layout != 0) return set_err(c, GAHHG_E_STATE, "autotune: the accelerated query is off");
   RCK(select_device(c));
+  RCK(check_poly_ids(c, d_poly, n));
   const int groups[] = {0, 4, 8, 16, 32, 64};  // 0: each polygon's own width
   const int blocks[] = {64, 128, 256};  // the query kernels' launch bound is 256
   const gahhg_params keep = c->prm;
@@ -775,6 +777,7 @@ int gahhg_trace_query_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d
   if (!d_poly && c->polys.size() != 1)
     return set_err(c, GAHHG_E_ARG, "trace: polygon ids required for a batch of %d", (int)c->polys.size());
   RCK(select_device(c));
+  RCK(check_poly_ids(c, d_poly, n));
   RCK(run_query_trace(c, d_xyz, d_poly, n, d_labels, d_trace));
   return hip_check(c, hipStreamSynchronize(c->stream), "trace sync");
 }
@@ -906,6 +909,31 @@ int gahhg_query_batch_xyz(gahhg_ctx* c, const double* xyz, const int32_t* poly, 
   if (!rc) rc = hip_check(c, hipMemcpy(d_poly, poly, n * 4, hipMemcpyHostToDevice), "query H2D");
   if (!rc) rc = gahhg_query_batch_xyz_device(c, d_xyz, d_poly, n, d_lab);
   if (!rc) rc = hip_check(c, hipMemcpy(labels, d_lab, n * 4, hipMemcpyDeviceToHost), "labels D2H");
+  dfree(d_xyz);
+  dfree(d_lab);
+  dfree(d_poly);
+  return rc;
+}
+
+int gahhg_query_pairs_xyz(gahhg_ctx* c, const double* xyz, int64_t n, int32_t* labels) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
+  if (n < 0 || (n > 0 && (!xyz || !labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
+  const int64_t total = n * (int64_t)c->polys.size();
+  if (total == 0) return GAHHG_OK;
+  if (total > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "query: %lld pairs exceed 2^31", (long long)total);
+  RCK(select_device(c));
+  double *d_in = nullptr, *d_xyz = nullptr;
+  int32_t *d_lab = nullptr, *d_poly = nullptr;
+  RCK(dalloc(c, (void**)&d_in, n * 24));
+  int rc = dalloc(c, (void**)&d_xyz, total * 24);
+  if (!rc) rc = dalloc(c, (void**)&d_lab, total * 4);
+  if (!rc) rc = dalloc(c, (void**)&d_poly, total * 4);
+  if (!rc) rc = hip_check(c, hipMemcpy(d_in, xyz, n * 24, hipMemcpyHostToDevice), "query H2D");
+  if (!rc) rc = expand_pairs(c, d_in, n, d_xyz, d_poly);
+  if (!rc) rc = gahhg_query_batch_xyz_device(c, d_xyz, d_poly, total, d_lab);
+  if (!rc) rc = hip_check(c, hipMemcpy(labels, d_lab, total * 4, hipMemcpyDeviceToHost), "labels D2H");
+  dfree(d_in);
   dfree(d_xyz);
   dfree(d_lab);
   dfree(d_poly);

@@ -131,6 +131,7 @@ struct DevState {
   unsigned long long steps_mark, steps_dispatch, partner_missing, cap_hits, unlocated, bad_index;
   unsigned long long zero_ref_edges, undefined_leaves;
   unsigned long long q_in, q_undef;  // label census of the last query
+  int32_t bad_poly;                  // a device polygon id was out of range
   LevelDesc lv[kMaxLevels];
 };
 
@@ -461,6 +462,12 @@ int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
+// GAHHG_E_ARG unless every d_poly[0..n) is a polygon of the batch (a
+// device pass and one synchronisation).
+int check_poly_ids(gahhg_ctx* c, const int32_t* d_poly, int64_t n);
+// Every polygon against the same n points: writes n_poly * n points and
+// polygon ids (point-major within each polygon) to d_xyz_out / d_poly_out.
+int expand_pairs(gahhg_ctx* c, const double* d_xyz, int64_t n, double* d_xyz_out, int32_t* d_poly_out);
 // Groups the points for the walk; *order = point indices in group order.
 // Their number goes to *m, or, with host_count false, stays on the device
 // at *d_count (*m = -1) so that nothing waits for it.

@@ -1580,6 +1580,48 @@ int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, in
   return hip_check(c, hipGetLastError(), "traced query kernel");
 }
 
+__global__ void k_check_ids(const int32_t* __restrict__ poly, int64_t n, int npoly, DevState* st) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if ((uint32_t)poly[i] >= (uint32_t)npoly) st->bad_poly = 1;  // any writer: same value
+}
+
+__global__ void k_clear_bad(DevState* st) { st->bad_poly = 0; }
+
+int check_poly_ids(gahhg_ctx* c, const int32_t* d_poly, int64_t n) {
+  if (!d_poly || n <= 0) return GAHHG_OK;
+  hipLaunchKernelGGL(k_clear_bad, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_check_ids, dim3((unsigned)blocks), dim3(256), 0, c->stream, d_poly, n,
+                     (int)c->polys.size(), c->d_st);
+  int32_t bad = 0;
+  int rc = hip_check(c, hipMemcpyAsync(&c->h_st->bad_poly, &c->d_st->bad_poly, 4, hipMemcpyDeviceToHost, c->stream),
+                     "polygon id check");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "polygon id check");
+  if (rc) return rc;
+  bad = c->h_st->bad_poly;
+  return bad ? set_err(c, GAHHG_E_ARG, "query: a polygon id is outside 0..%d", (int)c->polys.size() - 1) : GAHHG_OK;
+}
+
+__global__ void k_expand_pairs(const double* __restrict__ xyz, int64_t n, int64_t total, double* __restrict__ out,
+                               int32_t* __restrict__ poly) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = t / n, i = t - p * n;
+    out[3 * t] = xyz[3 * i];
+    out[3 * t + 1] = xyz[3 * i + 1];
+    out[3 * t + 2] = xyz[3 * i + 2];
+    poly[t] = (int32_t)p;
+  }
+}
+
+int expand_pairs(gahhg_ctx* c, const double* d_xyz, int64_t n, double* d_xyz_out, int32_t* d_poly_out) {
+  const int64_t total = n * (int64_t)c->polys.size();
+  if (total <= 0) return GAHHG_OK;
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_expand_pairs, dim3((unsigned)blocks), dim3(256), 0, c->stream, d_xyz, n, total, d_xyz_out,
+                     d_poly_out);
+  return hip_check(c, hipGetLastError(), "pair expansion");
+}
+
 __global__ void k_clear_census(DevState* st) {
   st->q_in = 0;
   st->q_undef = 0;

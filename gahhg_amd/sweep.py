@@ -163,10 +163,16 @@ def run_jobs(jobs, device=0, **gpisp_kw):
             return out
         g.build()
         out["build_ms"] = build_ms = g.stats()["pre_ms"]
-        xyz = np.concatenate([points[jobs[i].tpfile] for i in live])
         cnt = np.array([len(points[jobs[i].tpfile]) for i in live], dtype=np.int64)
-        poly = np.repeat(np.arange(len(live), dtype=np.int32), cnt)
-        labels = g.query_batch_xyz(xyz, poly)
+        tps = {jobs[i].tpfile for i in live}
+        if len(tps) == 1:
+            # a series (one query file for every polygon): the points go to
+            # the device once and every polygon answers all of them
+            labels = g.query_pairs_xyz(points[tps.pop()]).reshape(-1)
+        else:
+            xyz = np.concatenate([points[jobs[i].tpfile] for i in live])
+            poly = np.repeat(np.arange(len(live), dtype=np.int32), cnt)
+            labels = g.query_batch_xyz(xyz, poly)
         out["query_ms"] = query_ms = g.stats()["query_ms"]
         out["wall_ms"] = (time.perf_counter() - t0) * 1e3
         off = np.concatenate([[0], np.cumsum(cnt)])

@@ -207,10 +207,17 @@ int gahhg_set_polygons_lonlat(gahhg_ctx* ctx, const double* lon, const double* l
  * test/explist.bat with one batch. */
 int gahhg_load_polygons_txt(gahhg_ctx* ctx, const char* const* paths, int32_t n);
 int gahhg_num_polygons(gahhg_ctx* ctx, int32_t* n);
-/* Queries against a batch: poly[i] selects the polygon of point i. */
+/* Queries against a batch: poly[i] selects the polygon of point i (the
+ * device form checks every id on the device first: GAHHG_E_ARG if one is
+ * outside the batch). */
 int gahhg_query_batch_xyz(gahhg_ctx* ctx, const double* xyz, const int32_t* poly, int64_t n, int32_t* labels);
 int gahhg_query_batch_xyz_device(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n,
                                  int32_t* d_labels);
+/* Every polygon of the batch against the same n points, as a series of
+ * GPISP.exe runs with one query file (test/explist.bat): labels[p * n + i]
+ * is point i in polygon p.  The points are sent once and repeated on the
+ * device; n x polygons must stay below 2^31. */
+int gahhg_query_pairs_xyz(gahhg_ctx* ctx, const double* xyz, int64_t n, int32_t* labels);
 /* Per-polygon statistics and grid export in that polygon's own numbering. */
 int gahhg_get_polygon_stats(gahhg_ctx* ctx, int32_t p, gahhg_stats* out);
 int gahhg_polygon_grid_counts(gahhg_ctx* ctx, int32_t p, int64_t* counts5);

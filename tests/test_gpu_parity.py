@@ -387,3 +387,40 @@ def test_leaf_index_agrees():
     assert np.count_nonzero(l1 != o.query_xyz(xyz)) == 0
     g0.close()
     g1.close()
+
+
+def test_pairs_query_and_device_id_check():
+    """gahhg_query_pairs_xyz (every polygon x the same points) equals the
+    batch query with the points repeated per polygon; device polygon ids
+    outside the batch are rejected before any query kernel runs."""
+    say("pairs query and device polygon-id check")
+    items = ["Square_72.txt", "Radiometer.txt", "Mars_300.txt"]
+    g = gahhg_amd.GPISP(device=0)
+    g.load_polygons_txt([refpath(n) for n in items]).build()
+    P = g.n_polygons
+    rng = np.random.default_rng(9)
+    lon, lat = datasets.uniform_queries(rng, 7001)
+    xyz = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    pairs = g.query_pairs_xyz(xyz)
+    assert pairs.shape == (P, len(xyz))
+    rep = g.query_batch_xyz(np.tile(xyz, (P, 1)), np.repeat(np.arange(P, dtype=np.int32), len(xyz)))
+    assert (pairs.reshape(-1) == rep).all()
+    for k, name in enumerate(items):
+        o = ol.Oracle().load_txt(refpath(name)).build()
+        assert np.count_nonzero(pairs[k] != o.query_xyz(xyz)) == 0, name
+    n = len(xyz)
+    dx, dp, dl = g.alloc(xyz.nbytes), g.alloc(4 * n), g.alloc(4 * n)
+    dx.upload(xyz)
+    for bad in (P, -1):
+        ids = np.zeros(n, np.int32)
+        ids[n // 2] = bad
+        dp.upload(ids)
+        with pytest.raises(gahhg_amd.GahhgError):
+            g.query_batch_device(dx, dp, n, dl)
+    ids = np.full(n, P - 1, np.int32)
+    dp.upload(ids)
+    g.query_batch_device(dx, dp, n, dl)
+    assert (dl.download(np.empty(n, np.int32)) == pairs[P - 1]).all()
+    for b in (dx, dp, dl):
+        b.free()
+    g.close()
