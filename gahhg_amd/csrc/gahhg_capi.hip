@@ -343,9 +343,16 @@ int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
   RCK(ctx_ok(c));
   if (!path) return set_err(c, GAHHG_E_ARG, "null path");
   RCK(select_device(c));
+  PolygonText t;
+  RCK(parse_polygon_txt(path, &t, &c->err));
+  // large polygons: the vertices' conversion on the device (same bits)
+  std::vector<double> xyz;
+  if ((int64_t)t.lo.size() >= kGpuConvertMin) {
+    xyz.resize(3 * t.lo.size());
+    RCK(lonlat_to_xyz_gpu(c, t.lo.data(), t.la.data(), (int64_t)t.lo.size(), xyz.data()));
+  }
   Polygon p;
-  int rc = load_polygon_txt(path, &p, &c->err);
-  if (rc) return rc;
+  RCK(polygon_from_text(t, xyz.empty() ? nullptr : xyz.data(), &p, &c->err));
   c->polys.clear();
   c->polys.push_back(std::move(p));
   return upload_polygons(c);
@@ -401,24 +408,52 @@ int gahhg_load_polygons_txt(gahhg_ctx* c, const char* const* paths, int32_t n) {
   for (int k = 0; k < n; k++)
     if (!paths[k]) return set_err(c, GAHHG_E_ARG, "batch: null path %d", k);
   // files parsed on up to 16 host threads; the first failing file (in list
-  // order) gives the error, as a sequential load would
+  // order) gives the error, as a sequential load would.  All vertices are
+  // then converted to xyz in one device pass (the same bits as the host
+  // conversion) and the polygons assembled on the threads again.
+  std::vector<PolygonText> ts((size_t)n);
   std::vector<Polygon> ps((size_t)n);
   std::vector<int> rcs((size_t)n, GAHHG_OK);
   std::vector<std::string> errs((size_t)n);
-  std::atomic<int> next(0);
   const int threads = std::max(1, std::min({16, n, (int)std::thread::hardware_concurrency()}));
-  auto work = [&]() {
-    for (int k; (k = next.fetch_add(1)) < n;) rcs[(size_t)k] = load_polygon_txt(paths[k], &ps[(size_t)k], &errs[(size_t)k]);
+  auto parallel = [&](auto&& fn) {
+    std::atomic<int> next(0);
+    auto work = [&]() {
+      for (int k; (k = next.fetch_add(1)) < n;) fn(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (std::thread& th : pool) th.join();
   };
-  std::vector<std::thread> pool;
-  for (int t = 1; t < threads; t++) pool.emplace_back(work);
-  work();
-  for (std::thread& th : pool) th.join();
-  for (int k = 0; k < n; k++)
-    if (rcs[(size_t)k]) {
-      c->err = errs[(size_t)k];
-      return rcs[(size_t)k];
+  auto first_error = [&]() {
+    for (int k = 0; k < n; k++)
+      if (rcs[(size_t)k]) {
+        c->err = errs[(size_t)k];
+        return rcs[(size_t)k];
+      }
+    return (int)GAHHG_OK;
+  };
+  parallel([&](int k) { rcs[(size_t)k] = parse_polygon_txt(paths[k], &ts[(size_t)k], &errs[(size_t)k]); });
+  RCK(first_error());
+  std::vector<int64_t> voff((size_t)n + 1, 0);
+  for (int k = 0; k < n; k++) voff[(size_t)k + 1] = voff[(size_t)k] + (int64_t)ts[(size_t)k].lo.size();
+  const int64_t V = voff[(size_t)n];
+  std::vector<double> xyz;
+  if (V >= kGpuConvertMin) {
+    std::vector<double> lo((size_t)V), la((size_t)V);
+    for (int k = 0; k < n; k++) {
+      std::copy(ts[(size_t)k].lo.begin(), ts[(size_t)k].lo.end(), lo.begin() + voff[(size_t)k]);
+      std::copy(ts[(size_t)k].la.begin(), ts[(size_t)k].la.end(), la.begin() + voff[(size_t)k]);
     }
+    xyz.resize((size_t)(3 * V));
+    RCK(lonlat_to_xyz_gpu(c, lo.data(), la.data(), V, xyz.data()));
+  }
+  parallel([&](int k) {
+    const double* x = xyz.empty() ? nullptr : xyz.data() + 3 * voff[(size_t)k];
+    rcs[(size_t)k] = polygon_from_text(ts[(size_t)k], x, &ps[(size_t)k], &errs[(size_t)k]);
+  });
+  RCK(first_error());
   c->polys = std::move(ps);
   return upload_polygons(c);
 }

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "gahhg_internal.h"
+#include "gahhg_decimal.h"
 #include "gahhg_trig.h"
 
 #pragma clang fp contract(off)
@@ -63,11 +64,6 @@ static bool ring_is_clockwise(const double* lo, const double* la, int n) {
   V3 nrm = unit(cross(a0, sub(a1, a0)));
   return !(dot(nrm, b) > 0);
 }
-
-struct Ring {
-  int64_t start;
-  int32_t n;  // unique vertices
-};
 
 // xyz, when given, holds every vertex's point (parallel to lo/la), as the
 // device loader computed it with the same conversion.
@@ -143,85 +139,125 @@ int polygon_from_lonlat_xyz(const double* lon, const double* lat, const double* 
 }
 
 namespace {
+// Lines of a whole file read at once: a line ends at '\n' (not stored); a
+// last line without one counts when it is not empty.
 struct LineReader {
-  FILE* f;
-  std::string line;
+  std::string buf;
+  size_t pos = 0;
+  const char* line = nullptr;
+  size_t len = 0;
   int64_t lineno = 0;
+  bool open(const char* path) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    char tmp[1 << 16];
+    size_t got;
+    while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) buf.append(tmp, got);
+    fclose(f);
+    return true;
+  }
   bool next() {
-    line.clear();
-    int ch;
-    bool any = false;
-    while ((ch = fgetc(f)) != EOF) {
-      any = true;
-      if (ch == '\n') break;
-      line.push_back((char)ch);
-    }
-    if (!any) return false;
+    if (pos >= buf.size()) return false;
+    const char* b = buf.data() + pos;
+    const void* nl = memchr(b, '\n', buf.size() - pos);
+    len = nl ? (size_t)((const char*)nl - b) : buf.size() - pos;
+    line = b;
+    pos += len + (nl ? 1 : 0);
     lineno++;
     return true;
   }
+  std::string str() const { return std::string(line, len); }
 };
+
+// "%lf %lf" as sscanf reads it: the correctly rounded decimal fast path
+// (gahhg_decimal.h, equal to strtod on its grammar), sscanf for the rest.
+bool two_doubles(const LineReader& rd, double* x, double* y) {
+  const uint8_t* b = (const uint8_t*)rd.line;
+  const int64_t e = (int64_t)rd.len;
+  int64_t i = dec::skip_space(b, 0, e);
+  if (i < e && dec::parse_decimal(b, i, e, *x)) {
+    i = dec::skip_space(b, i, e);
+    if (i < e && dec::parse_decimal(b, i, e, *y)) return true;
+  }
+  return sscanf(rd.str().c_str(), "%lf %lf", x, y) == 2;
+}
 }  // namespace
 
-int load_polygon_txt(const char* path, Polygon* out, std::string* err) {
-  FILE* f = fopen(path, "rb");
-  if (!f) {
+int parse_polygon_txt(const char* path, PolygonText* out, std::string* err) {
+  LineReader rd;
+  if (!rd.open(path)) {
     *err = std::string("Can not open ") + path;
     return GAHHG_E_IO;
   }
-  LineReader rd{f};
   auto fail = [&](const char* what) {
-    fclose(f);
     *err = std::string(path) + ": line " + std::to_string(rd.lineno) + ": " + what;
     return GAHHG_E_FORMAT;
   };
   int holes = 0;
-  if (!rd.next() || sscanf(rd.line.c_str(), "POLYGON 1 %d", &holes) != 1 || holes < 0)
+  if (!rd.next() || sscanf(rd.str().c_str(), "POLYGON 1 %d", &holes) != 1 || holes < 0)
     return fail("wrong format");
-  std::vector<double> lo, la;
-  std::vector<Ring> rings;
+  std::vector<double>& lo = out->lo;
+  std::vector<double>& la = out->la;
+  lo.clear();
+  la.clear();
+  out->rings.clear();
   for (int r = 0; r <= holes; r++) {
     int cnt = 0, tmp = 0;
     bool ok = rd.next();
     if (r == 0)
-      ok = ok && sscanf(rd.line.c_str(), "0 %d", &cnt) == 1;
+      ok = ok && sscanf(rd.str().c_str(), "0 %d", &cnt) == 1;
     else
-      ok = ok && sscanf(rd.line.c_str(), "%d %d", &tmp, &cnt) == 2;
+      ok = ok && sscanf(rd.str().c_str(), "%d %d", &tmp, &cnt) == 2;
     if (!ok) return fail("wrong format");
     if (cnt < 4) return fail("number of edges should be greater than 3");
-    rings.push_back(Ring{(int64_t)lo.size(), cnt - 1});
+    out->rings.push_back(Ring{(int64_t)lo.size(), cnt - 1});
     for (int j = 0; j < cnt; j++) {
       double x, y;
-      if (!rd.next() || sscanf(rd.line.c_str(), "%lf %lf", &x, &y) != 2) return fail("wrong format");
+      if (!rd.next() || !two_doubles(rd, &x, &y)) return fail("wrong format");
       if (j + 1 < cnt) {  // the closing duplicate is not stored
         lo.push_back(x);
         la.push_back(y);
       }
     }
   }
-  fclose(f);
-  return assemble(lo.data(), la.data(), nullptr, rings, out, err);
+  return GAHHG_OK;
+}
+
+int polygon_from_text(const PolygonText& t, const double* xyz, Polygon* out, std::string* err) {
+  return assemble(t.lo.data(), t.la.data(), xyz, t.rings, out, err);
+}
+
+int load_polygon_txt(const char* path, Polygon* out, std::string* err) {
+  PolygonText t;
+  const int rc = parse_polygon_txt(path, &t, err);
+  return rc ? rc : polygon_from_text(t, nullptr, out, err);
 }
 
 int read_query_csv(const char* path, std::vector<double>* lon, std::vector<double>* lat,
                    std::string* err) {
-  FILE* f = fopen(path, "rb");
-  if (!f) {
+  LineReader rd;
+  if (!rd.open(path)) {
     *err = std::string("Can not open ") + path;
     return GAHHG_E_IO;
   }
-  LineReader rd{f};
   while (rd.next()) {
     double x, y;
-    if (sscanf(rd.line.c_str(), "%lf,%lf", &x, &y) != 2) {
-      fclose(f);
+    // "%lf,%lf": the decimal fast path, sscanf for what it leaves
+    const uint8_t* b = (const uint8_t*)rd.line;
+    const int64_t e = (int64_t)rd.len;
+    int64_t i = dec::skip_space(b, 0, e);
+    bool ok = i < e && dec::parse_decimal(b, i, e, x) && i < e && b[i] == ',';
+    if (ok) {
+      i = dec::skip_space(b, i + 1, e);
+      ok = i < e && dec::parse_decimal(b, i, e, y);
+    }
+    if (!ok && sscanf(rd.str().c_str(), "%lf,%lf", &x, &y) != 2) {
       *err = std::string(path) + ": line " + std::to_string(rd.lineno) + ": wrong format";
       return GAHHG_E_FORMAT;
     }
     lon->push_back(x);
     lat->push_back(y);
   }
-  fclose(f);
   return GAHHG_OK;
 }
 

@@ -342,8 +342,24 @@ struct Polygon {
   std::vector<int32_t> pe; // 2N
 };
 
+// A ring of a parsed polygon file: unique vertices [start, start + n) of
+// the file's lon/lat arrays (the closing duplicate is not stored).
+struct Ring {
+  int64_t start;
+  int32_t n;
+};
+
+// The text of a polygon file, checked as the reference's loader checks it.
+struct PolygonText {
+  std::vector<double> lo, la;
+  std::vector<Ring> rings;
+};
+
 // ---- host (gahhg_host.cpp) ----
 int load_polygon_txt(const char* path, Polygon* out, std::string* err);
+int parse_polygon_txt(const char* path, PolygonText* out, std::string* err);
+// Polygon from parsed text; xyz (parallel to lo/la) or null = convert here.
+int polygon_from_text(const PolygonText& t, const double* xyz, Polygon* out, std::string* err);
 int polygon_from_lonlat(const double* lon, const double* lat, const int64_t* ring_off,
                         int32_t n_rings, Polygon* out, std::string* err);
 // As polygon_from_lonlat with each vertex's point already converted (xyz
@@ -381,6 +397,11 @@ struct TextLoad {
 int load_query_csv_device(gahhg_ctx* c, const char* path, int64_t* n, gahhg_load_info* info);
 int parse_query_csv_device(gahhg_ctx* c, const char* text, int64_t bytes, int64_t* n, gahhg_load_info* info);
 int load_polygon_txt_device(gahhg_ctx* c, const char* path, Polygon* out, gahhg_load_info* info);
+// LLVertex::LL2XYZ of n host points on the device (the same bits as
+// lonlat_to_xyz), result in host xyz.
+int lonlat_to_xyz_gpu(gahhg_ctx* c, const double* lon, const double* lat, int64_t n, double* xyz);
+// Vertex counts from which the TXT loaders convert on the device.
+constexpr int64_t kGpuConvertMin = 16384;
 void free_text_load(gahhg_ctx* c);
 }  // namespace gahhg
 

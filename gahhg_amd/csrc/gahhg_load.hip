@@ -190,6 +190,14 @@ __global__ void __launch_bounds__(kLoadThreads) k_parse_vertices(const uint8_t* 
   }
 }
 
+// LLVertex::LL2XYZ of n points (lon, lat interleaved per point in ll).
+__global__ void __launch_bounds__(kLoadThreads) k_ll_to_xyz(const double* __restrict__ ll, int64_t n,
+                                                            double* __restrict__ xyz) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  store3(xyz + 3 * j, ll_to_xyz_dev(ll[2 * j], ll[2 * j + 1]));
+}
+
 // Byte ranges of listed lines (index = first + flags[k]).
 __global__ void k_flag_ranges(const int32_t* __restrict__ flags, int n, const int64_t* __restrict__ starts,
                               int64_t nl, int64_t bytes, int64_t first, int64_t* __restrict__ out) {
@@ -606,6 +614,27 @@ int load_polygon_txt_device(gahhg_ctx* c, const char* path, Polygon* out, gahhg_
     info->total_ms = now_ms() - t0;
   }
   return GAHHG_OK;
+}
+
+int lonlat_to_xyz_gpu(gahhg_ctx* c, const double* lon, const double* lat, int64_t n, double* xyz) {
+  if (n <= 0) return GAHHG_OK;
+  std::vector<double> ll((size_t)(2 * n));
+  for (int64_t j = 0; j < n; j++) {
+    ll[(size_t)(2 * j)] = lon[j];
+    ll[(size_t)(2 * j + 1)] = lat[j];
+  }
+  double* d = nullptr;
+  if (dalloc(c, (void**)&d, n * 40)) return GAHHG_E_NOMEM;
+  int rc = hip_check(c, hipMemcpyAsync(d, ll.data(), n * 16, hipMemcpyHostToDevice, c->stream), "vertex upload");
+  if (!rc) {
+    hipLaunchKernelGGL(k_ll_to_xyz, dim3((unsigned)((n + kLoadThreads - 1) / kLoadThreads)), dim3(kLoadThreads), 0,
+                       c->stream, (const double*)d, n, d + 2 * n);
+    rc = hip_check(c, hipGetLastError(), "vertex conversion");
+  }
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(xyz, d + 2 * n, n * 24, hipMemcpyDeviceToHost, c->stream), "vertex D2H");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "vertex conversion");
+  dfree(d);
+  return rc;
 }
 
 void free_text_load(gahhg_ctx* c) {
