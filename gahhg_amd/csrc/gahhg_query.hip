@@ -269,20 +269,25 @@ __device__ __forceinline__ int bin_of(V3 p, int B, double& fx, double& fy) {
 }
 
 // Conservative state of the reference's inside test (n.p >= -eps for every
-// border, in border order) over a cap (centre u, angular radius rho):
-// 0 = fails for every point, 1 = passes for every point, 2 = undecided.
-// The bounds carry a 1e-10 margin, far above the rounding of n.p and of the
-// trigonometric bound itself.
-__device__ int cell_state_cap(const GridView& g, int cid, V3 u, double rho) {
+// border, in border order) over a cap (centre u, angular radius rho, given
+// as cr = cos rho, sr = sin rho): 0 = fails for every point, 1 = passes for
+// every point, 2 = undecided.  With theta the angle between n and u, n.p
+// over the cap spans [|n| cos(min(theta + rho, pi)), |n| cos(max(theta -
+// rho, 0))]; the cosines come from cos theta and sin theta by the addition
+// formulas (no inverse or direct trigonometric call per border).  Near the
+// decision (n.p near 0, theta near pi/2 -+ rho) every term is well
+// conditioned, so the bounds are good to ~1e-15; they carry a 1e-10 margin.
+__device__ int cell_state_cap(const GridView& g, int cid, V3 u, double cr, double sr) {
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
   bool all_in = true;
   for (int i = 0; i < bn; i++) {
     const V3 n = load3(g.b_n + 3 * (int64_t)(bs + i));
     const double nn = sqrt(dot(n, n));
     const double ct = fmin(1.0, fmax(-1.0, dot(n, u) / nn));
-    const double th = acos(ct);
-    const double lo = nn * cos(fmin(th + rho, M_PI)) - 1e-10;
-    const double hi = nn * cos(fmax(th - rho, 0.0)) + 1e-10;
+    const double st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+    // theta + rho <= pi  <=>  cos theta >= -cos rho;  theta >= rho  <=>  cos theta <= cos rho
+    const double lo = (ct >= -cr ? nn * (ct * cr - st * sr) : -nn) - 1e-10;
+    const double hi = (ct <= cr ? nn * (ct * cr + st * sr) : nn) + 1e-10;
     if (hi < -kEps) return 0;  // this border rejects every point of the cap
     if (!(lo >= -kEps)) all_in = false;
   }
@@ -313,10 +318,11 @@ __device__ void bin_cap(int f, int B, int i, int j, int S, int si, int sj, V3& c
 // cell0 .. cell0 + 41).  Codes: >= 0 start cell, -1 start with the level-0
 // scan, -(2 + label) resolved leaf without refs.
 __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, double rho) {
+  const double cr = cos(rho), sr = sin(rho);
   int cur = from;
   if (cur < 0) {
     for (int c = 0; c < kGrid0Cells; c++) {
-      const int s = cell_state_cap(g, cell0 + c, cen, rho);
+      const int s = cell_state_cap(g, cell0 + c, cen, cr, sr);
       if (s == 0) continue;
       if (s == 1) cur = cell0 + c;
       break;  // certain first match, or undecided: stop either way
@@ -329,7 +335,7 @@ __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, dou
     if (sc == -1) break;
     int next = -1;
     bool stop = false;
-    int s = cell_state_cap(g, sc, cen, rho);
+    int s = cell_state_cap(g, sc, cen, cr, sr);
     if (s == 1) {
       next = sc;
     } else if (s == 2) {
@@ -339,7 +345,7 @@ __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, dou
       for (int k = 0; k < bn; k++) {
         const int tc = g.b_subcell[bs + k];
         if (tc == -1) continue;
-        s = cell_state_cap(g, tc, cen, rho);
+        s = cell_state_cap(g, tc, cen, cr, sr);
         if (s == 0) continue;
         if (s == 1) next = tc;
         else stop = true;
