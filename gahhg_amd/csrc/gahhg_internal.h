@@ -263,7 +263,7 @@ struct QueryLayout {
   int64_t bytes = 0;
   // allocations behind the pointers above (a pointer is null when its part
   // of the layout is not built)
-  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart;
+  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart, b_l0;
   int32_t* tstart = nullptr;  // per polygon: first table entry (np + 1 values, ascending)
 };
 

@@ -317,19 +317,32 @@ __device__ void bin_cap(int f, int B, int i, int j, int S, int si, int sj, V3& c
 // starting from cell `from` (-1: the level-0 scan of the polygon's cells
 // cell0 .. cell0 + 41).  Codes: >= 0 start cell, -1 start with the level-0
 // scan, -(2 + label) resolved leaf without refs.
+// Level-0 part of cap_entry: the level-0 cell (0..41, relative to cell0)
+// that the scan certainly matches first for every point of the cap, or -1.
+__device__ int cap_level0(const GridView& g, int cell0, V3 cen, double cr, double sr) {
+  for (int c = 0; c < kGrid0Cells; c++) {
+    const int s = cell_state_cap(g, cell0 + c, cen, cr, sr);
+    if (s == 0) continue;
+    return s == 1 ? c : -1;  // certain first match, or undecided: stop either way
+  }
+  return -1;
+}
+
+__device__ int32_t cap_descend(const GridView& g, int cur, V3 cen, double cr, double sr);
+
 __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, double rho) {
   const double cr = cos(rho), sr = sin(rho);
   int cur = from;
   if (cur < 0) {
-    for (int c = 0; c < kGrid0Cells; c++) {
-      const int s = cell_state_cap(g, cell0 + c, cen, cr, sr);
-      if (s == 0) continue;
-      if (s == 1) cur = cell0 + c;
-      break;  // certain first match, or undecided: stop either way
-    }
-    if (cur < 0) return -1;
+    const int c = cap_level0(g, cell0, cen, cr, sr);
+    if (c < 0) return -1;
+    cur = cell0 + c;
   }
-  // descend while the first-match child is certain
+  return cap_descend(g, cur, cen, cr, sr);
+}
+
+// From cell cur: descend while the first-match child is certain.
+__device__ int32_t cap_descend(const GridView& g, int cur, V3 cen, double cr, double sr) {
   for (;;) {
     const int sc = g.c_subcell[cur];
     if (sc == -1) break;
@@ -359,11 +372,38 @@ __device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, dou
   return cur;
 }
 
-// Level-1 entries: one per bin of every polygon's cube map.
+// First entry of the level-0 table of B bins per side (B = 16 * 2^k): the
+// tables of 16, 32, ... bins follow one another.
+__host__ __device__ __forceinline__ int64_t l0_offset(int B) {
+  int64_t off = 0;
+  for (int b = 16; b < B; b *= 2) off += 6LL * b * b;
+  return off;
+}
+
+// Level-0 part of every bin's entry for 16, 32, ..., Bmax bins per side.
+// Level-0 geometry is the same for every polygon (polygon 0's cells are
+// used), so a batch computes it once per resolution, not once per polygon.
+__global__ void k_level0_table(GridView g, int cell0, int Bmax, int32_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= l0_offset(2 * Bmax)) return;
+  int B = 16;
+  while (t >= l0_offset(2 * B)) B *= 2;
+  const int64_t u = t - l0_offset(B);
+  const int f = (int)(u / ((int64_t)B * B));
+  const int i = (int)((u / B) % B);
+  const int j = (int)(u % B);
+  V3 cen;
+  double rho;
+  bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
+  out[t] = cap_level0(g, cell0, cen, cos(rho), sin(rho));
+}
+
 // Entry tt of all polygons' tables: the polygon owning it is found by a
 // binary search over the tables' first entries (tstart, np + 1 values).
+// With l0 (a batch), the level-0 part comes from k_level0_table.
 __global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly,
-                              const int32_t* __restrict__ tstart, int32_t* __restrict__ table_all) {
+                              const int32_t* __restrict__ tstart, const int32_t* __restrict__ l0,
+                              int32_t* __restrict__ table_all) {
   const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tt >= tstart[npoly]) return;
   int lo = 0, hi = npoly - 1;  // last polygon with tstart <= tt
@@ -381,7 +421,12 @@ __global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int np
   V3 cen;
   double rho;
   bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
-  table_all[tt] = cap_entry(g, pd[poly].cell0, -1, cen, rho);
+  if (l0) {
+    const int c = l0[l0_offset(B) + t];
+    table_all[tt] = c < 0 ? -1 : cap_descend(g, pd[poly].cell0 + c, cen, cos(rho), sin(rho));
+  } else {
+    table_all[tt] = cap_entry(g, pd[poly].cell0, -1, cen, rho);
+  }
 }
 
 // A bin gets a sub-table when its entry is not final: the level-0 scan or a
@@ -1257,8 +1302,26 @@ int pack_query_layout(gahhg_ctx* c) {
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "table starts");  // ts leaves scope
     if (rc) return rc;
     const int64_t per = 6LL * q.bins * q.bins;
+    // a batch whose tables have 16, 32, ... bins per side: level-0 parts
+    // computed once per resolution and shared by the polygons
+    int32_t* l0 = nullptr;
+    int Bmax = 0;
+    bool pow2 = np > 1;
+    for (int p = 0; p < np && pow2; p++) {
+      const int B = c->pd[p].tbins;
+      pow2 = B >= 16 && B <= 4096 && (B & (B - 1)) == 0;
+      Bmax = std::max(Bmax, B);
+    }
+    if (pow2) {
+      const int64_t nl0 = l0_offset(2 * Bmax);
+      if (use_buf(c, q.b_l0, nl0 * 4, (void**)&l0)) return GAHHG_E_NOMEM;
+      hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((nl0 + 255) / 256)), dim3(256), 0, c->stream, g,
+                         c->pd[0].cell0, Bmax, l0);
+      rc = hip_check(c, hipGetLastError(), "level-0 table");
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
-                       (const PolyDev*)c->d_pd, np, (const int32_t*)q.tstart, q.table);
+                       (const PolyDev*)c->d_pd, np, (const int32_t*)q.tstart, (const int32_t*)l0, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
     // sub-tables for one polygon only: on the 256-polygon batch they cost
     // 11.5 ms of build for 0.9 ms less per 64M points

@@ -314,12 +314,13 @@ def test_batch_matches_single_polygon_builds():
     gs.set_polygons_lonlat(items).build()
     assert (gs.query_batch_xyz(xyz, poly) == got).all()
     gs.close()
-    # start tables of one size for all polygons (budget 0) and of sizes
-    # between the default's and the full one (budget 40)
-    for budget in (0, 40):
-        gt = gahhg_amd.GPISP(device=0, query_table_budget=budget)
+    # start tables of one size for all polygons (budget 0), of sizes
+    # between the default's and the full one (budget 40), and of sizes that
+    # are not powers of two (no shared level-0 parts)
+    for budget, bins in ((0, 256), (40, 256), (8, 100)):
+        gt = gahhg_amd.GPISP(device=0, query_table_budget=budget, query_bins=bins)
         gt.set_polygons_lonlat(items).build()
-        assert (gt.query_batch_xyz(xyz, poly) == got).all(), budget
+        assert (gt.query_batch_xyz(xyz, poly) == got).all(), (budget, bins)
         gt.close()
     g.close()
     gp.close()
