@@ -80,7 +80,7 @@ EXPORTED = [
     "gahhg_mlp_adam", "gahhg_set_depth_model", "gahhg_set_query_hint", "gahhg_depth_features",
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
-    "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz",
+    "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz", "gahhg_autotune_layout",
 ]
 
 # Query trace record fields (6 int32 per point) and build trace layout.
@@ -139,6 +139,7 @@ def lib():
         "gahhg_query_batch_xyz": ([P, dp, ip, i64, ip], ctypes.c_int),
         "gahhg_query_batch_xyz_device": ([P, P, P, i64, P], ctypes.c_int),
         "gahhg_query_pairs_xyz": ([P, dp, i64, ip], ctypes.c_int),
+        "gahhg_autotune_layout": ([P, P, P, i64, P, ctypes.c_double, P], ctypes.c_int),
         "gahhg_get_polygon_stats": ([P, i32, ctypes.POINTER(Stats)], ctypes.c_int),
         "gahhg_polygon_grid_counts": ([P, i32, lp], ctypes.c_int),
         "gahhg_export_polygon_grid": ([P, i32, P, P, P, P, P, P], ctypes.c_int),
@@ -491,6 +492,18 @@ class GPISP:
         self._check(self.L.gahhg_autotune_query(self.h, d_xyz.ptr, d_poly.ptr if d_poly is not None else None,
                                                 int(n), d_labels.ptr, out.ctypes.data))
         return int(out[0]), int(out[1]), int(out[2])
+
+    def autotune_layout_device(self, d_xyz, d_poly, n, d_labels, passes=1.0):
+        """Re-packs the query layout for each candidate start-table / sub-
+        table setting, keeps the one with the least layout ms / passes +
+        query ms; returns a dict of the chosen settings and their times."""
+        out = np.zeros(6, dtype=np.float64)
+        self._check(self.L.gahhg_autotune_layout(self.h, d_xyz.ptr, d_poly.ptr if d_poly is not None else None,
+                                                 int(n), d_labels.ptr, float(passes), out.ctypes.data))
+        keys = ("query_bins", "query_sub", "query_sub_budget", "query_table_budget")
+        res = {k: int(v) for k, v in zip(keys, out[:4])}
+        res.update(layout_ms=float(out[4]), query_ms=float(out[5]))
+        return res
 
     def set_depth_model(self, params, cost):
         """Learned refinement depth (params: the model's 5313 values; cost:

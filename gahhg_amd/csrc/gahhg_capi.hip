@@ -804,6 +804,74 @@ int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
   return GAHHG_OK;
 }
 
+int gahhg_autotune_layout(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                          double passes, double* out6) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "autotune before build");
+  if (n <= 0 || !d_xyz || !d_labels || !(passes >= 1.0)) return set_err(c, GAHHG_E_ARG, "autotune: bad arguments");
+  if (!d_poly && c->polys.size() != 1)
+    return set_err(c, GAHHG_E_ARG, "autotune: polygon ids required for a batch of %d", (int)c->polys.size());
+  if (c->prm.query_layout != 0) return set_err(c, GAHHG_E_STATE, "autotune: the accelerated query is off");
+  RCK(select_device(c));
+  RCK(check_poly_ids(c, d_poly, n));
+  struct Cand {
+    int bins, sub, sub_budget, table_budget;
+  };
+  std::vector<Cand> cands;
+  const gahhg_params keep = c->prm;
+  if (c->polys.size() == 1) {
+    cands = {{128, 1, 4, keep.query_table_budget},
+             {256, 1, 4, keep.query_table_budget},
+             {256, 8, 4, keep.query_table_budget},
+             {256, 16, 8, keep.query_table_budget}};
+  } else {
+    cands = {{keep.query_bins, keep.query_sub, keep.query_sub_budget, 8},
+             {keep.query_bins, keep.query_sub, keep.query_sub_budget, 32},
+             {keep.query_bins, keep.query_sub, keep.query_sub_budget, 0}};
+  }
+  int best = -1;
+  double best_cost = 0.0, best_pack = 0.0, best_q = 0.0;
+  auto pack = [&](const Cand& k, float* ms) {
+    c->prm.query_bins = k.bins;
+    c->prm.query_sub = k.sub;
+    c->prm.query_sub_budget = k.sub_budget;
+    c->prm.query_table_budget = k.table_budget;
+    int rc = hip_check(c, hipEventRecord(c->ev[0], c->stream), "event");
+    if (!rc) rc = pack_query_layout(c);
+    if (!rc) rc = hip_check(c, hipEventRecord(c->ev[1], c->stream), "event");
+    if (!rc) rc = hip_check(c, hipEventSynchronize(c->ev[1]), "autotune sync");
+    if (!rc && ms) (void)hipEventElapsedTime(ms, c->ev[0], c->ev[1]);
+    return rc;
+  };
+  for (size_t i = 0; i < cands.size(); i++) {
+    float pms = 0.f;
+    int rc = pack(cands[i], &pms);
+    float qms = 0.f;
+    for (int rep = 0; rep < 2 && !rc; rep++) {  // the first run warms up
+      rc = hip_check(c, hipEventRecord(c->ev[0], c->stream), "event");
+      if (!rc) rc = run_query(c, d_xyz, d_poly, n, d_labels);
+      if (!rc) rc = hip_check(c, hipEventRecord(c->ev[1], c->stream), "event");
+      if (!rc) rc = hip_check(c, hipEventSynchronize(c->ev[1]), "autotune sync");
+      if (!rc) (void)hipEventElapsedTime(&qms, c->ev[0], c->ev[1]);
+    }
+    if (rc) {  // restore the context's own layout
+      c->prm = keep;
+      (void)pack_query_layout(c);
+      return rc;
+    }
+    const double cost = pms / passes + qms;
+    if (best < 0 || cost < best_cost) best = (int)i, best_cost = cost, best_pack = pms, best_q = qms;
+  }
+  RCK(pack(cands[(size_t)best], nullptr));
+  if (out6) {
+    const Cand& k = cands[(size_t)best];
+    out6[0] = k.bins, out6[1] = k.sub, out6[2] = k.sub_budget, out6[3] = k.table_budget;
+    out6[4] = best_pack, out6[5] = best_q;
+  }
+  RCK(run_query(c, d_xyz, d_poly, n, d_labels));
+  return hip_check(c, hipStreamSynchronize(c->stream), "autotune sync");
+}
+
 int gahhg_trace_query_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n,
                              int32_t* d_labels, int32_t* d_trace) {
   RCK(ctx_ok(c));

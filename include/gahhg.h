@@ -241,6 +241,18 @@ int gahhg_set_polygon_depths(gahhg_ctx* ctx, const int32_t* depths, int32_t n);
  * receives the labels (the same for every setting). */
 int gahhg_autotune_query(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                          int32_t* out3);
+/* Query-layout autotuner (no counterpart in the reference): re-packs the
+ * built hierarchy's query layout for each candidate (one polygon: start
+ * table of 128 / 256 bins per side with no sub-tables, sub-tables of 8 at
+ * 4 entries per edge, or 16 at 8; a batch: start-table budget 8, 32 or 0
+ * entries per edge), times the layout build and the query on the given
+ * device points, keeps the candidate with the least layout ms / passes +
+ * query ms (passes >= 1: query passes expected per build) and writes
+ * {query_bins, query_sub, query_sub_budget, query_table_budget, layout ms,
+ * query ms} of it to out6 (may be NULL).  Labels are the same for every
+ * candidate; d_labels receives them. */
+int gahhg_autotune_layout(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                          double passes, double* out6);
 
 /* Trace recorder (no counterpart in the reference).  Query trace: the
  * accelerated query of gahhg_query_batch_xyz_device (same labels) that also

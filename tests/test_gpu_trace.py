@@ -158,6 +158,49 @@ def test_autotune_keeps_labels():
     g.close()
 
 
+def test_autotune_layout_keeps_labels():
+    """The layout autotuner re-packs the query layout per candidate; labels
+    stay the oracle's for every candidate and after the choice."""
+    say("layout autotune: fractal polygon and a 5-polygon batch")
+    w = datasets.fractal_100k(n_queries=400_000, n_outer=20_000)
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    n = len(xyz)
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    ref = g.query_xyz(xyz)
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    assert np.count_nonzero(ref != o.query_xyz(xyz)) == 0
+    dx, dl = g.alloc(xyz.nbytes), g.alloc(4 * n)
+    dx.upload(xyz)
+    for passes in (1.0, 1000.0):
+        r = g.autotune_layout_device(dx, None, n, dl, passes=passes)
+        assert r["query_bins"] in (128, 256) and r["query_sub"] in (1, 8, 16) and r["query_ms"] > 0
+        assert (dl.download(np.empty(n, dtype=np.int32)) == ref).all()
+        assert (g.query_xyz(xyz) == ref).all()
+    for b in (dx, dl):
+        b.free()
+    g.close()
+    ps = datasets.batch_polygons(n_poly=5, seed=31)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    g = gahhg_amd.GPISP(device=0)
+    g.set_polygons_lonlat(items).build()
+    rng = np.random.default_rng(8)
+    lon, lat = datasets.uniform_queries(rng, 300_000)
+    xyz = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    poly = rng.integers(0, 5, len(lon)).astype(np.int32)
+    ref = g.query_batch_xyz(xyz, poly)
+    dx, dp, dl = g.alloc(xyz.nbytes), g.alloc(4 * len(poly)), g.alloc(4 * len(poly))
+    dx.upload(xyz)
+    dp.upload(poly)
+    r = g.autotune_layout_device(dx, dp, len(poly), dl, passes=10.0)
+    assert r["query_table_budget"] in (0, 8, 32)
+    assert (dl.download(np.empty(len(poly), dtype=np.int32)) == ref).all()
+    assert (g.query_batch_xyz(xyz, poly) == ref).all()
+    for b in (dx, dp, dl):
+        b.free()
+    g.close()
+
+
 def test_grid_save_load(tmp_path):
     """A saved hierarchy loads into a fresh context with identical arrays,
     statistics and labels (single polygon and batch)."""
