@@ -69,6 +69,29 @@ def test_read_query_csv_matches_oracle(L, tmp_path):
         gahhg_amd.read_query_csv(tmp_path / "missing.csv")
 
 
+def test_read_query_csv_edge_lines_match_oracle(L, tmp_path):
+    """The reader's decimal fast path and its sscanf fallback give what the
+    oracle's sscanf("%lf,%lf") gives: signs, exponents, more than 19
+    significant digits, hex, inf/nan, blanks and CRLF, trailing text."""
+    lines = ["1,2", " -0.5,+3e2", "12.5e-3, 7", "1.2345678901234567890123456789,-0.000000000000000000000001",
+             "0x1p3,5", "inf,-inf", "45.000000000000000000001,1", "3,4 trailing", "7,8\r", "1e308,-1e-300",
+             "0.1,0.2", "-0,0"]
+    p = tmp_path / "edge.csv"
+    p.write_bytes(("\n".join(lines) + "\n").encode())
+    a = gahhg_amd.read_query_csv(p)
+    b = ol.load_query_csv(str(p))
+    assert len(a[0]) == len(lines) == len(b[0])
+    for x, y in ((a[0], b[0]), (a[1], b[1])):
+        assert (np.isnan(x) == np.isnan(y)).all()
+        ok = ~np.isnan(x)
+        assert (x[ok].view(np.int64) == y[ok].view(np.int64)).all()
+    for bad_line in ("1.5 ,2", "1,", ",2", "abc,1", ""):
+        q = tmp_path / "bad_edge.csv"
+        q.write_bytes(("0,0\n" + bad_line + "\n1,1\n").encode())
+        with pytest.raises(gahhg_amd.GahhgError):
+            gahhg_amd.read_query_csv(q)
+
+
 def test_write_result_csv(L, tmp_path):
     p = tmp_path / "r.csv"
     gahhg_amd.write_result_csv(p, np.array([2, 1, 0, 2], dtype=np.int32))
