@@ -265,7 +265,7 @@ class GPISP:
 
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
                  query_layout=0, query_bins=256, query_group=0, query_sub=8, query_sub_budget=4,
-                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=0, query_table_budget=8,
+                 query_sort=1, query_sort_bits=14, query_index=1, query_xcd=32, query_table_budget=8,
                  query_graph=0):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,

@@ -263,7 +263,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_sort = 1;
   p->query_sort_bits = 14;
   p->query_index = 1;
-  p->query_xcd = 0;
+  p->query_xcd = 32;
   p->query_table_budget = 8;
   p->query_graph = 0;
   return GAHHG_OK;

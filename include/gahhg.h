@@ -79,7 +79,8 @@ typedef struct {
                               blocks of grouped points on one XCD at a time
                               (workgroups reach the 8 XCDs round-robin, each
                               with its own L2); 0 the plain block order;
-                              labels are unchanged                          */
+                              default 32 (fractal_1m's walk 7.63 -> 7.31
+                              ms); labels are unchanged                    */
   int32_t query_table_budget;/* batches: start-table entries allowed per
                               polygon edge (bins per face side a power of
                               two in [16, query_bins]; default 8: least
