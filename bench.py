@@ -63,7 +63,7 @@ def measured_traffic(depth, Q, prm):
     d = json.loads(m.group(1))
     same = (d.get("depth") == depth and d.get("query_sort_bits") == prm.query_sort_bits and d.get("table_pass")
             and d.get("leaf_index") == bool(prm.query_index) and d.get("query_sub") == prm.query_sub
-            and d.get("query_bins") == prm.query_bins)
+            and d.get("query_bins") == prm.query_bins and d.get("query_xcd", 0) == prm.query_xcd)
     return round(float(d["k_query_traffic_bytes"]), 0) if same else None
 
 
@@ -262,7 +262,8 @@ def main():
                 "frac": round(frac_hbm, 4), "traffic": None}
     roof["kernel"] = "k_query"
     roof["kernel_ms"] = round(kms, 4)
-    roof["traffic"] = measured_traffic(int(st["depth"]), Q, gahhg_amd.default_params(query_bins=args.bins))
+    roof["traffic"] = measured_traffic(int(st["depth"]), Q,
+                                       gahhg_amd.default_params(query_bins=args.bins, **query_kw(args)))
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
