@@ -43,7 +43,7 @@ void free_grid(gahhg_ctx* c) {
   g = DeviceGrid();
   QueryLayout& q = c->q;
   DevBuf* bufs[] = {&q.b_table, &q.b_sub, &q.b_tmp, &q.b_pcnt, &q.b_qc,   &q.b_hdr, &q.b_ib,
-                    &q.b_idx,   &q.b_sv,  &q.b_sn,  &q.b_sf,   &q.b_iscr, &q.b_tstart, &q.b_l0};
+                    &q.b_idx,   &q.b_sv,  &q.b_sn,  &q.b_sf,   &q.b_iscr, &q.b_tstart, &q.b_l0, &q.b_nodes};
   for (DevBuf* b : bufs) free_buf(*b);
   q = QueryLayout();
   drop_query_graph(c);
@@ -62,7 +62,7 @@ void reset_grid(gahhg_ctx* c) {
   kept.b_table = q.b_table, kept.b_sub = q.b_sub, kept.b_tmp = q.b_tmp, kept.b_pcnt = q.b_pcnt;
   kept.b_qc = q.b_qc, kept.b_hdr = q.b_hdr, kept.b_ib = q.b_ib, kept.b_idx = q.b_idx;
   kept.b_sv = q.b_sv, kept.b_sn = q.b_sn, kept.b_sf = q.b_sf, kept.b_iscr = q.b_iscr;
-  kept.b_tstart = q.b_tstart, kept.b_l0 = q.b_l0;
+  kept.b_tstart = q.b_tstart, kept.b_l0 = q.b_l0, kept.b_nodes = q.b_nodes;
   q = kept;
   c->qepoch++;
   c->built = false;

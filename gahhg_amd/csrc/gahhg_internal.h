@@ -263,9 +263,15 @@ struct QueryLayout {
   int64_t bytes = 0;
   // allocations behind the pointers above (a pointer is null when its part
   // of the layout is not built)
-  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart, b_l0;
+  DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart, b_l0, b_nodes;
   int32_t* tstart = nullptr;  // per polygon: first table entry (np + 1 values, ascending)
+  // per cell, kNodeInts int32 (192 B): the descent's view of the cell in one
+  // record -- [0] centre child (-1 = leaf), [1] borders | border children
+  // << 8 | type << 16 | (refs != 0) << 24, [2..7] border children in border
+  // order (-1 skipped), [10..45] its border normals (6 x 3 doubles)
+  int32_t* nodes = nullptr;
 };
+constexpr int kNodeInts = 48;
 
 // Learned refinement depth: model, cost constants, hint sample and the
 // per (polygon, top-level cell) device arrays of the build.

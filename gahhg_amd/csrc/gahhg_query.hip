@@ -80,6 +80,92 @@ __device__ __forceinline__ int descend(const GridView& g, V3 p, int cur, int& te
   return cur;
 }
 
+// The reference's inside test of cell x on its node record: every border
+// plane n.p >= -eps, in border order (the same normals and operation order
+// as q_inside).  On a pass, sub / meta receive x's centre child and meta.
+template <bool TRACE>
+__device__ __forceinline__ bool node_inside(const int32_t* __restrict__ nodes, int x, V3 p, int& sub, int& meta,
+                                            int& tests) {
+  const int32_t* r = nodes + (int64_t)x * kNodeInts;
+  const int2 h = *reinterpret_cast<const int2*>(r);
+  const int bn = h.y & 0xff;
+  const double* n = reinterpret_cast<const double*>(r + 10);
+  for (int i = 0; i < bn; i++) {
+    const double v = n[3 * i] * p.x + n[3 * i + 1] * p.y + n[3 * i + 2] * p.z;
+    if (v < -kEps) {
+      if (TRACE) tests += i + 1;
+      return false;
+    }
+  }
+  if (TRACE) tests += bn;
+  sub = h.x;
+  meta = h.y;
+  return true;
+}
+
+// descend() on the node records: one record per level holds the chosen
+// child's planes, its own centre child and border children, type and refs
+// flag.  Returns the leaf (-1 = none) and its meta word.
+template <bool TRACE>
+__device__ __forceinline__ int descend_nodes(const int32_t* __restrict__ nodes, V3 p, int cur, int& meta,
+                                             int& tests) {
+  const int2 h = *reinterpret_cast<const int2*>(nodes + (int64_t)cur * kNodeInts);
+  int sub = h.x;
+  meta = h.y;
+  while (sub != -1) {
+    int nsub = 0, nmeta = 0;
+    if (node_inside<TRACE>(nodes, sub, p, nsub, nmeta, tests)) {
+      cur = sub;
+      sub = nsub;
+      meta = nmeta;
+      continue;
+    }
+    const int32_t* r = nodes + (int64_t)cur * kNodeInts;
+    const int nch = (meta >> 8) & 0xff;
+    bool found = false;
+    for (int i = 0; i < nch; i++) {
+      const int tc = r[2 + i];
+      if (node_inside<TRACE>(nodes, tc, p, nsub, nmeta, tests)) {
+        cur = tc;
+        sub = nsub;
+        meta = nmeta;
+        found = true;
+        break;
+      }
+    }
+    if (!found) return -1;
+  }
+  return cur;
+}
+
+// Node record of every cell (layout at QueryLayout::nodes); a cell with
+// more than 6 borders (none in this hierarchy) is counted in bad.
+__global__ void k_pack_nodes(GridView g, int64_t C, int32_t* __restrict__ nodes, DevState* st) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  int32_t* r = nodes + c * kNodeInts;
+  const int bs = g.c_bstart[c], bn = g.c_bnum[c];
+  if (bn > 6) atomicAdd(&st->bad_index, 1ull);
+  const int b6 = min(bn, 6);
+  const int sc = g.c_subcell[c];
+  int nch = 0;
+  for (int i = 0; i < 8; i++) r[i] = -1;
+  if (sc != -1)
+    for (int i = 0; i < b6; i++) {
+      const int tc = g.b_subcell[bs + i];
+      if (tc != -1) r[2 + nch++] = tc;
+    }
+  r[0] = sc;
+  r[1] = b6 | (nch << 8) | ((g.c_type[c] & 0xff) << 16) | ((g.c_rnum[c] != 0 ? 1 : 0) << 24);
+  r[8] = r[9] = 0;
+  double* n = reinterpret_cast<double*>(r + 10);
+  for (int i = 0; i < 6; i++) {
+    const V3 v = i < b6 ? load3(g.b_n + 3 * (int64_t)(bs + i)) : v3(0.0, 0.0, 0.0);
+    n[3 * i] = v.x, n[3 * i + 1] = v.y, n[3 * i + 2] = v.z;
+  }
+  r[46] = r[47] = 0;
+}
+
 // Plain first-match descent from level 0 to the leaf holding p (-1 = none).
 __device__ int locate_leaf(const GridView& g, V3 p, int cell0) {
   int cur = -1;
@@ -940,7 +1026,8 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
                                                    const int32_t* __restrict__ order, int64_t n,
                                                    const int32_t* __restrict__ d_n,
                                                    int32_t* __restrict__ labels, LevelTab lt,
-                                                   int32_t* __restrict__ trace, int G_override, int xcd) {
+                                                   int32_t* __restrict__ trace, int G_override, int xcd,
+                                                   const int32_t* __restrict__ nodes) {
   // thread t handles point order[t] (points grouped by polygon and start
   // bin, so that a wave's gathers share cache lines), or point t; with xcd,
   // each XCD takes a contiguous run of t.  With d_n the count is *d_n (at
@@ -980,7 +1067,12 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
     }
     if (!done && cur < 0) {
       for (int i = 0; i < kGrid0Cells; i++) {
-        cur = TRACE ? q_inside_n(g, p, P.cell0 + i, tests) : q_inside(g, p, P.cell0 + i);
+        if (nodes) {
+          int sub0 = 0, meta0 = 0;
+          cur = node_inside<TRACE>(nodes, P.cell0 + i, p, sub0, meta0, tests) ? P.cell0 + i : -1;
+        } else {
+          cur = TRACE ? q_inside_n(g, p, P.cell0 + i, tests) : q_inside(g, p, P.cell0 + i);
+        }
         if (cur != -1) break;
       }
       if (cur == -1) {
@@ -989,15 +1081,26 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
       }
     }
     if (!done) {
-      cur = descend<TRACE>(g, p, cur, tests);
-      if (cur == -1) {
-        labels[q] = kUndefined;
+      int meta = 0;
+      bool refs = false;
+      if (nodes) {
+        cur = descend_nodes<TRACE>(nodes, p, cur, meta, tests);
+        type = (meta >> 16) & 0xff;
+        refs = (meta >> 24) != 0;
       } else {
-        type = g.c_type[cur];
-        if (g.c_rnum[cur] == 0)
-          labels[q] = type;
-        else
-          leaf = cur;
+        cur = descend<TRACE>(g, p, cur, tests);
+        if (cur != -1) {
+          type = g.c_type[cur];
+          refs = g.c_rnum[cur] != 0;
+        }
+      }
+      if (cur == -1) {
+        type = kUndefined;
+        labels[q] = kUndefined;
+      } else if (!refs) {
+        labels[q] = type;
+      } else {
+        leaf = cur;
       }
     }
     if (TRACE) {
@@ -1263,6 +1366,13 @@ int pack_query_layout(gahhg_ctx* c) {
                      (const double*)c->d_pv, (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
                      (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf, G);
   rc = hip_check(c, hipGetLastError(), "leaf write");
+  if (rc) return rc;
+  // node records for the descent (one 192-byte record per cell)
+  if (use_buf(c, q.b_nodes, nc * kNodeInts * 4, (void**)&q.nodes)) return GAHHG_E_NOMEM;
+  q.bytes += nc * kNodeInts * 4;
+  hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((c->tot_c + 255) / 256)), dim3(256), 0, c->stream, g,
+                     (int64_t)c->tot_c, q.nodes, c->d_st);
+  rc = hip_check(c, hipGetLastError(), "node records");
   if (rc) return rc;
   rc = build_leaf_index(c);
   if (rc) return rc;
@@ -1551,7 +1661,7 @@ static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
                      L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
                      (const int32_t*)L.idx, (const double*)L.sv,
                      (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_n, d_labels,
-                     level_tab(c), d_trace, c->prm.query_group, c->prm.query_xcd);
+                     level_tab(c), d_trace, c->prm.query_group, c->prm.query_xcd, (const int32_t*)L.nodes);
 }
 
 // The work of one accelerated query pass, enqueued with no host wait when
