@@ -970,16 +970,33 @@ __device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, i
                                            const uint8_t* __restrict__ sf, int32_t* __restrict__ labels) {
   const int gbase = lane & ~(G - 1);
   const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+  // each lane first gathers its own leaf's centre, stream range and index
+  // bucket (all lanes at once), so that the group's loop below only fetches
+  // index entries and records for each leaf in turn
+  V3 nq_own = v3(0.0, 0.0, 0.0);
+  int r0_own = 0, m_own = 0, e0_own = 0, e1_own = 0, base_own = -1;
+  if (leaf >= 0) {
+    const V3 c = load3(qc + 3 * (int64_t)leaf);
+    nq_own = arc_normal(c, p);
+    r0_own = hdr[2 * leaf];
+    m_own = hdr[2 * leaf + 1];
+    base_own = ib ? ib[leaf] : -1;
+    if (base_own >= 0) {
+      V3 u, v;
+      leaf_basis(c, u, v);
+      const int b = psi_bucket(nq_own, u, v);
+      e0_own = idx[base_own + b];
+      e1_own = idx[base_own + b + 1];
+    }
+  }
   unsigned long long mine = __ballot(leaf >= 0) & gmask;
   while (mine) {
     const int src = __ffsll((long long)mine) - 1;
     mine &= mine - 1;
-    const int lt = __shfl(leaf, src, 64);
     const V3 pp = v3(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64));
-    const V3 c = load3(qc + 3 * (int64_t)lt);
-    const V3 nq = arc_normal(c, pp);
-    const int r0 = hdr[2 * lt], m = hdr[2 * lt + 1];
-    const int base = ib ? ib[lt] : -1;
+    const V3 nq = v3(__shfl(nq_own.x, src, 64), __shfl(nq_own.y, src, 64), __shfl(nq_own.z, src, 64));
+    const int r0 = __shfl(r0_own, src, 64), m = __shfl(m_own, src, 64);
+    const int base = __shfl(base_own, src, 64);
     int cnt = 0;
     if (base < 0) {
       for (int b0 = 0; b0 < m - 1; b0 += G) {
@@ -988,12 +1005,8 @@ __device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, i
         cnt += __popcll(__ballot(cross) & gmask);
       }
     } else {
-      V3 u, v;
-      leaf_basis(c, u, v);
-      const int b = psi_bucket(nq, u, v);
-      const int* off = idx + base;
-      const int e0 = off[b], e1 = off[b + 1];
-      const int* ent = off + kIdxB + 1;
+      const int e0 = __shfl(e0_own, src, 64), e1 = __shfl(e1_own, src, 64);
+      const int* ent = idx + base + kIdxB + 1;
       for (int i0 = e0; i0 < e1; i0 += G) {
         const int i = i0 + (lane - gbase);
         const bool cross = i < e1 && record_cross(r0 + ent[i], nq, pp, sv, sn, sf);
