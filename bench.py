@@ -64,7 +64,7 @@ def measured_traffic(depth, Q, prm):
     same = (d.get("depth") == depth and d.get("query_sort_bits") == prm.query_sort_bits and d.get("table_pass")
             and d.get("leaf_index") == bool(prm.query_index) and d.get("query_sub") == prm.query_sub
             and d.get("query_bins") == prm.query_bins and d.get("query_xcd", 0) == prm.query_xcd
-            and d.get("descent") == "node records")
+            and d.get("descent") == "node records" and d.get("leaf_stage") == "per-lane prefetch")
     return round(float(d["k_query_traffic_bytes"]), 0) if same else None
 
 
