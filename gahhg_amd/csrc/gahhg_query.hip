@@ -1491,9 +1491,6 @@ static int ceil_log2(int64_t v) {
   return b;
 }
 
-// Groups the points by (polygon, start bin) with a device radix sort
-// (rocPRIM through hipCUB) over the key bits in use; *order receives the
-// point indices in key order.  Buffers are kept by the context and grow.
 // The grouping key of k_sort_key for one point.
 __device__ __forceinline__ uint32_t group_key(V3 p, uint32_t poly, int r) {
   double fx = 0.0, fy = 0.0;
