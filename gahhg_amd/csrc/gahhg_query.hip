@@ -1546,24 +1546,38 @@ __global__ void __launch_bounds__(256) k_query_table(const PolyDev* __restrict__
   if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// Second pass: the kept points, in point order, to [offs[block], ...).
-__global__ void __launch_bounds__(256) k_query_compact(const uint32_t* __restrict__ key, int64_t n,
+// Second pass: the kept points of table-pass block sb, in point order, to
+// [offs[sb], ...).  Each block takes 4 table-pass blocks at once (their
+// loads in flight together; 4x fewer blocks).
+constexpr int kCompactSub = 4;
+__global__ void __launch_bounds__(256) k_query_compact(const uint32_t* __restrict__ key, int64_t n, int64_t nblk,
                                                        const int32_t* __restrict__ offs, uint32_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout) {
-  __shared__ int part[4];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int part[kCompactSub][4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const uint32_t k = t < n ? key[t] : kNoKey;
-  const bool keep = k != kNoKey;
-  const unsigned long long m = __ballot(keep);
-  if (lane == 0) part[wid] = __popcll(m);
+  uint32_t k[kCompactSub];
+  unsigned long long m[kCompactSub];
+#pragma unroll
+  for (int j = 0; j < kCompactSub; j++) {
+    const int64_t sb = (int64_t)blockIdx.x * kCompactSub + j;
+    const int64_t t = sb * 256 + threadIdx.x;
+    k[j] = (sb < nblk && t < n) ? key[t] : kNoKey;
+  }
+#pragma unroll
+  for (int j = 0; j < kCompactSub; j++) {
+    m[j] = __ballot(k[j] != kNoKey);
+    if (lane == 0) part[j][wid] = __popcll(m[j]);
+  }
   __syncthreads();
-  int before = offs[blockIdx.x];
-  for (int w = 0; w < wid; w++) before += part[w];
-  if (keep) {
-    const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
-    kout[pos] = k;
-    vout[pos] = (uint32_t)t;
+#pragma unroll
+  for (int j = 0; j < kCompactSub; j++) {
+    const int64_t sb = (int64_t)blockIdx.x * kCompactSub + j;
+    if (sb >= nblk || k[j] == kNoKey) continue;
+    int before = offs[sb];
+    for (int w = 0; w < wid; w++) before += part[j][w];
+    const int pos = before + __popcll(m[j] & ((1ull << lane) - 1ull));
+    kout[pos] = k[j];
+    vout[pos] = (uint32_t)(sb * 256 + threadIdx.x);
   }
 }
 
@@ -1624,8 +1638,8 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
     if (rc) return rc;
     rc = scan_i32(c, bcount, boffs, nb, nullptr, btotal, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_query_compact, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t*)k1, n,
-                       (const int32_t*)boffs, k0, v0);
+    hipLaunchKernelGGL(k_query_compact, dim3((unsigned)((nb + kCompactSub - 1) / kCompactSub)), dim3(256), 0,
+                       c->stream, (const uint32_t*)k1, n, nb, (const int32_t*)boffs, k0, v0);
     rc = hip_check(c, hipGetLastError(), "compaction");
     if (rc) return rc;
   } else {
