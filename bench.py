@@ -272,7 +272,7 @@ def main():
         "config": {"workload": "fractal_1m: one polygon, 1M-vertex fractal ring (r=40 deg, centre lat 60 deg) "
                                "+ 3 holes x 50k vertices; 100M FP64 queries (70% uniform, 30% near-edge)",
                    "n_edges": int(st["n_edges"]), "n_queries": int(Q),
-                   "path": "single-polygon GPU: start-cell table, points grouped by start bin, lane-group leaf tests",
+                   "path": "single-polygon GPU: learned depth; start tables answer what they resolve, the rest grouped in Z-order (device radix sort) and walked on node records with angular-indexed leaf tests",
                    "parallelism": "1 GPU"},
         "build_ms": round(pre_ms, 3), "create_ms": round(create_ms, 3), "classify_ms": round(classify_ms, 3),
         "depth": int(st["depth"]), "depth_rule": "learned" if model is not None else "density",
