@@ -81,7 +81,11 @@ EXPORTED = [
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
     "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz", "gahhg_autotune_layout",
+    "gahhg_query_batch_xyz_device_noalloc",
 ]
+
+# Points per call of the allocation-free query (GAHHG_NOALLOC_MAX_POINTS).
+NOALLOC_MAX_POINTS = 4194304
 
 # Query trace record fields (6 int32 per point) and build trace layout.
 TRACE_FIELDS = ("leaf", "root", "level", "start_level", "plane_tests", "records")
@@ -138,6 +142,7 @@ def lib():
         "gahhg_num_polygons": ([P, ctypes.POINTER(i32)], ctypes.c_int),
         "gahhg_query_batch_xyz": ([P, dp, ip, i64, ip], ctypes.c_int),
         "gahhg_query_batch_xyz_device": ([P, P, P, i64, P], ctypes.c_int),
+        "gahhg_query_batch_xyz_device_noalloc": ([P, P, P, i64, P], ctypes.c_int),
         "gahhg_query_pairs_xyz": ([P, dp, i64, ip], ctypes.c_int),
         "gahhg_autotune_layout": ([P, P, P, i64, P, ctypes.c_double, P], ctypes.c_int),
         "gahhg_get_polygon_stats": ([P, i32, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -425,6 +430,13 @@ class GPISP:
 
     def query_batch_device(self, d_xyz, d_poly, n, d_labels):
         self._check(self.L.gahhg_query_batch_xyz_device(self.h, d_xyz.ptr, d_poly.ptr, int(n), d_labels.ptr))
+
+    def query_device_noalloc(self, d_xyz, d_poly, n, d_labels):
+        """gahhg_query_batch_xyz_device_noalloc: the same pass with its
+        scratch in the library's fixed device array (d_poly None for a
+        single polygon; at most NOALLOC_MAX_POINTS points)."""
+        self._check(self.L.gahhg_query_batch_xyz_device_noalloc(
+            self.h, d_xyz.ptr, None if d_poly is None else d_poly.ptr, int(n), d_labels.ptr))
 
     def set_polygon_depths(self, depths):
         """Fixed depth per polygon (0 = the context's rule); call after

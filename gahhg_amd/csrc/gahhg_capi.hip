@@ -302,6 +302,7 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
     return GAHHG_E_HIP;
   }
   for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
+  noalloc_init(c->device);
   if (hipMalloc((void**)&c->d_st, sizeof(DevState)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, 64, hipHostMallocDefault) != hipSuccess ||
@@ -719,6 +720,14 @@ int gahhg_load_grid(gahhg_ctx* c, const char* path) {
 
 int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n,
                                   int32_t* d_labels) {
+  return query_device(c, d_xyz, d_poly, n, d_labels, nullptr);
+}
+
+}  // extern "C"
+
+namespace gahhg {
+int query_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                 const SortScratch* ext) {
   RCK(ctx_ok(c));
   if (!c->built) return set_err(c, GAHHG_E_STATE, "query before build");
   if (n < 0 || (n > 0 && (!d_xyz || !d_labels))) return set_err(c, GAHHG_E_ARG, "query: bad arguments");
@@ -727,7 +736,7 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   RCK(select_device(c));
   RCK(check_poly_ids(c, d_poly, n));
   RCK(hip_check(c, hipEventRecord(c->ev[0], c->stream), "event"));
-  RCK(run_query(c, d_xyz, d_poly, n, d_labels));
+  RCK(run_query_with(c, d_xyz, d_poly, n, d_labels, ext));
   RCK(hip_check(c, hipEventRecord(c->ev[1], c->stream), "event"));
   RCK(hip_check(c, hipEventSynchronize(c->ev[1]), "query sync"));
   float ms = 0.f, kms = 0.f, sms = 0.f;
@@ -751,6 +760,9 @@ int gahhg_query_batch_xyz_device(gahhg_ctx* c, const double* d_xyz, const int32_
   c->st.outside = n - c->st.inside;
   return GAHHG_OK;
 }
+}  // namespace gahhg
+
+extern "C" {
 
 int gahhg_autotune_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                          int32_t* out3) {
@@ -1098,6 +1110,8 @@ int gahhg_get_stats(gahhg_ctx* c, gahhg_stats* out) {
   bytes += c->g.vcap * 24 + c->g.bcap * (6 * 4 + 24) + c->g.ccap * 7 * 4 + c->g.rcap * 4;
   bytes += c->q.bytes;
   bytes += c->n_edges * (24 + 8 + 72 + 12);
+  if (c->qsort.buf) bytes += sort_buffer_bytes(c->qsort.cap);  // grouping buffers (grow-only)
+  bytes += c->scap * 4;                                         // scan partial sums (grow-only)
   out->device_bytes = bytes;
   return GAHHG_OK;
 }

@@ -483,6 +483,12 @@ int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, con
 // Grows the scan scratch for n_bound elements now (a later scan of at most
 // that many then neither allocates nor synchronises).
 int scan_reserve(gahhg_ctx* c, int64_t n_bound);
+// int32 entries of scratch a scan of n_bound elements uses.
+int64_t scan_scratch_entries(int64_t n_bound);
+// scan_i32 over a caller's scratch of scan_scratch_entries(n_bound) int32
+// (nothing allocated); scratch == nullptr uses the context's.
+int scan_i32_with(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, const int32_t* d_n,
+                  int32_t* d_total, int inclusive, int32_t* scratch);
 
 // ---- build / classify / query (gahhg_build.hip etc.) ----
 int build_hierarchy(gahhg_ctx* c);
@@ -498,17 +504,44 @@ int expand_pairs(gahhg_ctx* c, const double* d_xyz, int64_t n, double* d_xyz_out
 // Groups the points for the walk; *order = point indices in group order.
 // Their number goes to *m, or, with host_count false, stays on the device
 // at *d_count (*m = -1) so that nothing waits for it.
+// The grouping's scratch: buf holds sort_buffer_bytes(cap) bytes (block
+// counts, offsets and total, two key and two value arrays of cap entries,
+// the radix histograms); scan = the scans' scratch, or nullptr for the
+// context's.
+struct SortScratch {
+  void* buf = nullptr;
+  int64_t cap = 0;
+  int32_t* scan = nullptr;
+};
+int64_t sort_buffer_bytes(int64_t cap);
+// ext: the scratch to use (nothing allocated; n <= ext->cap), or nullptr
+// for the context's grow-only buffers.
 int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
-                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count);
+                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count,
+                 const SortScratch* ext = nullptr);
 void drop_query_graph(gahhg_ctx* c);
+// run_query with the grouping's scratch from ext (nothing allocated, no
+// graph); ext == nullptr is run_query.
+int run_query_with(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                   const SortScratch* ext);
+// The body of gahhg_query_batch_xyz_device: checks, the timed pass (scratch
+// from ext when non-null), statistics and the label census.
+int query_device(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                 const SortScratch* ext);
+// Resolves, once per device, the device address of the allocation-free
+// query's scratch array (the device current); called by gahhg_create.
+void noalloc_init(int device);
 
 // ---- radix sort (gahhg_sort.hip) ----
 // Stable sort of *d_m (<= n_max) pairs (k0, v0) by key bits [0, end_bit);
 // (k1, v1) is the second buffer; *kres / *vres = where the result is
-// (one of the two).  hist holds radix_hist_entries(n_max) int32.
+// (one of the two).  hist holds radix_hist_entries(n_max) int32.  The
+// histogram scans use scan_scratch (scan_scratch_entries(256 * tiles)
+// int32), or the context's scan scratch when it is nullptr.
 int64_t radix_hist_entries(int64_t n_max);
 int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const int32_t* d_m,
-                     int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres);
+                     int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres,
+                     int32_t* scan_scratch = nullptr);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);

@@ -1606,18 +1606,30 @@ static int sort_buffers(gahhg_ctx* c, int64_t n) {
 // host_count it is also copied to pinned memory as soon as the compaction
 // has written it, and read after the sort is queued, so the GPU does not
 // idle while the host waits.  Buffers are kept by the context and grow.
+int64_t sort_buffer_bytes(int64_t cap) { return count_bytes(cap) + 16 * cap + 4 * radix_hist_entries(cap); }
+
 int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
-                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count) {
+                 const int32_t** order, int64_t* m, const int32_t** d_count, bool host_count,
+                 const SortScratch* ext) {
   if (n > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "sort: more than 2^31 points");
   // face coordinates at r bits per axis, r as large as the 32-bit key
   // leaves room for after the polygon and face bits (at most query_sort_bits)
   const int poly_bits = ceil_log2((int64_t)c->polys.size());
   const int r = std::max(1, std::min(c->prm.query_sort_bits, (32 - 3 - poly_bits) / 2));
   const int end_bit = std::min(32, 3 + 2 * r + poly_bits);
-  QuerySort& s = c->qsort;
   const int64_t nb = (n + 255) / 256;  // blocks of the table passes
-  int rc = sort_buffers(c, n);
-  if (rc) return rc;
+  int rc = GAHHG_OK;
+  SortScratch s;
+  if (ext) {
+    if (ext->cap < n) return set_err(c, GAHHG_E_LIMIT, "sort: %lld points exceed the scratch's %lld", (long long)n,
+                                     (long long)ext->cap);
+    s = *ext;
+  } else {
+    rc = sort_buffers(c, n);
+    if (rc) return rc;
+    s.buf = c->qsort.buf;
+    s.cap = c->qsort.cap;
+  }
   // buffer: [block counts | block offsets | total] [k0 | k1 | v0 | v1] [radix histograms]
   const int64_t cnt_cap = count_bytes(s.cap);
   int32_t* bcount = (int32_t*)s.buf;
@@ -1636,7 +1648,7 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
                        bcount);
     rc = hip_check(c, hipGetLastError(), "table pass");
     if (rc) return rc;
-    rc = scan_i32(c, bcount, boffs, nb, nullptr, btotal, 0);
+    rc = scan_i32_with(c, bcount, boffs, nb, nullptr, btotal, 0, s.scan);
     if (rc) return rc;
     hipLaunchKernelGGL(k_query_compact, dim3((unsigned)((nb + kCompactSub - 1) / kCompactSub)), dim3(256), 0,
                        c->stream, (const uint32_t*)k1, n, nb, (const int32_t*)boffs, k0, v0);
@@ -1654,7 +1666,7 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
     if (rc) return rc;
   }
   uint32_t *kres = nullptr, *vres = nullptr;
-  rc = radix_sort_pairs(c, k0, v0, k1, v1, btotal, n, end_bit, hist, &kres, &vres);
+  rc = radix_sort_pairs(c, k0, v0, k1, v1, btotal, n, end_bit, hist, &kres, &vres, s.scan);
   if (rc) return rc;
   *order = (const int32_t*)vres;
   *d_count = btotal;
@@ -1691,12 +1703,12 @@ static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
 // The work of one accelerated query pass, enqueued with no host wait when
 // host_count is false (then it can be recorded into a graph).
 static int enqueue_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
-                         bool host_count) {
+                         bool host_count, const SortScratch* ext = nullptr) {
   const int32_t* order = nullptr;
   const int32_t* d_m = nullptr;
   int64_t m = n;
   if (c->prm.query_sort == 2 || (c->prm.query_sort == 1 && n >= kSortMin)) {
-    const int rc = sort_queries(c, d_xyz, d_poly, n, d_labels, &order, &m, &d_m, host_count);
+    const int rc = sort_queries(c, d_xyz, d_poly, n, d_labels, &order, &m, &d_m, host_count, ext);
     if (rc) return rc;
   }
   if (host_count) (void)hipEventRecord(c->ev[2], c->stream);
@@ -1746,6 +1758,14 @@ static int graph_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly,
   const int rc = hip_check(c, hipGraphLaunch(G.exec, c->stream), "graph launch");
   (void)hipEventRecord(c->ev[3], c->stream);
   return rc;
+}
+
+int run_query_with(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
+                   const SortScratch* ext) {
+  if (!ext) return run_query(c, d_xyz, d_poly, n, d_labels);
+  if (n <= 0) return GAHHG_OK;
+  if (c->prm.query_layout == 1) return run_query(c, d_xyz, d_poly, n, d_labels);  // uses no scratch
+  return enqueue_query(c, d_xyz, d_poly, n, d_labels, true, ext);
 }
 
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels) {

@@ -150,6 +150,18 @@ int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, con
   return scan_rec(c, in, out, n_bound, d_n, d_total, inclusive, c->d_scan);
 }
 
+int64_t scan_scratch_entries(int64_t n_bound) { return scratch_need(std::max<int64_t>(n_bound, 1)); }
+
+int scan_i32_with(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, const int32_t* d_n,
+                  int32_t* d_total, int inclusive, int32_t* scratch) {
+  if (!scratch) return scan_i32(c, in, out, n_bound, d_n, d_total, inclusive);
+  if (n_bound <= 0) {
+    if (d_total) return fill_i32(c, d_total, 1, 0);
+    return GAHHG_OK;
+  }
+  return scan_rec(c, in, out, n_bound, d_n, d_total, inclusive, scratch);
+}
+
 int scan_reserve(gahhg_ctx* c, int64_t n_bound) {
   const int64_t need = scratch_need(std::max<int64_t>(n_bound, 1));
   if (need <= c->scap) return GAHHG_OK;

@@ -154,7 +154,8 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint32_t* __res
 int64_t radix_hist_entries(int64_t n_max) { return 256 * ((n_max + kRsTile - 1) / kRsTile) + 1; }
 
 int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const int32_t* d_m,
-                     int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres) {
+                     int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres,
+                     int32_t* scan_scratch) {
   if (n_max > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "radix sort: more than 2^31 pairs");
   const int64_t tiles = (n_max + kRsTile - 1) / kRsTile;
   int32_t* d_hn = hist + 256 * tiles;
@@ -166,7 +167,7 @@ int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uin
                          d_m, shift, mask, hist, d_hn);
       int rc = hip_check(c, hipGetLastError(), "radix histogram");
       if (rc) return rc;
-      rc = scan_i32(c, hist, hist, 256 * tiles, d_hn, nullptr, 0);
+      rc = scan_i32_with(c, hist, hist, 256 * tiles, d_hn, nullptr, 0, scan_scratch);
       if (rc) return rc;
       hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)tiles), dim3(kRsThreads), 0, c->stream, (const uint32_t*)ki,
                          (const uint32_t*)vi, d_m, shift, mask, (const int32_t*)hist, ko, vo);

@@ -214,6 +214,21 @@ int gahhg_num_polygons(gahhg_ctx* ctx, int32_t* n);
 int gahhg_query_batch_xyz(gahhg_ctx* ctx, const double* xyz, const int32_t* poly, int64_t n, int32_t* labels);
 int gahhg_query_batch_xyz_device(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n,
                                  int32_t* d_labels);
+/* Allocation-free form of gahhg_query_batch_xyz_device (d_poly may be null
+ * for a single polygon): the same labels and statistics, but the grouping
+ * pass's scratch (block counts, sort keys and point indices, radix
+ * histograms, scan partial sums) comes from a fixed device array of the
+ * library instead of buffers the context allocates or grows, and the pass is
+ * never recorded as a graph, so the call makes no hipMalloc / hipFree.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (serving loops and harnesses that forbid device allocation).  At most
+ * GAHHG_NOALLOC_MAX_POINTS points per call (GAHHG_E_LIMIT beyond).  Each GPU
+ * holds one copy of the array (about 68 MB) until the process ends, shared
+ * by every context on that GPU: calls that overlap on one GPU race.  The
+ * call still waits for its stream, as the allocating form does. */
+#define GAHHG_NOALLOC_MAX_POINTS 4194304
+int gahhg_query_batch_xyz_device_noalloc(gahhg_ctx* ctx, const double* d_xyz, const int32_t* d_poly, int64_t n,
+                                         int32_t* d_labels);
 /* Every polygon of the batch against the same n points, as a series of
  * GPISP.exe runs with one query file (test/explist.bat): labels[p * n + i]
  * is point i in polygon p.  The points are sent once and repeated on the
