@@ -59,6 +59,7 @@ _STAT_FIELDS = [
     ("walk_unlocated", ctypes.c_int64), ("zero_ref_edges", ctypes.c_int64),
     ("undefined_leaves_snapshot", ctypes.c_int64), ("undefined_leaves_final", ctypes.c_int64),
     ("query_kernel_ms", ctypes.c_double), ("query_sort_ms", ctypes.c_double),
+    ("layout_ms", ctypes.c_double),
 ]
 
 
@@ -81,8 +82,15 @@ EXPORTED = [
     "gahhg_autotune_query", "gahhg_save_grid", "gahhg_load_grid", "gahhg_write_polygon_report",
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
     "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz", "gahhg_autotune_layout",
-    "gahhg_query_batch_xyz_device_noalloc",
+    "gahhg_query_batch_xyz_device_noalloc", "gahhg_set_table_model", "gahhg_set_table_choice",
+    "gahhg_get_table_layout", "gahhg_table_features",
 ]
+
+# Start-table candidates of the learned table depth (bins per face side,
+# sub-table side); a batch considers the first TABLE_CAND_BATCH.
+TABLE_BINS = (0, 16, 32, 64, 128, 256, 256, 256)
+TABLE_SUB = (1, 1, 1, 1, 1, 1, 8, 16)
+TABLE_CAND_BATCH = 5
 
 # Points per call of the allocation-free query (GAHHG_NOALLOC_MAX_POINTS).
 NOALLOC_MAX_POINTS = 4194304
@@ -155,6 +163,10 @@ def lib():
         "gahhg_mlp_backward": ([P, P, i64, P, P, P], ctypes.c_int),
         "gahhg_mlp_adam": ([P, P, P, P, P, i32, ctypes.c_float, P], ctypes.c_int),
         "gahhg_set_depth_model": ([P, P, P], ctypes.c_int),
+        "gahhg_set_table_model": ([P, P, P], ctypes.c_int),
+        "gahhg_set_table_choice": ([P, P, i32], ctypes.c_int),
+        "gahhg_get_table_layout": ([P, P], ctypes.c_int),
+        "gahhg_table_features": ([P, P], ctypes.c_int),
         "gahhg_set_query_hint": ([P, P, P, i64, ctypes.c_double], ctypes.c_int),
         "gahhg_depth_features": ([P, i32, P], ctypes.c_int),
         "gahhg_autotune_query": ([P, P, P, i64, P, P], ctypes.c_int),
@@ -529,6 +541,41 @@ class GPISP:
         assert pa.size == 5313 and co.size == 5
         self._check(self.L.gahhg_set_depth_model(self.h, pa.ctypes.data, co.ctypes.data))
         return self
+
+    def set_table_model(self, params, cost):
+        """Learned start-table depth (label-neutral): params = the model's
+        5313 values, cost = 19 values (expected points without a hint,
+        query scale, per candidate the table build ns a_c, b_c per edge of
+        one polygon, ns per table bin in a batch).  params None switches it
+        off."""
+        if params is None:
+            self._check(self.L.gahhg_set_table_model(self.h, None, None))
+            return self
+        pa = np.ascontiguousarray(params, dtype=np.float32)
+        co = np.ascontiguousarray(cost, dtype=np.float64)
+        assert pa.size == 5313 and co.size == 3 + 2 * len(TABLE_BINS)
+        self._check(self.L.gahhg_set_table_model(self.h, pa.ctypes.data, co.ctypes.data))
+        return self
+
+    def set_table_choice(self, cand):
+        """Fixed start-table candidate per polygon (-1: chosen); None clears."""
+        if cand is None:
+            self._check(self.L.gahhg_set_table_choice(self.h, None, 0))
+            return self
+        a = np.ascontiguousarray(cand, dtype=np.int32)
+        self._check(self.L.gahhg_set_table_choice(self.h, a.ctypes.data, a.size))
+        return self
+
+    def table_layout(self):
+        """(bins per face side, sub-table side) per polygon of the last build."""
+        out = np.zeros((self.n_polygons, 2), dtype=np.int32)
+        self._check(self.L.gahhg_get_table_layout(self.h, out.ctypes.data))
+        return out
+
+    def table_features(self):
+        x = np.zeros((self.n_polygons, TOP_CELLS, len(TABLE_BINS), 12), dtype=np.float32)
+        self._check(self.L.gahhg_table_features(self.h, x.ctypes.data))
+        return x
 
     def set_query_hint(self, xyz, poly=None, weight=1.0):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

@@ -83,6 +83,7 @@ void free_scratch(gahhg_ctx* c) {
 int upload_polygons(gahhg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->depth_override.clear();  // fixed depths belong to the previous polygons
+  c->table_fixed.clear();
   c->learn.estat_valid = c->learn.hist_valid = false;
   free_grid(c);
   free_scratch(c);
@@ -497,6 +498,8 @@ int gahhg_build(gahhg_ctx* c) {
   RCK(build_hierarchy(c));
   const double t1 = now_ms();
   RCK(classify_hierarchy(c));
+  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+  const double tl = now_ms();
   RCK(pack_query_layout(c));
   RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
   const double t2 = now_ms();
@@ -521,6 +524,7 @@ int gahhg_build(gahhg_ctx* c) {
   c->st.create_ms = t1 - t0;
   c->st.classify_ms = t2 - t1;
   c->st.pre_ms = t2 - t0;
+  c->st.layout_ms = t2 - tl;
   const DevState& s = *c->h_st;
   c->st.walk_steps_mark = (int64_t)s.steps_mark;
   c->st.walk_steps_dispatch = (int64_t)s.steps_dispatch;
@@ -556,6 +560,57 @@ std::vector<Arr> grid_arrays(gahhg_ctx* c, int64_t V, int64_t B, int64_t C, int6
           {g.c_bstart, 4 * C},   {g.c_bnum, 4 * C},    {g.c_rstart, 4 * C},  {g.c_rnum, 4 * C},
           {g.c_upcell, 4 * C},   {g.c_subcell, 4 * C}, {g.c_type, 4 * C},    {g.c_poly, 4 * C},
           {g.c_root, 4 * C},     {g.refs, 4 * R}};
+}
+
+// Contents of the arrays of a grid file (in grid_arrays() order) checked
+// against every range a kernel indexes with; returns what is wrong, or
+// nullptr.  Levels: the global level ranges tile the cells in order, and a
+// cell's centre child and border children lie on the next level and its
+// parent on the previous one, so every descent ends within the levels.
+const char* check_grid_arrays(const std::vector<std::vector<char>>& h, const int64_t* cnt,
+                              const std::vector<LevelDesc>& glv, const std::vector<PolyDev>& pd, int64_t n_edges) {
+  const int64_t V = cnt[0], B = cnt[1], C = cnt[2], R = cnt[3];
+  auto I = [&](int k) { return (const int32_t*)h[k].data(); };
+  const int32_t *b_parent = I(1), *b_partner = I(2), *b_v0 = I(3), *b_v1 = I(4), *b_sub = I(5), *b_id = I(6);
+  const int32_t *c_bstart = I(8), *c_bnum = I(9), *c_rstart = I(10), *c_rnum = I(11), *c_up = I(12);
+  const int32_t *c_sub = I(13), *c_type = I(14), *c_poly = I(15), *c_root = I(16), *refs = I(17);
+  std::vector<int32_t> level((size_t)C, -1);
+  int64_t at = 0;
+  for (size_t l = 0; l < glv.size(); l++) {
+    if (glv[l].cstart != at) return "level ranges do not tile the cells";
+    for (int64_t k = 0; k < glv[l].cnum; k++) level[(size_t)(at + k)] = (int32_t)l;
+    at += glv[l].cnum;
+  }
+  if (at != C) return "level ranges do not tile the cells";
+  const int64_t np = (int64_t)pd.size();
+  for (int64_t k = 0; k < C; k++) {
+    const int l = level[(size_t)k];
+    if (c_bnum[k] < 3 || c_bnum[k] > 6 || c_bstart[k] < 0 || (int64_t)c_bstart[k] + c_bnum[k] > B)
+      return "cell border range";
+    if (c_type[k] < 0 || c_type[k] > 2 || c_poly[k] < 0 || c_poly[k] >= np || c_root[k] < 0 ||
+        c_root[k] >= kGrid0Cells)
+      return "cell type, polygon or root";
+    if (c_up[k] != -1 && (c_up[k] < 0 || c_up[k] >= C || level[(size_t)c_up[k]] != l - 1)) return "cell parent";
+    if (c_sub[k] != -1 && (c_sub[k] < 0 || c_sub[k] >= C || level[(size_t)c_sub[k]] != l + 1))
+      return "cell centre child";
+    const bool leaf = c_sub[k] == -1;
+    if (c_rnum[k] < 0 || (leaf && c_rnum[k] == 999 && c_rstart[k] < 0)) return "cell refs";
+    if (c_rnum[k] > 0 && !(c_rnum[k] == 999 && !leaf) &&
+        (c_rstart[k] < 0 || (int64_t)c_rstart[k] + c_rnum[k] > R))
+      return "cell ref range";
+  }
+  for (int64_t b = 0; b < B; b++) {
+    if (b_parent[b] < 0 || b_parent[b] >= C) return "border parent";
+    if (b_partner[b] < -1 || b_partner[b] >= B) return "border partner";
+    if (b_v0[b] < 0 || b_v0[b] >= V || b_v1[b] < 0 || b_v1[b] >= V) return "border vertex";
+    if (b_id[b] < 0 || b_id[b] >= 6) return "border position";
+    if (b_sub[b] != -1 &&
+        (b_sub[b] < 0 || b_sub[b] >= C || level[(size_t)b_sub[b]] != level[(size_t)b_parent[b]] + 1))
+      return "border child";
+  }
+  for (int64_t r = 0; r < R; r++)
+    if (refs[r] < 0 || refs[r] >= n_edges) return "ref edge";
+  return nullptr;
 }
 
 bool wr(FILE* f, const void* p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
@@ -670,22 +725,33 @@ int gahhg_load_grid(gahhg_ctx* c, const char* path) {
     fclose(f);
     return set_err(c, GAHHG_E_FORMAT, "load: %s is not a grid file of this library version", path);
   }
+  // every array is read and checked on the host before anything reaches
+  // the device: the build, pack and query kernels index through them
+  std::vector<std::vector<char>> host(18);
+  {
+    const int64_t V = cnt[0], B = cnt[1], C = cnt[2], R = cnt[3];
+    const int64_t sizes[18] = {24 * V, 4 * B, 4 * B, 4 * B, 4 * B, 4 * B, 4 * B, 24 * B, 4 * C,
+                               4 * C,  4 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * R};
+    for (int k = 0; k < 18 && ok; k++) {
+      host[k].resize((size_t)sizes[k]);
+      ok = rd(f, host[k].data(), (size_t)sizes[k]);
+    }
+  }
+  int64_t und[2] = {0, 0};
+  ok = ok && rd(f, und, 16);
+  fclose(f);
+  if (!ok) return set_err(c, GAHHG_E_FORMAT, "load: %s is truncated", path);
+  const char* bad = check_grid_arrays(host, cnt, glv, pd, e0);
+  if (bad) return set_err(c, GAHHG_E_FORMAT, "load: %s: %s", path, bad);
   c->polys = std::move(polys);
   int rc = upload_polygons(c);
   if (!rc) rc = attach_loaded_grid(c, cnt[0], cnt[1], cnt[2], cnt[3]);
-  std::vector<char> buf;
+  int k = 0;
   for (const Arr& a : grid_arrays(c, cnt[0], cnt[1], cnt[2], cnt[3])) {
     if (rc) break;
-    buf.resize((size_t)a.bytes);
-    if (!rd(f, buf.data(), (size_t)a.bytes)) {
-      rc = set_err(c, GAHHG_E_FORMAT, "load: %s is truncated", path);
-      break;
-    }
-    if (a.bytes) rc = hip_check(c, hipMemcpy(a.p, buf.data(), a.bytes, hipMemcpyHostToDevice), "load H2D");
+    if (a.bytes) rc = hip_check(c, hipMemcpy(a.p, host[k].data(), a.bytes, hipMemcpyHostToDevice), "load H2D");
+    k++;
   }
-  int64_t und[2] = {0, 0};
-  if (!rc && !rd(f, und, 16)) rc = set_err(c, GAHHG_E_FORMAT, "load: %s is truncated", path);
-  fclose(f);
   if (rc) {
     free_grid(c);
     return rc;
@@ -959,6 +1025,74 @@ int gahhg_set_depth_model(gahhg_ctx* c, const float* params, const double* cost)
   L.on = true;
   c->built = false;
   return GAHHG_OK;
+}
+
+int gahhg_set_table_model(gahhg_ctx* c, const float* params, const double* cost) {
+  RCK(ctx_ok(c));
+  RCK(select_device(c));
+  LearnState& L = c->learn;
+  if (!params) {
+    L.table_on = false;
+    c->built = false;
+    return GAHHG_OK;
+  }
+  if (!cost) return set_err(c, GAHHG_E_ARG, "table model: cost constants required");
+  for (int i = 0; i < 3 + 2 * kTableCand; i++)
+    if (!(cost[i] >= 0.0) || !std::isfinite(cost[i]))
+      return set_err(c, GAHHG_E_ARG, "table model: cost[%d] must be finite and >= 0", i);
+  std::vector<uint16_t> w((size_t)kMlpParams + 8, 0);
+  for (int k = 0; k < kMlpParams; k++) {
+    uint32_t u;
+    memcpy(&u, &params[k], 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return set_err(c, GAHHG_E_ARG, "table model: non-finite parameter");
+    u += 0x7fffu + ((u >> 16) & 1u);  // BF16, round to nearest even
+    w[k] = (uint16_t)(u >> 16);
+  }
+  if (!L.d_tweights) RCK(dalloc(c, (void**)&L.d_tweights, (int64_t)w.size() * 2));
+  RCK(hip_check(c, hipMemcpy(L.d_tweights, w.data(), w.size() * 2, hipMemcpyHostToDevice), "model upload"));
+  L.tparams.assign(params, params + kMlpParams);
+  for (int i = 0; i < 3 + 2 * kTableCand; i++) L.tcost[i] = cost[i];
+  L.table_on = true;
+  c->built = false;
+  return GAHHG_OK;
+}
+
+int gahhg_set_table_choice(gahhg_ctx* c, const int32_t* cand, int32_t n) {
+  RCK(ctx_ok(c));
+  if (!cand || n == 0) {
+    c->table_fixed.clear();
+    c->built = false;
+    return GAHHG_OK;
+  }
+  if (!c->have_poly || n != (int32_t)c->polys.size())
+    return set_err(c, GAHHG_E_ARG, "table choice: %d values for %d polygons", n, (int)c->polys.size());
+  for (int32_t k = 0; k < n; k++)
+    if (cand[k] < -1 || cand[k] >= kTableCand)
+      return set_err(c, GAHHG_E_ARG, "table choice: candidate %d of polygon %d outside -1..%d", cand[k], k,
+                     kTableCand - 1);
+  c->table_fixed.assign(cand, cand + n);
+  c->built = false;
+  return GAHHG_OK;
+}
+
+int gahhg_get_table_layout(gahhg_ctx* c, int32_t* bins_sub) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "table layout before build");
+  if (!bins_sub) return set_err(c, GAHHG_E_ARG, "table layout: null output");
+  const size_t np = c->polys.size();
+  for (size_t p = 0; p < np; p++) {
+    bins_sub[2 * p] = c->table_used.size() == 2 * np ? c->table_used[2 * p] : 0;
+    bins_sub[2 * p + 1] = c->table_used.size() == 2 * np ? c->table_used[2 * p + 1] : 1;
+  }
+  return GAHHG_OK;
+}
+
+int gahhg_table_features(gahhg_ctx* c, float* x) {
+  RCK(ctx_ok(c));
+  if (!c->built) return set_err(c, GAHHG_E_STATE, "table features before build");
+  if (!x) return set_err(c, GAHHG_E_ARG, "table features: null output");
+  RCK(select_device(c));
+  return table_features_after_build(c, x);
 }
 
 int gahhg_set_query_hint(gahhg_ctx* c, const double* xyz, const int32_t* poly, int64_t n, double weight) {

@@ -132,6 +132,7 @@ struct DevState {
   unsigned long long zero_ref_edges, undefined_leaves;
   unsigned long long q_in, q_undef;  // label census of the last query
   int32_t bad_poly;                  // a device polygon id was out of range
+  int32_t bad_nodes;                 // node records of cells with more than 6 borders
   LevelDesc lv[kMaxLevels];
 };
 
@@ -261,6 +262,7 @@ struct QueryLayout {
   int64_t leaves = 0;  // bottom cells with refs
   int32_t* pcnt = nullptr;  // per polygon: edge-stream records, leaves with refs
   int64_t bytes = 0;
+  int64_t bytes_nodes = 0;  // node records (counted into bytes once the leaf records are sized)
   // allocations behind the pointers above (a pointer is null when its part
   // of the layout is not built)
   DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart, b_l0, b_nodes;
@@ -305,7 +307,26 @@ struct LearnState {
   int64_t cap = 0;                        // polygons the arrays hold
   std::vector<int32_t> h_dec;
   std::vector<int32_t> rule_stop;  // level where the density rule stops, -1 = not yet
+  // learned start-table depth (label-neutral): model, cost constants
+  // (expected points without a hint, query scale, per candidate the table
+  // build ns a_c + b_c * N of one polygon, then the ns per table bin in a
+  // batch) and its per (polygon, top-level cell, candidate) arrays
+  bool table_on = false;
+  std::vector<float> tparams;
+  double tcost[3 + 2 * 8] = {0};
+  uint16_t* d_tweights = nullptr;
+  float* d_tx = nullptr;  // np x 42 x kTableCand x 12 features
+  float* d_ty = nullptr;  // np x 42 x kTableCand predictions
+  int64_t tcap = 0;
 };
+
+// Start-table candidates of the learned table depth: bins per cube-face side
+// and sub-table side.  A batch considers the first kTableCandBatch (no
+// sub-tables, at most 128 bins); one polygon all of them.
+constexpr int kTableCand = 8;
+constexpr int kTableCandBatch = 5;
+constexpr int kTableB[kTableCand] = {0, 16, 32, 64, 128, 256, 256, 256};
+constexpr int kTableS[kTableCand] = {1, 1, 1, 1, 1, 1, 8, 16};
 
 // Window of the learned depth decision (crossed cells / N below which a
 // polygon always refines; levels past the density rule's stop it may add).
@@ -431,6 +452,8 @@ struct gahhg_ctx {
   std::vector<float> pgamma;                        // per polygon density ratio
   std::vector<int32_t> depth_override;              // per polygon fixed depth (0 = rule)
   std::vector<int32_t> pmarks;                      // per polygon levels marked in the last build
+  std::vector<int32_t> table_fixed;                 // per polygon start-table candidate (-1 = chosen)
+  std::vector<int32_t> table_used;                  // per polygon (bins, sub side) of the last layout
   std::vector<gahhg::LevelDesc> glv;                // per level: global ranges over all polygons
   int64_t tot_v = 0, tot_b = 0, tot_c = 0, tot_r = 0;
 
@@ -554,9 +577,16 @@ int mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m, flo
 int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges);
 int edge_stats(gahhg_ctx* c, unsigned long long* d_out);
 int learn_prepare(gahhg_ctx* c);
+// learn_prepare with q_total expected points when there is no hint.
+int learn_prepare_with(gahhg_ctx* c, double q_total);
 int attach_loaded_grid(gahhg_ctx* c, int64_t V, int64_t B, int64_t C, int64_t R);
 int learn_level(gahhg_ctx* c, int level, int lo, int n, int32_t* h_dec);
 int learn_features_after_build(gahhg_ctx* c, int level, float* h_x);
+// Learned start-table depth: features of the built hierarchy for every
+// candidate (h_x: np x 42 x kTableCand x 12, null = keep on the device) and,
+// with a table model set, the chosen candidate per polygon into h_choice.
+int table_features_after_build(gahhg_ctx* c, float* h_x);
+int table_choose(gahhg_ctx* c, int32_t* h_choice);
 void learn_free(gahhg_ctx* c);
 
 GridView view(gahhg_ctx* c);

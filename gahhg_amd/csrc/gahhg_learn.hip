@@ -69,6 +69,25 @@ __global__ void k_hint_hist(GridView g, const PolyDev* __restrict__ pd, const do
 // is crossed (build time: marked; after the build: refined).  loc[t] holds
 // the point's cell `steps` levels above (or -1 once it ended in a leaf above
 // the band, where it stays); the kernel descends it and stores the cell.
+// `steps` levels of the first-match descent from cell cur (-1 once the
+// point ends in a leaf above, or no child holds it).
+__device__ __forceinline__ int descend(const GridView& g, V3 q, int cur, int steps) {
+  for (int s = 0; s < steps && cur >= 0; s++) {
+    const int sc = g.c_subcell[cur];
+    if (sc == -1) return -1;
+    int next = inside(g, q, sc) ? sc : -1;
+    if (next < 0) {
+      const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
+      for (int i = 0; i < bn && next < 0; i++) {
+        const int tc = g.b_subcell[bs + i];
+        if (tc != -1 && inside(g, q, tc)) next = tc;
+      }
+    }
+    cur = next;
+  }
+  return cur;
+}
+
 __global__ void k_hint_band(GridView g, const double* __restrict__ xyz, int32_t* __restrict__ loc, int64_t n,
                             int steps, int marked, int64_t nbins, uint32_t* __restrict__ band) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,22 +96,7 @@ __global__ void k_hint_band(GridView g, const double* __restrict__ xyz, int32_t*
     int cur = loc[t];
     if (cur >= 0) {
       const V3 q = load3(xyz + 3 * t);
-      for (int s = 0; s < steps && cur >= 0; s++) {
-        const int sc = g.c_subcell[cur];
-        if (sc == -1) {
-          cur = -1;
-          break;
-        }
-        int next = inside(g, q, sc) ? sc : -1;
-        if (next < 0) {
-          const int bs = g.c_bstart[cur], bn = g.c_bnum[cur];
-          for (int i = 0; i < bn && next < 0; i++) {
-            const int tc = g.b_subcell[bs + i];
-            if (tc != -1 && inside(g, q, tc)) next = tc;
-          }
-        }
-        cur = next;
-      }
+      cur = descend(g, q, cur, steps);
       loc[t] = cur;
       if (cur >= 0 && (marked ? g.c_rnum[cur] != 0 : g.c_subcell[cur] != -1))
         key = g.c_poly[cur] * kGrid0Cells + g.c_root[cur];
@@ -143,16 +147,54 @@ __global__ void k_level_stats(GridView g, const PolyDev* __restrict__ pd, int lo
   wave_add<int32_t>(st, key >= 0 ? key + 1 : -1, crossed);
 }
 
-// Features of rows (p, r, k), row = (p * 42 + r) * 2 + k.
-__global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
-                           const unsigned long long* __restrict__ estat, const float* __restrict__ w,
-                           const uint32_t* __restrict__ hist, const uint32_t* __restrict__ band,
-                           float* __restrict__ X) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= npoly * kGrid0Cells * 2) return;
-  const int k = row & 1;
-  const int pr = row >> 1;
-  const int p = pr / kGrid0Cells;
+// The last refined level of polygon P (its bottom level - 1; 0 when it has
+// one level): the level whose refined cells stand for the crossed ones.
+__device__ __forceinline__ int last_refined(const PolyDev& P) { return max(P.L - 1, 0); }
+
+// k_level_stats after a build, at each polygon's last refined level: every
+// cell of the hierarchy is visited and counted when it lies on that level.
+__global__ void k_level_stats_last(GridView g, const PolyDev* __restrict__ pd, LevelTab lt,
+                                   int32_t* __restrict__ st) {
+  const int cid = blockIdx.x * blockDim.x + threadIdx.x;
+  int key = -1, crossed = 0;
+  if (cid < lt.cstart[lt.n]) {
+    int l = 0;
+    while (l + 1 < lt.n && lt.cstart[l + 1] <= cid) l++;
+    const int p = g.c_poly[cid];
+    if (l == last_refined(pd[p])) {
+      key = 2 * (p * kGrid0Cells + g.c_root[cid]);
+      crossed = g.c_subcell[cid] != -1 ? 1 : 0;
+    }
+  }
+  wave_add<int32_t>(st, key, key >= 0 ? 1 : 0);
+  wave_add<int32_t>(st, key >= 0 ? key + 1 : -1, crossed);
+}
+
+// k_hint_band after a build: each hint point descends from its top-level
+// cell (loc) to its polygon's last refined level and counts when the cell
+// it reaches there is refined.
+__global__ void k_hint_band_last(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ xyz,
+                                 const int32_t* __restrict__ hpoly, const int32_t* __restrict__ loc, int64_t n,
+                                 int npoly, int64_t nbins, uint32_t* __restrict__ band) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int key = -1;
+  if (t < n) {
+    const int p = hpoly ? hpoly[t] : 0;
+    int cur = loc[t];
+    if (cur >= 0 && p >= 0 && p < npoly) {
+      cur = descend(g, load3(xyz + 3 * t), cur, last_refined(pd[p]));
+      if (cur >= 0 && g.c_subcell[cur] != -1) key = g.c_poly[cur] * kGrid0Cells + g.c_root[cur];
+    }
+  }
+  wave_add<uint32_t>(band + (blockIdx.x % kStripes) * nbins, key, 1u);
+}
+
+// Features 0 and 2..11 of (polygon p, top-level cell r) at `level` (feature
+// 1 is the caller's: refine-or-stop or the start-table depth).
+__device__ void cell_features(float* x, int level, int p, int pr, const PolyDev* __restrict__ pd,
+                              const int32_t* __restrict__ st, const unsigned long long* __restrict__ estat,
+                              const float* __restrict__ w, const uint32_t* __restrict__ hist,
+                              const uint32_t* __restrict__ band) {
   const float Nr = (float)estat[3 * pr];
   const float Ur = (float)st[2 * pr + 1];
   const float Cr = (float)st[2 * pr];
@@ -161,9 +203,7 @@ __global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd,
   float wsum = 0.f;
   for (int r = 0; r < kGrid0Cells; r++) wsum += w[p * kGrid0Cells + r];
   const double cell = 0.55 / (double)(1LL << min(level, 60));
-  float* x = X + (int64_t)row * kMlpIn;
   x[0] = level / 16.0f;
-  x[1] = (float)k;
   x[2] = log2f(1.0f + Nr) / 20.0f;
   x[3] = log2f(1.0f + Ur) / 20.0f;
   x[4] = log2f(1.0f + Cr) / 20.0f;
@@ -177,6 +217,78 @@ __global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd,
   x[9] = hist ? (float)band[pr] / fmaxf((float)hist[pr], 1.0f) : Ur / fmaxf(Cr, 1.0f);
   x[10] = log2f(1.0f + (float)pd[p].N) / 20.0f;
   x[11] = wsum > 0.f ? w[pr] / wsum : Nr / fmaxf((float)pd[p].N, 1.0f);
+}
+
+// Start-table depth feature of candidate c: log2 of the bins per face side
+// times the sub-table side (0 without a table), / 12.
+__device__ __forceinline__ float table_depth_feature(int c) {
+  return log2f(1.0f + (float)(kTableB[c] * kTableS[c])) / 12.0f;
+}
+
+// Features of rows (p, r, c) after a build, row = (p * 42 + r) * kTableCand + c,
+// at each polygon's last refined level.
+__global__ void k_table_features(int npoly, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
+                                 const unsigned long long* __restrict__ estat, const float* __restrict__ w,
+                                 const uint32_t* __restrict__ hist, const uint32_t* __restrict__ band,
+                                 float* __restrict__ X) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= npoly * kGrid0Cells * kTableCand) return;
+  const int cand = row % kTableCand;
+  const int pr = row / kTableCand;
+  const int p = pr / kGrid0Cells;
+  float* x = X + (int64_t)row * kMlpIn;
+  cell_features(x, last_refined(pd[p]), p, pr, pd, st, estat, w, hist, band);
+  x[1] = table_depth_feature(cand);
+  // expected query density: the points expected in r (in place of the edges
+  // per crossed cell, which features 3 and 5 already give)
+  x[6] = log2f(1.0f + w[pr]) / 32.0f;
+}
+
+// Table build cost per candidate: a[c] + b[c] * N ns for one polygon; in a
+// batch, whose launches are shared, kbin ns per table bin (6 B^2 bins).
+struct TableCost {
+  double a[kTableCand], b[kTableCand];
+  double kbin;
+};
+
+// Start-table candidate per polygon: the least predicted build + query
+// time, query time = query_scale * sum_r (expected points of r) * 2^y.
+// Candidates beyond ncand, with more bins than bins_max or a sub-table side
+// above sub_max are not considered.
+__global__ void k_decide_table(int npoly, const PolyDev* __restrict__ pd, const float* __restrict__ Y,
+                               const float* __restrict__ w, int ncand, int bins_max, int sub_max,
+                               double query_scale, TableCost cost, int32_t* __restrict__ dec) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoly) return;
+  int best = 0;
+  double best_t = 0.0;
+  for (int c = 0; c < ncand; c++) {
+    if (kTableB[c] > bins_max || kTableS[c] > max(sub_max, 1)) continue;
+    double q = 0.0;
+    for (int r = 0; r < kGrid0Cells; r++) {
+      const int pr = p * kGrid0Cells + r;
+      q += (double)w[pr] * exp2((double)Y[(int64_t)pr * kTableCand + c]);
+    }
+    const double build = npoly == 1 ? cost.a[c] + cost.b[c] * pd[p].N : cost.kbin * 6.0 * kTableB[c] * kTableB[c];
+    const double t = query_scale * q + build;
+    if (c == 0 || t < best_t) best = c, best_t = t;
+  }
+  dec[p] = best;
+}
+
+// Features of rows (p, r, k), row = (p * 42 + r) * 2 + k.
+__global__ void k_features(int npoly, int level, const PolyDev* __restrict__ pd, const int32_t* __restrict__ st,
+                           const unsigned long long* __restrict__ estat, const float* __restrict__ w,
+                           const uint32_t* __restrict__ hist, const uint32_t* __restrict__ band,
+                           float* __restrict__ X) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= npoly * kGrid0Cells * 2) return;
+  const int k = row & 1;
+  const int pr = row >> 1;
+  const int p = pr / kGrid0Cells;
+  float* x = X + (int64_t)row * kMlpIn;
+  cell_features(x, level, p, pr, pd, st, estat, w, hist, band);
+  x[1] = (float)k;
 }
 
 // Refine-or-stop per polygon still refining without a fixed depth.
@@ -247,7 +359,9 @@ unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // Edge statistics and expected points per (polygon, top-level cell); needs
 // the level-0 grid and the polygon records on the device.
-int learn_prepare(gahhg_ctx* c) {
+int learn_prepare(gahhg_ctx* c) { return learn_prepare_with(c, c->learn.cost[0]); }
+
+int learn_prepare_with(gahhg_ctx* c, double q_total) {
   LearnState& L = c->learn;
   int rc = ensure_learn(c);
   if (rc) return rc;
@@ -289,7 +403,7 @@ int learn_prepare(gahhg_ctx* c) {
     L.hloc_level = 0;
     hist = L.d_hist;
   }
-  const double q_default = np > 0 ? L.cost[0] / np : 0.0;
+  const double q_default = np > 0 ? q_total / np : 0.0;
   hipLaunchKernelGGL(k_weights, dim3(blocks((int64_t)np * kGrid0Cells, 256)), dim3(256), 0, c->stream, np,
                      (const PolyDev*)c->d_pd, hist, L.hint_weight, q_default,
                      (const unsigned long long*)L.d_estat, L.d_w);
@@ -368,10 +482,83 @@ int learn_features_after_build(gahhg_ctx* c, int level, float* h_x) {
   return rc;
 }
 
+// Features of every (polygon, top-level cell, table candidate) of the built
+// hierarchy into L.d_tx (and h_x when non-null).
+int table_features_after_build(gahhg_ctx* c, float* h_x) {
+  LearnState& L = c->learn;
+  const int np = (int)c->polys.size();
+  int rc = learn_prepare_with(c, L.tcost[0]);
+  if (rc) return rc;
+  const int64_t pr = (int64_t)np * kGrid0Cells;
+  if (L.tcap < np) {
+    dfree(L.d_tx);
+    dfree(L.d_ty);
+    L.d_tx = L.d_ty = nullptr;
+    L.tcap = 0;
+    if (dalloc(c, (void**)&L.d_tx, pr * kTableCand * kMlpIn * 4) || dalloc(c, (void**)&L.d_ty, pr * kTableCand * 4))
+      return GAHHG_E_NOMEM;
+    L.tcap = np;
+  }
+  rc = fill_i32(c, L.d_stat, pr * 2, 0);
+  if (rc) return rc;
+  const LevelTab lt = level_tab(c);
+  if (c->tot_c > 0) {
+    hipLaunchKernelGGL(k_level_stats_last, dim3(blocks(c->tot_c, 256)), dim3(256), 0, c->stream, view(c),
+                       (const PolyDev*)c->d_pd, lt, L.d_stat);
+    rc = hip_check(c, hipGetLastError(), "level stats");
+    if (rc) return rc;
+  }
+  const bool hint = L.d_hint_xyz && L.hint_n > 0;
+  if (hint) {
+    rc = fill_i32(c, (int32_t*)L.d_stripes, pr * kStripes, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_hint_band_last, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
+                       (const PolyDev*)c->d_pd, (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly,
+                       (const int32_t*)L.d_hloc0, L.hint_n, np, pr, L.d_stripes);
+    rc = hip_check(c, hipGetLastError(), "hint band");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_stripes, dim3(blocks(pr, 256)), dim3(256), 0, c->stream, (const uint32_t*)L.d_stripes,
+                       pr, L.d_band);
+    rc = hip_check(c, hipGetLastError(), "hint band sum");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_table_features, dim3(blocks(pr * kTableCand, 256)), dim3(256), 0, c->stream, np,
+                     (const PolyDev*)c->d_pd, (const int32_t*)L.d_stat, (const unsigned long long*)L.d_estat,
+                     (const float*)L.d_w, hint ? (const uint32_t*)L.d_hist : nullptr,
+                     hint ? (const uint32_t*)L.d_band : nullptr, L.d_tx);
+  rc = hip_check(c, hipGetLastError(), "table features");
+  if (rc || !h_x) return rc;
+  rc = hip_check(c, hipMemcpyAsync(h_x, L.d_tx, pr * kTableCand * kMlpIn * 4, hipMemcpyDeviceToHost, c->stream),
+                 "features D2H");
+  return rc ? rc : hip_check(c, hipStreamSynchronize(c->stream), "features sync");
+}
+
+// The learned start-table candidate of every polygon (one synchronisation).
+int table_choose(gahhg_ctx* c, int32_t* h_choice) {
+  LearnState& L = c->learn;
+  const int np = (int)c->polys.size();
+  int rc = table_features_after_build(c, nullptr);
+  if (rc) return rc;
+  const int64_t rows = (int64_t)np * kGrid0Cells * kTableCand;
+  rc = mlp_forward(c, L.d_tx, rows, L.d_tweights, L.d_ty);
+  if (rc) return rc;
+  TableCost tc;
+  for (int k = 0; k < kTableCand; k++) tc.a[k] = L.tcost[2 + 2 * k], tc.b[k] = L.tcost[3 + 2 * k];
+  tc.kbin = L.tcost[2 + 2 * kTableCand];
+  const int ncand = np == 1 ? kTableCand : kTableCandBatch;
+  const int bins_max = np == 1 ? c->prm.query_bins : std::min(c->prm.query_bins, 128);
+  const int sub_max = np == 1 ? c->prm.query_sub : 1;
+  hipLaunchKernelGGL(k_decide_table, dim3(blocks(np, 64)), dim3(64), 0, c->stream, np, (const PolyDev*)c->d_pd,
+                     (const float*)L.d_ty, (const float*)L.d_w, ncand, bins_max, sub_max, L.tcost[1], tc, L.d_dec);
+  rc = hip_check(c, hipGetLastError(), "table decision");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(h_choice, L.d_dec, np * 4, hipMemcpyDeviceToHost, c->stream), "decisions");
+  return rc ? rc : hip_check(c, hipStreamSynchronize(c->stream), "decision sync");
+}
+
 void learn_free(gahhg_ctx* c) {
   LearnState& L = c->learn;
   void* ptrs[] = {L.d_estat, L.d_hist, L.d_band, L.d_w, L.d_stat, L.d_x, L.d_y, L.d_dec, L.d_weights,
-                  L.d_hint_xyz, L.d_hint_poly, L.d_hloc, L.d_hloc0, L.d_stripes};
+                  L.d_hint_xyz, L.d_hint_poly, L.d_hloc, L.d_hloc0, L.d_stripes, L.d_tweights, L.d_tx, L.d_ty};
   for (void* q : ptrs) dfree(q);
   L = LearnState();
 }

@@ -139,13 +139,14 @@ __device__ __forceinline__ int descend_nodes(const int32_t* __restrict__ nodes, 
 }
 
 // Node record of every cell (layout at QueryLayout::nodes); a cell with
-// more than 6 borders (none in this hierarchy) is counted in bad.
+// more than 6 borders (none in a built or validated loaded hierarchy) sets
+// st->bad_nodes, and the layout is refused.
 __global__ void k_pack_nodes(GridView g, int64_t C, int32_t* __restrict__ nodes, DevState* st) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   int32_t* r = nodes + c * kNodeInts;
   const int bs = g.c_bstart[c], bn = g.c_bnum[c];
-  if (bn > 6) atomicAdd(&st->bad_index, 1ull);
+  if (bn > 6) st->bad_nodes = 1;  // any writer: same value
   const int b6 = min(bn, 6);
   const int sc = g.c_subcell[c];
   int nch = 0;
@@ -1148,9 +1149,10 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
 // Sub-tables for the bins whose level-1 entry is not final.  Polygon p gets
 // S_p x S_p sub-bins per such bin: the smallest power of two that makes a
 // sub-bin no wider than a cell of p's bottom level (level-0 cells span about
-// 0.55 rad, halving per level), capped by query_sub and by a budget of
-// query_sub_budget entries per edge of p (S_p = 1: no sub-tables).
-static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
+// 0.55 rad, halving per level), capped by sub_cap (query_sub, or the side of
+// the polygon's start-table candidate) and by a budget of sub_budget entries
+// per edge of p (S_p = 1: no sub-tables).
+static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb, int sub_cap, int sub_budget) {
   QueryLayout& q = c->q;
   const int np = (int)c->polys.size();
   const int B = q.bins;
@@ -1181,9 +1183,9 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb) {
   for (int p = 0; p < np; p++) {
     PolyDev& P = c->pd[p];
     const double cell = 0.55 / (double)(1LL << std::min(P.L, 40));
-    const double budget = (double)c->prm.query_sub_budget * P.N + 4096.0;
+    const double budget = (double)sub_budget * P.N + 4096.0;
     int S = 1;
-    while (S < c->prm.query_sub && bin_angle / S > cell) S *= 2;
+    while (S < sub_cap && bin_angle / S > cell) S *= 2;
     while (S > 1 && (double)K[p] * S * S > budget) S /= 2;
     P.tsub = S;
     if (S > 1) total += (int64_t)K[p] * S * S;
@@ -1347,9 +1349,20 @@ int pack_query_layout(gahhg_ctx* c) {
   if (rc) return rc;
   rc = scan_i32(c, c->d_f0, c->d_o0, c->tot_c, nullptr, &c->d_st->nrec, 0);
   if (rc) return rc;
+  // node records for the descent (one 192-byte record per cell), checked
+  // with the readback below
+  if (use_buf(c, q.b_nodes, nc * kNodeInts * 4, (void**)&q.nodes)) return GAHHG_E_NOMEM;
+  q.bytes_nodes = nc * kNodeInts * 4;
+  rc = fill_i32(c, &c->d_st->bad_nodes, 1, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((c->tot_c + 255) / 256)), dim3(256), 0, c->stream, g,
+                     (int64_t)c->tot_c, q.nodes, c->d_st);
+  rc = hip_check(c, hipGetLastError(), "node records");
+  if (rc) return rc;
   rc = hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream), "readback");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   if (rc) return rc;
+  if (c->h_st->bad_nodes) return set_err(c, GAHHG_E_FORMAT, "query layout: a cell has more than 6 borders");
   q.nrec = std::max<int32_t>(c->h_st->nrec, 1);
   // lanes per edge-test group of each polygon's queries: the smallest power
   // of two >= half its mean edge-stream records per leaf, in [4, 64]
@@ -1380,15 +1393,31 @@ int pack_query_layout(gahhg_ctx* c) {
                      (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf, G);
   rc = hip_check(c, hipGetLastError(), "leaf write");
   if (rc) return rc;
-  // node records for the descent (one 192-byte record per cell)
-  if (use_buf(c, q.b_nodes, nc * kNodeInts * 4, (void**)&q.nodes)) return GAHHG_E_NOMEM;
-  q.bytes += nc * kNodeInts * 4;
-  hipLaunchKernelGGL(k_pack_nodes, dim3((unsigned)((c->tot_c + 255) / 256)), dim3(256), 0, c->stream, g,
-                     (int64_t)c->tot_c, q.nodes, c->d_st);
-  rc = hip_check(c, hipGetLastError(), "node records");
-  if (rc) return rc;
+  q.bytes += q.bytes_nodes;
   rc = build_leaf_index(c);
   if (rc) return rc;
+  // start-table candidate (bins per face side, sub-table side) per polygon:
+  // fixed by the caller, else chosen by the learned table model when one is
+  // set, else (-1) the size rule below.  Every candidate gives the same
+  // labels: a table entry is only a cell that every point of its bin reaches.
+  std::vector<int32_t> cand((size_t)np, -1);
+  if ((int)c->table_fixed.size() == np) cand = c->table_fixed;
+  bool open = false;
+  for (int p = 0; p < np; p++) open |= cand[p] < 0;
+  if (open && c->learn.table_on && c->prm.query_bins > 0) {
+    std::vector<int32_t> chosen((size_t)np, 0);
+    rc = table_choose(c, chosen.data());
+    if (rc) return rc;
+    for (int p = 0; p < np; p++)
+      if (cand[p] < 0) cand[p] = chosen[p];
+  }
+  int sub_cap = c->prm.query_sub, sub_budget = c->prm.query_sub_budget;
+  if (np == 1 && cand[0] >= 0) {
+    q.bins = kTableB[cand[0]];
+    sub_cap = kTableS[cand[0]];
+    sub_budget = std::max(1, sub_cap / 2);
+  }
+  c->table_used.assign((size_t)np * 2, 0);
   if (q.bins > 0) {
     // bins per face side: the requested ones for one polygon; in a batch
     // each polygon gets at most query_table_budget entries per edge (at
@@ -1399,7 +1428,9 @@ int pack_query_layout(gahhg_ctx* c) {
     std::vector<int32_t> ts((size_t)np + 1, 0);
     for (int p = 0; p < np; p++) {
       int Bp = q.bins;
-      if (np > 1 && tb > 0) {
+      if (np > 1 && cand[p] >= 0) {
+        Bp = std::min(kTableB[cand[p]], q.bins);
+      } else if (np > 1 && tb > 0) {
         Bp = 16;
         while (2 * Bp <= q.bins && 6LL * (2 * Bp) * (2 * Bp) <= tb * c->pd[p].N) Bp *= 2;
         Bp = std::min(Bp, q.bins);
@@ -1412,6 +1443,17 @@ int pack_query_layout(gahhg_ctx* c) {
       ts[p + 1] = (int32_t)next;
     }
     const int64_t nb = ts[np];
+    if (nb == 0) {  // no polygon gets a table: the query starts at level 0
+      q.bins = 0;
+      for (int p = 0; p < np; p++) c->pd[p].tbl = -1, c->pd[p].tbins = 0, c->pd[p].tsub = 1;
+      rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
+                                       c->stream),
+                     "polygon upload");
+      if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "polygon upload");
+      if (!rc)
+        for (int p = 0; p < np; p++) c->table_used[2 * p] = 0, c->table_used[2 * p + 1] = 1;
+      return rc;
+    }
     if (use_buf(c, q.b_table, nb * 4, (void**)&q.table) ||
         use_buf(c, q.b_tstart, ((int64_t)np + 1) * 4, (void**)&q.tstart))
       return GAHHG_E_NOMEM;
@@ -1432,9 +1474,11 @@ int pack_query_layout(gahhg_ctx* c) {
     bool pow2 = np > 1;
     for (int p = 0; p < np && pow2; p++) {
       const int B = c->pd[p].tbins;
+      if (B == 0) continue;  // no table for this polygon
       pow2 = B >= 16 && B <= 4096 && (B & (B - 1)) == 0;
       Bmax = std::max(Bmax, B);
     }
+    pow2 = pow2 && Bmax > 0;
     if (pow2) {
       const int64_t nl0 = l0_offset(2 * Bmax);
       if (use_buf(c, q.b_l0, nl0 * 4, (void**)&l0)) return GAHHG_E_NOMEM;
@@ -1448,8 +1492,12 @@ int pack_query_layout(gahhg_ctx* c) {
     rc = hip_check(c, hipGetLastError(), "table build");
     // sub-tables for one polygon only: on the 256-polygon batch they cost
     // 11.5 ms of build for 0.9 ms less per 64M points
-    if (!rc && c->prm.query_sub > 1 && np == 1) rc = build_sub_tables(c, per, nb);
+    if (!rc && sub_cap > 1 && np == 1 && nb > 0) rc = build_sub_tables(c, per, nb, sub_cap, sub_budget);
   }
+  if (!rc && q.bins == 0)
+    for (int p = 0; p < np; p++) c->pd[p].tbins = 0, c->pd[p].tsub = 1;  // no table this time
+  if (!rc)
+    for (int p = 0; p < np; p++) c->table_used[2 * p] = c->pd[p].tbins, c->table_used[2 * p + 1] = c->pd[p].tsub;
   return rc;
 }
 // Spreads the low 16 bits of v to the even bit positions.

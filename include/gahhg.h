@@ -131,6 +131,7 @@ typedef struct {
   int64_t undefined_leaves_final;
   double query_kernel_ms; /* last query: the query kernel alone                */
   double query_sort_ms;   /* last query: grouping of the points before it      */
+  double layout_ms;       /* last build: query layout (part of classify_ms)    */
 } gahhg_stats;
 
 /* Fills defaults (GPISPParam::GPISPParam, source/GPISPParam.h:66-77). */
@@ -306,7 +307,43 @@ int gahhg_mlp_backward(gahhg_ctx* ctx, const float* d_x, int64_t rows, const uin
 int gahhg_mlp_adam(gahhg_ctx* ctx, float* d_master, const float* d_grad, float* d_m1, float* d_m2, int32_t step,
                    float lr, uint16_t* d_w);
 
-/* Learned refinement depth.  With a model set, every polygon without a
+/* Learned start-table depth (label-neutral; no counterpart in the
+ * reference).  The hierarchy -- and with it every label -- stays the density
+ * rule's (source/GPISPAHHG.cu:1595-1605).  What the model chooses per
+ * polygon is the depth of the query's start structure: candidate c of
+ *   bins per cube-face side {0, 16, 32, 64, 128, 256, 256, 256}
+ *   sub-table side          {1,  1,  1,  1,   1,   1,   8,  16}
+ * (a batch considers c <= 4; no candidate above query_bins / query_sub).
+ * A table entry is a cell every point of its bin reaches under the
+ * reference's first-match descent, so every candidate gives the same labels.
+ * For each top-level cell r of the polygon the model predicts y_r(c), the
+ * log2 query time per point (ns) of points in r with candidate c (feature 1
+ * = log2(1 + bins x side) / 12, feature 6 = log2(1 + w_r) / 32; the other 10
+ * as for the depth model, at the polygon's last refined level), and the
+ * build takes
+ *   argmin_c  cost[1] * sum_r w_r 2^y_r(c) + build_c
+ * with w_r the points expected in r (query hint, else cost[0] per polygon
+ * spread by edge share) and build_c = cost[2 + 2c] + cost[3 + 2c] * N ns
+ * for one polygon, cost[18] * 6 * bins_c^2 ns in a batch (whose table
+ * launches are shared).  params: GAHHG_MLP_PARAMS values (rounded to BF16
+ * here); NULL switches the model off (the size rule of query_table_budget
+ * applies).  cost: 19 values.  Takes effect at the next build. */
+int gahhg_set_table_model(gahhg_ctx* ctx, const float* params, const double* cost);
+/* Fixed start-table candidate per polygon (-1: the model's or the size
+ * rule's choice), n = the polygons loaded; NULL or n = 0 clears.  Takes
+ * effect at the next build. */
+int gahhg_set_table_choice(gahhg_ctx* ctx, const int32_t* cand, int32_t n);
+/* The start table of each polygon in the last build: bins_sub[2p] = bins per
+ * face side (0 = none), bins_sub[2p + 1] = sub-table side (1 = none). */
+int gahhg_get_table_layout(gahhg_ctx* ctx, int32_t* bins_sub);
+/* The table model's features of the last build for every (polygon,
+ * top-level cell, candidate): x[((p * 42 + r) * 8 + c) * 12 + f]. */
+int gahhg_table_features(gahhg_ctx* ctx, float* x);
+
+/* Learned refinement depth (changes labels: the reference's labels depend
+ * on the depth of its hierarchy, so a depth other than the density rule's
+ * gives other labels for some points; not used by the default paths).
+ * With a model set, every polygon without a
  * fixed depth is refined level by level while the model's predicted query
  * time saved exceeds the predicted build time of the next level:
  *   sum_r w_r (2^y_r(stop) - 2^y_r(refine)) * cost[1]
