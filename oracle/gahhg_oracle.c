@@ -103,6 +103,9 @@ struct og_ctx {
     int32_t* loc_prev; /* dev_r[1]: start cell at the level before            */
     og_diag diag;
     int built;
+    /* provenance of every cell's type (test hook): rule (OG_PROV_*), the
+     * cell it came from and the border crossed (-1 = none) */
+    int32_t* prov; int32_t* prov_src; int32_t* prov_b; int64_t provcap;
 };
 
 static void set_err(og_ctx* c, const char* msg) {
@@ -117,8 +120,11 @@ og_ctx* og_new(void) {
 static void free_grid(og_ctx* c) {
     free(c->v); free(c->b); free(c->c); free(c->refs);
     free(c->loc_cur); free(c->loc_prev);
+    free(c->prov); free(c->prov_src); free(c->prov_b);
     c->v = NULL; c->b = NULL; c->c = NULL; c->refs = NULL;
     c->loc_cur = c->loc_prev = NULL;
+    c->prov = c->prov_src = c->prov_b = NULL;
+    c->provcap = 0;
     c->vcap = c->bcap = c->ccap = 0;
     c->nref = 0;
     c->gnum = 0;
@@ -1122,6 +1128,15 @@ static int cp_prop(const og_ctx* c, int cell, v3 center) {
     return GRE_ZERO(temp) ? CELL_IN : CELL_OUT;
 }
 
+/* Provenance of a cell's type (test hook; no counterpart in the
+ * reference): which classification rule set it, from which cell, across
+ * which border. */
+enum { OG_PROV_NONE = 0, OG_PROV_NE = 1, OG_PROV_BOTTOM_DE = 2, OG_PROV_IE = 3, OG_PROV_DE_COPY = 4,
+       OG_PROV_DE_PARITY = 5, OG_PROV_TOP = 6 };
+static inline void prov_set(og_ctx* c, int cell, int rule, int src, int border) {
+    c->prov[cell] = rule; c->prov_src[cell] = src; c->prov_b[cell] = border;
+}
+
 /* Parity of crossings from the centre of `from` to the midpoint of border
  * `mb` over the refs of `from` (:1094-1119, :1332-1357). */
 static int parity_type(const og_ctx* c, int from, int mb) {
@@ -1154,7 +1169,9 @@ static int64_t ie_fixpoint(og_ctx* c, int lv, int32_t* snap) {
                 if (pe_ == -1) continue;
                 int pc = c->b[pe_].parent;
                 int pt = snap[pc - G->cstart];
-                if (c->c[pc].rnum == 0 && pt != CELL_UNDEFINED) { cc->type = pt; changed++; break; }
+                if (c->c[pc].rnum == 0 && pt != CELL_UNDEFINED) {
+                    cc->type = pt; prov_set(c, G->cstart + t, OG_PROV_IE, pc, pe_); changed++; break;
+                }
             }
         }
         if (round == 0) {
@@ -1174,11 +1191,25 @@ int og_classify(og_ctx* c) {
     for (int l = 0; l < c->gnum; l++) if (c->g[l].cnum > maxc) maxc = c->g[l].cnum;
     int32_t* snap = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxc + 1));
     int64_t und_snapshot = 0;
+    {
+        const int64_t C = (int64_t)GL->cstart + GL->cnum;
+        if (c->provcap < C) {
+            free(c->prov); free(c->prov_src); free(c->prov_b);
+            c->prov = (int32_t*)malloc(sizeof(int32_t) * (size_t)C);
+            c->prov_src = (int32_t*)malloc(sizeof(int32_t) * (size_t)C);
+            c->prov_b = (int32_t*)malloc(sizeof(int32_t) * (size_t)C);
+            c->provcap = C;
+        }
+        for (int64_t k = 0; k < C; k++) c->prov[k] = OG_PROV_NONE, c->prov_src[k] = -1, c->prov_b[k] = -1;
+    }
     /* kernel_Classify_setBottomNECell :1046-1071 */
 #pragma omp parallel for schedule(dynamic, 64)
     for (int t = 0; t < GL->cnum; t++) {
         int cid = GL->cstart + t;
-        if (c->c[cid].rnum != 0) c->c[cid].type = cp_prop(c, cid, cell_center_sum(c, cid));
+        if (c->c[cid].rnum != 0) {
+            c->c[cid].type = cp_prop(c, cid, cell_center_sum(c, cid));
+            prov_set(c, cid, OG_PROV_NE, -1, -1);
+        }
     }
     /* kernel_Classify_setBottomDECell :1074-1124, snapshot of types */
     for (int t = 0; t < GL->cnum; t++) snap[t] = c->c[GL->cstart + t].type;
@@ -1193,6 +1224,7 @@ int og_classify(og_ctx* c) {
             int nb = c->b[pe_].parent;
             if (snap[nb - GL->cstart] == CELL_UNDEFINED) continue;
             cc->type = parity_type(c, nb, pe_);
+            prov_set(c, cid, OG_PROV_BOTTOM_DE, nb, pe_);
         }
     }
     und_snapshot += ie_fixpoint(c, L, snap);
@@ -1214,8 +1246,13 @@ int og_classify(og_ctx* c) {
             int target = c->b[pe_].subcell;
             if (target < 0) continue;
             while (c->c[target].subcell != -1) target = c->c[target].subcell;
-            if (c->c[target].rnum == 0) cc->type = c->c[target].type;
-            else cc->type = parity_type(c, target, pe_);
+            if (c->c[target].rnum == 0) {
+                cc->type = c->c[target].type;
+                prov_set(c, cid, OG_PROV_DE_COPY, target, pe_);
+            } else {
+                cc->type = parity_type(c, target, pe_);
+                prov_set(c, cid, OG_PROV_DE_PARITY, target, pe_);
+            }
         }
         und_snapshot += ie_fixpoint(c, lv, snap);
     }
@@ -1230,7 +1267,9 @@ int og_classify(og_ctx* c) {
             for (int i = 0; i < cc->bnum; i++) {
                 int pe_ = c->b[cc->bstart + i].partner;
                 if (pe_ != -1 && snap[c->b[pe_].parent] != CELL_UNDEFINED) {
-                    cc->type = snap[c->b[pe_].parent]; changed++; break;
+                    cc->type = snap[c->b[pe_].parent];
+                    prov_set(c, t, OG_PROV_TOP, c->b[pe_].parent, pe_);
+                    changed++; break;
                 }
             }
         }
@@ -1371,6 +1410,25 @@ void og_export(og_ctx* c, double* v, int32_t* cells, int32_t* bi, double* bn, in
         if (bn) { bn[3 * i] = b->n[0]; bn[3 * i + 1] = b->n[1]; bn[3 * i + 2] = b->n[2]; }
     }
     if (refs) memcpy(refs, c->refs, sizeof(int32_t) * (size_t)cnt[4]);
+}
+
+/* Provenance of every cell's type (rule, source cell, border crossed; -1 =
+ * none) after classification. */
+void og_provenance(og_ctx* c, int32_t* rule, int32_t* src, int32_t* border) {
+    int64_t cnt[5];
+    og_counts(c, cnt);
+    for (int64_t i = 0; i < cnt[3]; i++) {
+        rule[i] = c->prov ? c->prov[i] : OG_PROV_NONE;
+        src[i] = c->prov ? c->prov_src[i] : -1;
+        border[i] = c->prov ? c->prov_b[i] : -1;
+    }
+}
+
+/* Overwrites one cell's type: plants a mistake for tests of the checkers. */
+void og_debug_set_type(og_ctx* c, int32_t cell, int32_t type) {
+    int64_t cnt[5];
+    og_counts(c, cnt);
+    if (cell >= 0 && cell < cnt[3]) c->c[cell].type = type;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -113,6 +113,14 @@ void og_query_xyz(og_ctx* c, const double* xyz, int64_t n, int32_t* labels,
 /* Leaf cell (index into the exported cells, -1 = unlocated) that the
  * first-match descent of kernel_PISP reaches for each point. */
 void og_query_leaf(og_ctx* c, const double* xyz, int64_t n, int32_t* leaf);
+/* Provenance of every cell's type after og_classify: rule[i] = 0 none,
+ * 1 setBottomNECell (getCPProp), 2 setBottomDECell (parity from src across
+ * border), 3 setIECell (adopted from src), 4 setDECell copy of src, 5
+ * setDECell parity from src across border, 6 setTopECell (adopted from
+ * src). */
+void og_provenance(og_ctx* c, int32_t* rule, int32_t* src, int32_t* border);
+/* Test hook: overwrites a cell's type (plants a mistake). */
+void og_debug_set_type(og_ctx* c, int32_t cell, int32_t type);
 
 /* Exact FP64 primitives exposed for known-answer tests. */
 void og_kat_normalize(const double* in3, double* out3);

@@ -78,6 +78,10 @@ def lib():
                                    ctypes.POINTER(OgQCount)]
         L.og_query_leaf.argtypes = [P, dp, ctypes.c_int64, ip]
         L.og_query_leaf.restype = None
+        L.og_provenance.argtypes = [P, ip, ip, ip]
+        L.og_provenance.restype = None
+        L.og_debug_set_type.argtypes = [P, ctypes.c_int32, ctypes.c_int32]
+        L.og_debug_set_type.restype = None
         L.og_kat_normalize.argtypes = [dp, dp]
         L.og_kat_cross.argtypes = [dp, dp, dp]
         L.og_kat_is_ee_inter.argtypes = [dp, dp, dp, dp]
@@ -223,6 +227,18 @@ class Oracle:
         if with_counts:
             return labels, dict((f, getattr(q, f)) for f, _ in OgQCount._fields_)
         return labels
+
+    def provenance(self):
+        """(rule, source cell, border) per cell: which classification rule
+        typed it (0 none, 1 setBottomNECell, 2 setBottomDECell, 3 setIECell,
+        4 setDECell copy, 5 setDECell parity, 6 setTopECell)."""
+        C = self.counts()["C"]
+        r, s, b = (np.zeros(C, dtype=np.int32) for _ in range(3))
+        self.L.og_provenance(self.h, r, s, b)
+        return r, s, b
+
+    def debug_set_type(self, cell, t):
+        self.L.og_debug_set_type(self.h, int(cell), int(t))
 
     def query_leaf(self, xyz):
         """Leaf cell the first-match descent reaches for each point (-1 = none)."""
