@@ -142,19 +142,26 @@ def test_query_csv_reader(tmp_path):
 
 # ---------------------------------------------------------------- depth / space
 def test_depths_match_reference_sweep():
+    """The density rule reproduces the reference's hand-picked /maxdepth of
+    test/testAHHG.bat on every dataset of test/polygon/various and the
+    Tennessee simplifications shipped here, except NA2: its crossed-cell
+    ratio sits on the stop threshold and the rule refines once more (7 vs
+    6)."""
     d = json.load(open(refpath("ahhg_maxdepth.json")))["maxdepth"]
-    checked = 0
+    various = os.path.join(os.path.dirname(refpath("x")), "..", "various")
+    got = {}
     for key, depth in d.items():
         fname = {"various/Square_72": "Square_72.txt", "various/Radiometer": "Radiometer.txt",
-                 "various/NA0": "NA0.txt"}.get(key)
+                 "various/NA0": "NA0.txt", "various/Tennessee": "Tennessee.txt"}.get(key)
         if key.startswith("Tennessee/"):
             fname = key.replace("/", "_") + ".txt"
-        if fname is None:
-            continue
-        o = ol.Oracle().load_txt(refpath(fname)).build()
-        assert o.counts()["depth"] == depth, key
-        checked += 1
-    assert checked >= 9
+        path = refpath(fname) if fname else os.path.join(various, key.split("/")[1] + ".txt")
+        o = ol.Oracle().load_txt(path).build()
+        got[key] = o.counts()["depth"]
+    assert len(got) == 20
+    differ = {k: (got[k], d[k]) for k in d if got[k] != d[k]}
+    assert differ == {"various/NA2": (7, 6)}, differ
+    assert sum(k.startswith("various/") for k in got) == 13
 
 
 def test_na0_report_figures():
