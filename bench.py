@@ -140,6 +140,7 @@ def main():
     ap.add_argument("--no-model", action="store_true", help="library default start tables (no learned model)")
     ap.add_argument("--no-graph", action="store_true", help="launch each query pass kernel by kernel")
     ap.add_argument("--no-parity", action="store_true", help="skip the full oracle label check")
+    ap.add_argument("--no-ab", action="store_true", help="skip the learned-vs-default table A/B")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size reference for "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -203,7 +204,7 @@ def main():
     # the same hierarchy with the library's default start table against the
     # learned one, for 1M, 10M and all points (build + one pass)
     ab = None
-    if model is not None:
+    if model is not None and not args.no_ab:
         ab = {}
         for n in sorted({min(Q, 1_000_000), min(Q, 10_000_000), Q}):
             row = {}

@@ -77,7 +77,9 @@ void free_scratch(gahhg_ctx* c) {
   c->d_f0 = c->d_f1 = c->d_o0 = c->d_o1 = c->d_scan = c->d_ecnt = nullptr;
   c->d_f2 = c->d_f3 = c->d_o2 = c->d_o3 = nullptr;
   c->d_rcell = c->d_rflag = c->d_redge = c->d_rscan = c->d_loc_cur = c->d_loc_prev = nullptr;
-  c->fcap = c->scap = c->ecap = c->r2cap = 0;
+  dfree(c->d_wrec);
+  c->d_wrec = nullptr;
+  c->fcap = c->scap = c->ecap = c->r2cap = c->wrec_cap = 0;
 }
 
 int upload_polygons(gahhg_ctx* c) {

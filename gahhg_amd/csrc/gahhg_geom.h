@@ -93,11 +93,18 @@ GH_HD V3 arc_normal(V3 a, V3 b) { return unit(cross(a, sub(b, a))); }
 // Border-plane vs edge classification given the precomputed border normal nb
 // (= arc_normal(v0, v1)) and edge normal ne (= arc_normal(p0, p1)); the same
 // values device_isEdgeBorderInter (source/GPISPAHHG.cu:226-284) computes.
+GH_HD int edge_border_class_r(double r0, double r1, double r2, double r3, int* reserved);
 GH_HD int edge_border_class(V3 nb, V3 ne, V3 p0, V3 p1, V3 v0, V3 v1, int* reserved) {
   double r0 = dot(nb, p0);
   double r1 = dot(nb, p1);
   double r2 = dot(ne, v0);
   double r3 = dot(ne, v1);
+  return edge_border_class_r(r0, r1, r2, r3, reserved);
+}
+
+// The classification from the four plane values r0 = nb.p0, r1 = nb.p1,
+// r2 = ne.v0, r3 = ne.v1.
+GH_HD int edge_border_class_r(double r0, double r1, double r2, double r3, int* reserved) {
   if (eq0(r0) && eq0(r1)) return kInterOverlap;
   bool s23 = same_side(r2, r3);
   bool s01 = same_side(r0, r1);

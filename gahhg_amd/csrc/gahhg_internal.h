@@ -162,6 +162,27 @@ struct DeviceGrid {
   int32_t* refs = nullptr;
 };
 
+// What one step of the build's edge walk reads about a cell, in one record:
+// its border range and, per border, the plane normal, the first end vertex,
+// the partner border and the partner's cell -- the same doubles as the grid
+// arrays, so the walk's results are unchanged.  A cell's borders form a
+// closed loop (border k ends where border k + 1 starts), so border k's second
+// vertex is border k + 1's first; a cell whose loop does not close gets
+// kWalkRecOpen in its bnum word and is walked on the grid arrays.  A step
+// then costs one record fetch instead of a chain of dependent gathers
+// through the border and vertex arrays.
+struct WalkBorder {
+  double n[3], v0[3];
+  int32_t partner, pparent;
+};
+struct alignas(16) WalkRec {
+  int32_t bstart, bnum;
+  WalkBorder b[6];
+  int32_t pad[2];
+};
+static_assert(sizeof(WalkRec) == 352, "walk record layout");
+constexpr int32_t kWalkRecOpen = 0x100;
+
 // Raw pointer bundle passed to kernels by value.
 struct GridView {
   const double* v;
@@ -476,6 +497,8 @@ struct gahhg_ctx {
   int32_t* d_ecnt = nullptr;  // per edge
   int64_t r2cap = 0;
   int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr, *d_rscan = nullptr;
+  gahhg::WalkRec* d_wrec = nullptr;  // walk record of every cell (build scratch)
+  int64_t wrec_cap = 0;
 
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
