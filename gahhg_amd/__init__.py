@@ -83,7 +83,8 @@ EXPORTED = [
     "gahhg_load_polygons_txt", "gahhg_load_query_csv_device", "gahhg_parse_query_csv_device",
     "gahhg_load_polygon_txt_device", "gahhg_sort_pairs", "gahhg_query_pairs_xyz", "gahhg_autotune_layout",
     "gahhg_query_batch_xyz_device_noalloc", "gahhg_set_table_model", "gahhg_set_table_choice",
-    "gahhg_get_table_layout", "gahhg_table_features",
+    "gahhg_get_table_layout", "gahhg_table_features", "gahhg_read_lune_points", "gahhg_read_lune_polygon",
+    "gahhg_load_polygon_lune", "gahhg_kat_cp_prop",
 ]
 
 # Start-table candidates of the learned table depth (bins per face side,
@@ -172,6 +173,10 @@ def lib():
         "gahhg_autotune_query": ([P, P, P, i64, P, P], ctypes.c_int),
         "gahhg_save_grid": ([P, ctypes.c_char_p], ctypes.c_int),
         "gahhg_load_grid": ([P, ctypes.c_char_p], ctypes.c_int),
+        "gahhg_read_lune_points": ([ctypes.c_char_p, P, P, i64, ctypes.POINTER(i64)], ctypes.c_int),
+        "gahhg_read_lune_polygon": ([ctypes.c_char_p, P, P, i64, ctypes.POINTER(i64), dp, ip], ctypes.c_int),
+        "gahhg_load_polygon_lune": ([P, ctypes.c_char_p, dp, ip], ctypes.c_int),
+        "gahhg_kat_cp_prop": ([P, ip, i64, ip, dp, i64, i32, ip, dp], ctypes.c_int),
         "gahhg_load_polygons_txt": ([P, ctypes.POINTER(ctypes.c_char_p), i32], ctypes.c_int),
         "gahhg_load_query_csv_device": ([P, ctypes.c_char_p, P, P, P], ctypes.c_int),
         "gahhg_parse_query_csv_device": ([P, ctypes.c_char_p, i64, P, P, P], ctypes.c_int),
@@ -219,6 +224,42 @@ def read_query_csv(path):
     if rc:
         raise GahhgError(rc, f"read {path}")
     return lon, lat
+
+
+def read_lune_points(path):
+    """Lune/Ketzner-format points (test/data_for_Lune) -> lon, lat degrees
+    in the reference's convention, read by the library."""
+    L = lib()
+    n = ctypes.c_int64(0)
+    rc = L.gahhg_read_lune_points(str(path).encode(), None, None, 0, ctypes.byref(n))
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    lon = np.zeros(n.value)
+    lat = np.zeros(n.value)
+    rc = L.gahhg_read_lune_points(str(path).encode(), lon.ctypes.data, lat.ctypes.data, n.value, ctypes.byref(n))
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    return lon, lat
+
+
+def read_lune_polygon(path):
+    """A Lune/Ketzner-format polygon -> (lon, lat, ring_off, ref) read by
+    the library: one ring with its closing vertex, and the point the file
+    gives as inside (lon, lat), or None."""
+    L = lib()
+    n = ctypes.c_int64(0)
+    ref = np.zeros(2)
+    has = np.zeros(1, dtype=np.int32)
+    rc = L.gahhg_read_lune_polygon(str(path).encode(), None, None, 0, ctypes.byref(n), ref, has)
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    lon = np.zeros(n.value)
+    lat = np.zeros(n.value)
+    rc = L.gahhg_read_lune_polygon(str(path).encode(), lon.ctypes.data, lat.ctypes.data, n.value,
+                                   ctypes.byref(n), ref, has)
+    if rc:
+        raise GahhgError(rc, f"read {path}")
+    return lon, lat, np.array([0, n.value], dtype=np.int64), ((float(ref[0]), float(ref[1])) if has[0] else None)
 
 
 def write_result_csv(path, labels):
@@ -321,6 +362,14 @@ class GPISP:
         return self
 
     loadPolyFromTXT = load_polygon_txt
+
+    def load_polygon_lune(self, path):
+        """A Lune/Ketzner-format polygon file; returns the point the file
+        gives as inside (lon, lat degrees) or None."""
+        ref = np.zeros(2)
+        has = np.zeros(1, dtype=np.int32)
+        self._check(self.L.gahhg_load_polygon_lune(self.h, str(path).encode(), ref, has))
+        return (float(ref[0]), float(ref[1])) if has[0] else None
 
     def load_polygon_txt_device(self, path):
         """load_polygon_txt with the vertex lines parsed on the GPU; the load
@@ -635,6 +684,18 @@ class GPISP:
         nb = k.size if n_bound is None else int(n_bound)
         self._check(self.L.gahhg_sort_pairs(self.h, k.ctypes.data, v.ctypes.data, k.size, nb, end_bit))
         return k, v
+
+    def kat_cp_prop(self, refs, rs_rn, centers, group):
+        """getCPProp's choice per case (see gahhg_kat_cp_prop): int32 (n, 3)
+        type / slot / vertex and float64 (n,) squared distances."""
+        refs = np.ascontiguousarray(refs, dtype=np.int32)
+        rs_rn = np.ascontiguousarray(rs_rn, dtype=np.int32).reshape(-1)
+        centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
+        n = rs_rn.size // 2
+        out = np.zeros(3 * n, dtype=np.int32)
+        md = np.zeros(n)
+        self._check(self.L.gahhg_kat_cp_prop(self.h, refs, refs.size, rs_rn, centers, n, int(group), out, md))
+        return out.reshape(n, 3), md
 
     def kat(self, op, inputs):
         inputs = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 12)

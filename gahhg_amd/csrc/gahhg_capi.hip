@@ -362,6 +362,31 @@ int gahhg_load_polygon_txt(gahhg_ctx* c, const char* path) {
   return upload_polygons(c);
 }
 
+int gahhg_load_polygon_lune(gahhg_ctx* c, const char* path, double* ref_lonlat, int32_t* has_ref) {
+  RCK(ctx_ok(c));
+  if (!path) return set_err(c, GAHHG_E_ARG, "null path");
+  RCK(select_device(c));
+  PolygonText t;
+  double ref[2] = {0.0, 0.0};
+  bool hr = false;
+  RCK(parse_polygon_lune(path, &t, ref, &hr, &c->err));
+  std::vector<double> xyz;
+  if ((int64_t)t.lo.size() >= kGpuConvertMin) {
+    xyz.resize(3 * t.lo.size());
+    RCK(lonlat_to_xyz_gpu(c, t.lo.data(), t.la.data(), (int64_t)t.lo.size(), xyz.data()));
+  }
+  Polygon p;
+  RCK(polygon_from_text(t, xyz.empty() ? nullptr : xyz.data(), &p, &c->err));
+  c->polys.clear();
+  c->polys.push_back(std::move(p));
+  if (ref_lonlat) {
+    ref_lonlat[0] = ref[0];
+    ref_lonlat[1] = ref[1];
+  }
+  if (has_ref) *has_ref = hr ? 1 : 0;
+  return upload_polygons(c);
+}
+
 int gahhg_load_query_csv_device(gahhg_ctx* c, const char* path, const double** d_xyz, int64_t* n,
                                 gahhg_load_info* info) {
   RCK(ctx_ok(c));
@@ -1238,6 +1263,47 @@ int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap
   return GAHHG_OK;
 }
 
+int gahhg_read_lune_points(const char* path, double* lon, double* lat, int64_t cap, int64_t* n) {
+  if (!path || !n) return GAHHG_E_ARG;
+  std::vector<double> a, b;
+  std::string err;
+  int rc = read_lune_points(path, &a, &b, &err);
+  if (rc) return rc;
+  *n = (int64_t)a.size();
+  if (lon && lat) {
+    if (cap < (int64_t)a.size()) return GAHHG_E_ARG;
+    std::copy(a.begin(), a.end(), lon);
+    std::copy(b.begin(), b.end(), lat);
+  }
+  return GAHHG_OK;
+}
+
+int gahhg_read_lune_polygon(const char* path, double* lon, double* lat, int64_t cap, int64_t* n,
+                            double* ref_lonlat, int32_t* has_ref) {
+  if (!path || !n) return GAHHG_E_ARG;
+  PolygonText t;
+  double ref[2] = {0.0, 0.0};
+  bool hr = false;
+  std::string err;
+  int rc = parse_polygon_lune(path, &t, ref, &hr, &err);
+  if (rc) return rc;
+  const int64_t m = (int64_t)t.lo.size() + 1;  // with the closing vertex
+  *n = m;
+  if (lon && lat) {
+    if (cap < m) return GAHHG_E_ARG;
+    std::copy(t.lo.begin(), t.lo.end(), lon);
+    std::copy(t.la.begin(), t.la.end(), lat);
+    lon[m - 1] = t.lo[0];
+    lat[m - 1] = t.la[0];
+  }
+  if (ref_lonlat) {
+    ref_lonlat[0] = ref[0];
+    ref_lonlat[1] = ref[1];
+  }
+  if (has_ref) *has_ref = hr ? 1 : 0;
+  return GAHHG_OK;
+}
+
 int gahhg_get_stats(gahhg_ctx* c, gahhg_stats* out) {
   RCK(ctx_ok(c));
   if (!out) return GAHHG_E_ARG;
@@ -1440,6 +1506,45 @@ int gahhg_kat_device(gahhg_ctx* c, int32_t op, const double* in12, int64_t n, do
   if (!rc) rc = hip_check(c, hipMemcpy(out3, dout, n * 24, hipMemcpyDeviceToHost), "kat D2H");
   dfree(di);
   dfree(dout);
+  return rc;
+}
+
+int gahhg_kat_cp_prop(gahhg_ctx* c, const int32_t* refs, int64_t n_refs, const int32_t* rs_rn,
+                      const double* centers, int64_t n, int32_t group, int32_t* out3, double* mindist) {
+  RCK(ctx_ok(c));
+  if (n < 0 || n > INT32_MAX / 3 || n_refs < 0 || (n > 0 && (!refs || !rs_rn || !centers || !out3 || !mindist)))
+    return set_err(c, GAHHG_E_ARG, "kat cp_prop: bad arguments");
+  if (group < 1 || group > 64 || (group & (group - 1)) != 0)
+    return set_err(c, GAHHG_E_ARG, "kat cp_prop: group must be a power of two up to 64");
+  // the edge records (end points, plane normal) are made by the build
+  if (!c->built || c->n_edges <= 0 || !c->d_edge) return set_err(c, GAHHG_E_STATE, "kat cp_prop before build");
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t rs = rs_rn[2 * k], rn = rs_rn[2 * k + 1];
+    if (rs < 0 || rn < 1 || rs + rn > n_refs) return set_err(c, GAHHG_E_ARG, "kat cp_prop: case %lld refs out of range", (long long)k);
+  }
+  for (int64_t i = 0; i < n_refs; i++)
+    if (refs[i] < 0 || refs[i] >= c->n_edges) return set_err(c, GAHHG_E_ARG, "kat cp_prop: ref %lld is no edge", (long long)i);
+  if (n == 0) return GAHHG_OK;
+  RCK(select_device(c));
+  int32_t *dr = nullptr, *drn = nullptr, *dout = nullptr;
+  double *dc = nullptr, *dm = nullptr;
+  int rc = dalloc(c, (void**)&dr, std::max<int64_t>(n_refs, 1) * 4);
+  if (!rc) rc = dalloc(c, (void**)&drn, n * 8);
+  if (!rc) rc = dalloc(c, (void**)&dc, n * 24);
+  if (!rc) rc = dalloc(c, (void**)&dout, n * 12);
+  if (!rc) rc = dalloc(c, (void**)&dm, n * 8);
+  if (!rc) rc = hip_check(c, hipMemcpy(dr, refs, n_refs * 4, hipMemcpyHostToDevice), "kat H2D");
+  if (!rc) rc = hip_check(c, hipMemcpy(drn, rs_rn, n * 8, hipMemcpyHostToDevice), "kat H2D");
+  if (!rc) rc = hip_check(c, hipMemcpy(dc, centers, n * 24, hipMemcpyHostToDevice), "kat H2D");
+  if (!rc) rc = run_kat_cp_prop(c, dr, drn, dc, n, group, dout, dm);
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "kat sync");
+  if (!rc) rc = hip_check(c, hipMemcpy(out3, dout, n * 12, hipMemcpyDeviceToHost), "kat D2H");
+  if (!rc) rc = hip_check(c, hipMemcpy(mindist, dm, n * 8, hipMemcpyDeviceToHost), "kat D2H");
+  dfree(dr);
+  dfree(drn);
+  dfree(dc);
+  dfree(dout);
+  dfree(dm);
   return rc;
 }
 

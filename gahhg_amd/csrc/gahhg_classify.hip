@@ -54,75 +54,6 @@ __device__ __forceinline__ V3 mean_center(const GridView& g, int cid) {
   return unit(c);
 }
 
-// device_getCPProp, :61-171: side of the centre relative to the nearest
-// polygon feature among the cell's refs.
-__device__ int cp_prop(const GridView& g, const double* __restrict__ pv, const int32_t* __restrict__ pe,
-                       const double* __restrict__ edge, int cell, V3 center) {
-  double mindist = FLT_MAX;
-  const int rs = g.c_rstart[cell], rn = g.c_rnum[cell];
-  int ne = g.refs[rs];
-  V3 ne_normal = v3(0.0, 0.0, 0.0);
-  int nv_index = -1;
-  for (int i = 0; i < rn; i++) {
-    const int se = g.refs[rs + i];
-    const double* er = edge + 9 * (int64_t)se;
-    const V3 v0 = load3(er), v1 = load3(er + 3), n0 = load3(er + 6);
-    double curdist;
-    V3 n1 = unit(cross(center, n0));
-    V3 inter = unit(cross(n0, n1));
-    if (lt0(dot(inter, v0))) inter = neg(inter);
-    V3 t0 = unit(sub(v0, inter));
-    V3 t1 = unit(sub(v1, inter));
-    const double t3 = dot(t0, t1);
-    int cur_nv;
-    if (lt0(t3)) {
-      curdist = sqlen(sub(inter, center));
-      cur_nv = -1;
-    } else {
-      curdist = sqlen(sub(center, v0));
-      const double vd = sqlen(sub(center, v1));
-      if (vd < curdist) {
-        curdist = vd;
-        cur_nv = pe[2 * se + 1];
-      } else {
-        cur_nv = pe[2 * se];
-      }
-    }
-    if (nv_index != -1 && nv_index == cur_nv) {
-      double a = dot(ne_normal, center);
-      double b = dot(n0, center);
-      if ((le0(a) && ge0(b)) || (ge0(a) && le0(b))) {
-        // both nearest features are the shared vertex: keep the edge whose
-        // far end leans more towards the centre
-        const int oth0 = (pe[2 * ne] == nv_index) ? pe[2 * ne + 1] : pe[2 * ne];
-        const V3 pnv = load3(pv + 3 * (int64_t)nv_index);
-        V3 a0 = sub(load3(pv + 3 * (int64_t)oth0), pnv);
-        const int oth1 = (pe[2 * se] == nv_index) ? pe[2 * se + 1] : pe[2 * se];
-        V3 a1 = sub(load3(pv + 3 * (int64_t)oth1), pnv);
-        V3 bb = sub(center, pnv);
-        a0 = unit(a0);
-        a1 = unit(a1);
-        a = dot(a0, bb);
-        b = dot(a1, bb);
-        if (b > a) {
-          mindist = curdist;
-          ne = se;
-          ne_normal = n0;
-          nv_index = cur_nv;
-          continue;
-        }
-      }
-    }
-    if (curdist < mindist) {
-      mindist = curdist;
-      ne = se;
-      ne_normal = n0;
-      nv_index = cur_nv;
-    }
-  }
-  return gt0(dot(center, ne_normal)) ? kIn : kOut;
-}
-
 // Crossing parity from the centre of `from` to the midpoint of border `mb`
 // over the refs of `from` (dev_isInterSeg, :174-205), giving the label on
 // the far side.
@@ -150,13 +81,6 @@ __device__ __forceinline__ int cell_phase(const GridView& g, const PolyDev* pd, 
   return L == lv ? 0 : (L > lv ? 1 : 2);
 }
 
-// setBottomNECell with a group of G lanes per cell (G a power of two up to
-// 64, chosen from the level's mean refs per leaf): the lanes evaluate the
-// nearest feature of the cell's refs in parallel (the per-ref part of
-// cp_prop), then every lane of the group replays cp_prop's selection over
-// those values in ref order, so the chosen feature -- including the
-// shared-vertex tie rule, whose comparison needs only each ref's own values
-// -- is exactly cp_prop's.  Conditions below are uniform within a group.
 __device__ __forceinline__ double shfl_d(double v, int src) {
   return __hiloint2double(__shfl(__double2hiint(v), src, 64), __shfl(__double2loint(v), src, 64));
 }
@@ -176,18 +100,25 @@ __device__ __forceinline__ Group group_of(int G) {
   return r;
 }
 
-__global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __restrict__ pd, const double* pv,
-                                                const int32_t* pe, const double* edge, int cstart, int cnum, int lv,
-                                                int G) {
-  const Group gr = group_of(G);
-  const int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
-  if (w >= cnum) return;
-  const int cid = cstart + w;
-  if (cell_phase(g, pd, cid, lv) != 0) return;
-  const int rn = g.c_rnum[cid];
-  if (rn == 0) return;
-  const V3 center = mean_center(g, cid);
-  const int rs = g.c_rstart[cid];
+// The feature device_getCPProp (:61-171) picks among one cell's refs.
+struct CpChoice {
+  int slot;        // position in the refs of the chosen edge; -1: none chosen
+  int nv;          // its nearest polygon vertex, -1 when the nearest point is inside the edge
+  double mindist;  // squared chord distance from the centre to that point
+  double side;     // dot(edge normal, centre) of the chosen edge
+};
+
+// device_getCPProp's loop over refs[0, rn) with a group of G lanes (G a
+// power of two up to 64): the lanes evaluate the nearest feature of the
+// refs in parallel (the per-ref part of the loop), then every lane of the
+// group replays the reference's selection over those values in ref order,
+// so the chosen feature -- including the shared-vertex tie rule, whose
+// comparison needs only each ref's own values -- is the reference's.
+// Conditions are uniform within a group; every lane returns the choice.
+__device__ __forceinline__ CpChoice cp_select(const int32_t* __restrict__ refs, int rn,
+                                              const double* __restrict__ pv, const int32_t* __restrict__ pe,
+                                              const double* __restrict__ edge, V3 center, const Group& gr) {
+  const int G = gr.G;
   double mindist = FLT_MAX;
   int best = -1;            // ref slot of the chosen feature
   double best_side = 0.0;   // dot(n0, center) of the chosen edge
@@ -198,7 +129,7 @@ __global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __res
     double curdist = 0.0, side = 0.0, lean = 0.0;
     int cur_nv = -1;
     if (i < rn) {
-      const int se = g.refs[rs + i];
+      const int se = refs[i];
       const double* er = edge + 9 * (int64_t)se;
       const V3 v0 = load3(er), v1 = load3(er + 3), n0 = load3(er + 6);
       V3 n1 = unit(cross(center, n0));
@@ -222,6 +153,8 @@ __global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __res
       }
       side = dot(n0, center);
       if (cur_nv != -1) {
+        // the tie rule's lean of this edge: its far end towards the centre,
+        // seen from the shared vertex
         const int oth = (pe[2 * se] == cur_nv) ? pe[2 * se + 1] : pe[2 * se];
         const V3 pnv = load3(pv + 3 * (int64_t)cur_nv);
         const V3 a1 = unit(sub(load3(pv + 3 * (int64_t)oth), pnv));
@@ -255,11 +188,48 @@ __global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __res
       }
     }
   }
+  return CpChoice{best, nv_index, mindist, best_side};
+}
+
+// The type getCPProp gives: inside when the centre is on the inner side of
+// the chosen edge.  With no feature chosen (all distances NaN) the reference
+// keeps a zero normal, whose side test fails.
+__device__ __forceinline__ int cp_type(const CpChoice& ch) { return gt0(ch.slot >= 0 ? ch.side : 0.0) ? kIn : kOut; }
+
+// setBottomNECell (:1046-1071): a group of G lanes per bottom boundary leaf
+// (G chosen from the level's mean refs per leaf), typed by getCPProp at the
+// leaf's centre.
+__global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __restrict__ pd, const double* pv,
+                                                const int32_t* pe, const double* edge, int cstart, int cnum, int lv,
+                                                int G) {
+  const Group gr = group_of(G);
+  const int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  if (w >= cnum) return;
+  const int cid = cstart + w;
+  if (cell_phase(g, pd, cid, lv) != 0) return;
+  const int rn = g.c_rnum[cid];
+  if (rn == 0) return;
+  const V3 center = mean_center(g, cid);
+  const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pv, pe, edge, center, gr);
+  if (gr.gl == 0) g.c_type[cid] = cp_type(ch);
+}
+
+// Known-answer hook: getCPProp's choice at given centres over given ref
+// lists (case k: refs[rs_rn[2k], + rs_rn[2k+1]), centre centers[3k..]).
+__global__ void __launch_bounds__(kCBlock) k_kat_cp_prop(const double* pv, const int32_t* pe, const double* edge,
+                                                         const int32_t* refs, const int32_t* rs_rn,
+                                                         const double* centers, int n, int G, int32_t* out3,
+                                                         double* mindist) {
+  const Group gr = group_of(G);
+  const int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  if (w >= n) return;
+  const V3 center = load3(centers + 3 * (int64_t)w);
+  const CpChoice ch = cp_select(refs + rs_rn[2 * w], rs_rn[2 * w + 1], pv, pe, edge, center, gr);
   if (gr.gl == 0) {
-    // no feature chosen (all distances NaN): the reference keeps a zero
-    // normal, whose side test fails
-    const double s = best >= 0 ? best_side : 0.0;
-    g.c_type[cid] = gt0(s) ? kIn : kOut;
+    out3[3 * w] = cp_type(ch);
+    out3[3 * w + 1] = ch.slot;
+    out3[3 * w + 2] = ch.nv;
+    mindist[w] = ch.mindist;
   }
 }
 
@@ -444,6 +414,15 @@ static int ie_fixpoint(gahhg_ctx* c, int lv, int32_t* snap, unsigned long long* 
     if (!c->h_st->changed) return GAHHG_OK;
   }
   return set_err(c, GAHHG_E_STATE, "classification did not converge");
+}
+
+int run_kat_cp_prop(gahhg_ctx* c, const int32_t* d_refs, const int32_t* d_rs_rn, const double* d_centers, int64_t n,
+                    int G, int32_t* d_out3, double* d_mindist) {
+  if (n <= 0) return GAHHG_OK;
+  hipLaunchKernelGGL(k_kat_cp_prop, dim3(gblocks((int)n, G)), dim3(kCBlock), 0, c->stream, (const double*)c->d_pv,
+                     (const int32_t*)c->d_pe, (const double*)c->d_edge, d_refs, d_rs_rn, d_centers, (int)n, G, d_out3,
+                     d_mindist);
+  return hip_check(c, hipGetLastError(), "kat cp_prop kernel");
 }
 
 int classify_hierarchy(gahhg_ctx* c) {

@@ -261,6 +261,76 @@ int read_query_csv(const char* path, std::vector<double>* lon, std::vector<doubl
   return GAHHG_OK;
 }
 
+// Lune/Ketzner-format files.  The reference ships copies of its polygons and
+// query points in the format of the comparators it was measured against
+// (README.md:5, test/data_for_Lune): one "colatitude,longitude" pair per
+// line, radians, the longitude counted from 180 deg West, so that
+//   lat = 90 - deg(colatitude),  lon = (deg(longitude) mod 360) - 180
+// with deg(x) = x * (180 / pi) and the mod of Python / numpy (the result
+// takes the divisor's sign: fmod, plus 360 when fmod is negative).  Under
+// this rule the format's Square_72, Radiometer and Tennessee are the TXT
+// files' polygons.
+namespace {
+constexpr double kDegPerRad = 180.0 / 3.14159265358979323846264338327950288;
+
+double lune_lat(double colat) { return 90.0 - colat * kDegPerRad; }
+
+double lune_lon(double lon) {
+  double m = fmod(lon * kDegPerRad, 360.0);
+  if (m != 0.0) {
+    if (m < 0.0) m += 360.0;
+  } else {
+    m = 0.0;  // +0 whatever the sign of the dividend
+  }
+  return m - 180.0;
+}
+}  // namespace
+
+int read_lune_points(const char* path, std::vector<double>* lon, std::vector<double>* lat, std::string* err) {
+  std::vector<double> colat, lo;
+  const int rc = read_query_csv(path, &colat, &lo, err);  // the same "%lf,%lf" lines
+  if (rc) return rc;
+  lon->resize(lo.size());
+  lat->resize(lo.size());
+  for (size_t i = 0; i < lo.size(); i++) {
+    (*lon)[i] = lune_lon(lo[i]);
+    (*lat)[i] = lune_lat(colat[i]);
+  }
+  return GAHHG_OK;
+}
+
+// A polygon file is one ring, closed by repeating its first vertex (the
+// format has no holes), then the point the comparators use as known inside.
+int parse_polygon_lune(const char* path, PolygonText* out, double* ref_lonlat, bool* has_ref, std::string* err) {
+  std::vector<double> colat, lo;
+  const int rc = read_query_csv(path, &colat, &lo, err);
+  if (rc) return rc;
+  size_t close = 0;
+  for (size_t i = 1; i < lo.size() && close == 0; i++)
+    if (colat[i] == colat[0] && lo[i] == lo[0]) close = i;
+  if (close == 0) {
+    *err = std::string(path) + ": the ring is not closed";
+    return GAHHG_E_FORMAT;
+  }
+  if (close < 3) {
+    *err = std::string(path) + ": a ring needs at least 3 vertices";
+    return GAHHG_E_FORMAT;
+  }
+  out->lo.resize(close);
+  out->la.resize(close);
+  for (size_t i = 0; i < close; i++) {
+    out->lo[i] = lune_lon(lo[i]);
+    out->la[i] = lune_lat(colat[i]);
+  }
+  out->rings.assign(1, Ring{0, (int32_t)close});
+  *has_ref = close + 1 < lo.size();
+  if (*has_ref) {
+    ref_lonlat[0] = lune_lon(lo[close + 1]);
+    ref_lonlat[1] = lune_lat(colat[close + 1]);
+  }
+  return GAHHG_OK;
+}
+
 // Level-0 grid.  The reference ships it only inside a compiled library
 // (initgrid0, declared at source/common.h:162-201, called at
 // source/GPISPAHHG.cu:1962); its shape is 42 cells (12 pentagons + 30

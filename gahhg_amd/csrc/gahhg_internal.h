@@ -420,6 +420,11 @@ void build_grid0(double* v80, int32_t* bi240x6, double* bn240x3, int32_t* ci42x7
 void query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
 int read_query_csv(const char* path, std::vector<double>* lon, std::vector<double>* lat,
                    std::string* err);
+// Lune/Ketzner-format files ("colatitude,longitude" radians per line):
+// points as lon/lat degrees in the reference's convention, and a polygon
+// file's ring (a PolygonText of one ring) plus the contained point after it.
+int read_lune_points(const char* path, std::vector<double>* lon, std::vector<double>* lat, std::string* err);
+int parse_polygon_lune(const char* path, PolygonText* out, double* ref_lonlat, bool* has_ref, std::string* err);
 
 }  // namespace gahhg
 
@@ -589,6 +594,10 @@ int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uin
                      int64_t n_max, int end_bit, int32_t* hist, uint32_t** kres, uint32_t** vres,
                      int32_t* scan_scratch = nullptr);
 int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out);
+// getCPProp's choice at n centres over given ref lists (device arrays),
+// with groups of G lanes.
+int run_kat_cp_prop(gahhg_ctx* c, const int32_t* d_refs, const int32_t* d_rs_rn, const double* d_centers, int64_t n,
+                    int G, int32_t* d_out3, double* d_mindist);
 int run_query_trace(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels,
                     int32_t* d_trace);
 LevelTab level_tab(const gahhg_ctx* c);

@@ -1,31 +1,20 @@
-"""Adapters for the point and polygon files the reference ships for the
-external comparators (test/data_for_Lune, README.md:5): one vertex or point
-per line as `colatitude_rad,longitude_rad`, the longitude counted from
-180 deg West in [0, 2 pi) (lon_deg = degrees(longitude) - 180: the files'
-Square_72 and Radiometer are the TXT files' polygons under that rule); a polygon
+"""Lune/Ketzner-format files (test/data_for_Lune, README.md:5): the format of
+the comparators the reference was measured against, one point per line as
+`colatitude_rad,longitude_rad` with the longitude counted from 180 deg West
+in [0, 2 pi) (lon_deg = degrees(longitude) - 180, taken mod 360).  A polygon
 file is a single closed ring (first vertex repeated) followed by one more
-line, a point the polygon contains (the reference point of Ketzner's
-containment test).
+line, a point the comparators take as inside.
 
-They convert to and from the reference's own convention -- lon/lat degrees,
-lon in [-180, 180), rings closed as in the polygon TXT files
-(source/GPISPPolygon.cu:58-186) -- so such files can be loaded through
-GPISP.set_polygon_lonlat / query_lonlat like the reference's own data.
+Reading is the library's (gahhg_read_lune_points, gahhg_read_lune_polygon,
+gahhg_load_polygon_lune in include/gahhg.h); the result is in the
+reference's own convention -- lon/lat degrees, lon in [-180, 180), rings
+closed as in the polygon TXT files (source/GPISPPolygon.cu:58-186) -- so it
+goes through GPISP.set_polygon_lonlat / query_lonlat like the reference's own
+data.  The writers below produce the format from that convention.
 """
-import math
-
 import numpy as np
 
-
-def _read_pairs(path):
-    a = np.loadtxt(path, delimiter=",", dtype=np.float64, ndmin=2)
-    return a[:, 0], a[:, 1]
-
-
-def colat_lon_to_lonlat(colat, lon):
-    lat = 90.0 - np.degrees(colat)
-    lo = np.mod(np.degrees(lon), 360.0) - 180.0
-    return lo, lat
+from . import read_lune_points as _read_points, read_lune_polygon as _read_polygon
 
 
 def lonlat_to_colat_lon(lon, lat):
@@ -37,28 +26,14 @@ def lonlat_to_colat_lon(lon, lat):
 def read_lune_polygon(path, with_reference_point=False):
     """A Lune-format polygon -> (lon, lat, ring_off) in the reference's
     convention (one ring, closing vertex kept); with_reference_point also
-    returns the contained point (lon, lat) the file ends with."""
-    colat, lon = _read_pairs(path)
-    lo, la = colat_lon_to_lonlat(colat, lon)
-    # the ring closes where the first vertex comes back; what follows is the
-    # reference point
-    close = [i for i in range(1, len(lo)) if colat[i] == colat[0] and lon[i] == lon[0]]
-    if not close:
-        raise ValueError(f"{path}: the ring is not closed")
-    end = close[0] + 1
-    if end < 4:
-        raise ValueError(f"{path}: a ring needs at least 3 vertices")
-    ring = (lo[:end].copy(), la[:end].copy(), np.array([0, end], dtype=np.int64))
-    if with_reference_point:
-        ref = (float(lo[end]), float(la[end])) if end < len(lo) else None
-        return ring + (ref,)
-    return ring
+    returns the contained point (lon, lat) the file ends with, or None."""
+    lon, lat, off, ref = _read_polygon(path)
+    return (lon, lat, off, ref) if with_reference_point else (lon, lat, off)
 
 
 def read_lune_points(path):
     """Lune-format points -> (lon, lat) degrees."""
-    colat, lon = _read_pairs(path)
-    return colat_lon_to_lonlat(colat, lon)
+    return _read_points(path)
 
 
 def write_lune_points(path, lon, lat):

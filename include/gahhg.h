@@ -385,6 +385,25 @@ int gahhg_load_grid(gahhg_ctx* ctx, const char* path);
 /* Reads a query CSV ("lon,lat" per line) into host arrays.  With lon/lat
  * NULL it returns the count in *n. */
 int gahhg_read_query_csv(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);
+/* Lune/Ketzner-format files: the reference ships copies of its polygons and
+ * query points in the format of the comparators it was measured against
+ * (README.md:5, test/data_for_Lune) and no reader of them; these read them
+ * into the reference's own convention (lon/lat degrees) so that they take
+ * the reference's path from there.  A line is "colatitude,longitude" in
+ * radians, longitude counted from 180 deg West: lat = 90 - deg(colat),
+ * lon = (deg(lon) mod 360) - 180, mod taking the divisor's sign.
+ * A polygon file is one ring closed by repeating its first vertex, then a
+ * point known to be inside (ref_lonlat, *has_ref = 1 when present).
+ *
+ * gahhg_read_lune_points: as gahhg_read_query_csv (NULL lon/lat = count).
+ * gahhg_read_lune_polygon: the ring with its closing vertex (*n of them;
+ *   NULL lon/lat = count), for gahhg_set_polygon_lonlat with one ring.
+ * gahhg_load_polygon_lune: as gahhg_load_polygon_txt on the ring. */
+int gahhg_read_lune_points(const char* path, double* lon, double* lat, int64_t cap, int64_t* n);
+int gahhg_read_lune_polygon(const char* path, double* lon, double* lat, int64_t cap, int64_t* n,
+                            double* ref_lonlat, int32_t* has_ref);
+int gahhg_load_polygon_lune(gahhg_ctx* ctx, const char* path, double* ref_lonlat, int32_t* has_ref);
+
 /* Query lon/lat -> XYZ exactly as the reference converts test points
  * (FakeLLVertex round trip, source/GPISPPolygon.cuh:72,82-92). */
 int gahhg_query_lonlat_to_xyz(const double* lon, const double* lat, int64_t n, double* xyz);
@@ -425,6 +444,16 @@ int gahhg_export_grid(gahhg_ctx* ctx, double* v, int32_t* cells, int32_t* border
  * op 2: isEEInter(a, b, c, d) -> out[0] in {0,1};
  * op 3: isEdgeBorderInter(a, b, c, d) -> out[0] = class, out[1] = reserved. */
 int gahhg_kat_device(gahhg_ctx* ctx, int32_t op, const double* in12, int64_t n, double* out3);
+/* Known-answer hook for device_getCPProp (source/GPISPAHHG.cu:61-171) as
+ * setBottomNECell runs it (a group of `group` lanes per cell, group a power
+ * of two up to 64), over the built context's polygon edges (the edge
+ * records the build makes): case k takes the refs
+ * refs[rs_rn[2k], + rs_rn[2k+1]) (edge indices, at least one) and the
+ * centre centers[3k..3k+2]; out3[3k..] = type (1 out, 2 in), the slot in the
+ * case's refs of the chosen edge (-1: none) and its nearest vertex (-1: a
+ * point inside the edge); mindist[k] = that feature's squared distance. */
+int gahhg_kat_cp_prop(gahhg_ctx* ctx, const int32_t* refs, int64_t n_refs, const int32_t* rs_rn,
+                      const double* centers, int64_t n, int32_t group, int32_t* out3, double* mindist);
 
 /* Parity hook of the grouping sort (no reference counterpart): sorts the n
  * host pairs (keys[i], vals[i]) in place, stably, by key bits

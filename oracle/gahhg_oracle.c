@@ -1068,18 +1068,22 @@ int og_build_create_only(og_ctx* c, int32_t max_depth, int32_t auto_depth) {
 /* Classification (source/GPISPAHHG.cu:1046-1124,1292-1419,1787-1817,      */
 /* 2018-2059).                                                             */
 /* ---------------------------------------------------------------------- */
-/* device_getCPProp, source/GPISPAHHG.cu:61-171 */
-static int cp_prop(const og_ctx* c, int cell, v3 center) {
+/* device_getCPProp, source/GPISPAHHG.cu:61-171, over refs[0, rn); the
+ * chosen feature goes to slot (its position in refs, -1 if none was
+ * chosen), nvo (its vertex, -1 for a point inside the edge) and md (its
+ * squared distance) when those are not NULL. */
+static int cp_prop_refs(const v3* pv, const int32_t* pe, const int32_t* refs, int rn, v3 center,
+                        int* slot, int* nvo, double* md) {
     double mindist = FLT_MAX;
-    const og_cell* cc = &c->c[cell];
-    int ne = c->refs[cc->rstart];
+    int ne = refs[0];
     v3 ne_normal = mk(0.0, 0.0, 0.0);
     int nv_index = -1;
-    for (int i = 0; i < cc->rnum; i++) {
-        int se = c->refs[cc->rstart + i];
+    int sl = -1;
+    for (int i = 0; i < rn; i++) {
+        int se = refs[i];
         double curdist = FLT_MAX;
-        v3 v0 = c->pv[c->pe[se * 2]];
-        v3 v1 = c->pv[c->pe[se * 2 + 1]];
+        v3 v0 = pv[pe[se * 2]];
+        v3 v1 = pv[pe[se * 2 + 1]];
         v3 n0 = vnorm(vcross(v0, vsub(v1, v0)));
         v3 n1 = vnorm(vcross(center, n0));
         v3 inter = vnorm(vcross(n0, n1));
@@ -1095,37 +1099,45 @@ static int cp_prop(const og_ctx* c, int cell, v3 center) {
         } else {
             curdist = vsq(vsub(center, v0));
             double vdist = vsq(vsub(center, v1));
-            if (vdist < curdist) { curdist = vdist; cur_nv = c->pe[se * 2 + 1]; }
-            else cur_nv = c->pe[se * 2];
+            if (vdist < curdist) { curdist = vdist; cur_nv = pe[se * 2 + 1]; }
+            else cur_nv = pe[se * 2];
         }
         if (nv_index != -1 && nv_index == cur_nv) {
             double a1d = vdot(ne_normal, center);
             double a2d = vdot(n0, center);
             if ((LESS_EQU_ZERO(a1d) && GRE_EQU_ZERO(a2d)) || (GRE_EQU_ZERO(a1d) && LESS_EQU_ZERO(a2d))) {
                 int another;
-                if (c->pe[ne * 2] == nv_index) another = c->pe[ne * 2 + 1];
-                else another = c->pe[ne * 2];
-                v3 a0 = vsub(c->pv[another], c->pv[nv_index]);
-                if (c->pe[se * 2] == nv_index) another = c->pe[se * 2 + 1];
-                else another = c->pe[se * 2];
-                v3 a1 = vsub(c->pv[another], c->pv[nv_index]);
-                v3 bb = vsub(center, c->pv[nv_index]);
+                if (pe[ne * 2] == nv_index) another = pe[ne * 2 + 1];
+                else another = pe[ne * 2];
+                v3 a0 = vsub(pv[another], pv[nv_index]);
+                if (pe[se * 2] == nv_index) another = pe[se * 2 + 1];
+                else another = pe[se * 2];
+                v3 a1 = vsub(pv[another], pv[nv_index]);
+                v3 bb = vsub(center, pv[nv_index]);
                 a0 = vnorm(a0);
                 a1 = vnorm(a1);
                 a1d = vdot(a0, bb);
                 a2d = vdot(a1, bb);
                 if (a2d > a1d) {
-                    mindist = curdist; ne = se; ne_normal = n0; nv_index = cur_nv;
+                    mindist = curdist; ne = se; ne_normal = n0; nv_index = cur_nv; sl = i;
                     continue;
                 }
             }
         }
         if (curdist < mindist) {
-            mindist = curdist; ne = se; ne_normal = n0; nv_index = cur_nv;
+            mindist = curdist; ne = se; ne_normal = n0; nv_index = cur_nv; sl = i;
         }
     }
+    if (slot) *slot = sl;
+    if (nvo) *nvo = nv_index;
+    if (md) *md = mindist;
     double temp = vdot(center, ne_normal);
     return GRE_ZERO(temp) ? CELL_IN : CELL_OUT;
+}
+
+static int cp_prop(const og_ctx* c, int cell, v3 center) {
+    const og_cell* cc = &c->c[cell];
+    return cp_prop_refs(c->pv, c->pe, c->refs + cc->rstart, cc->rnum, center, NULL, NULL, NULL);
 }
 
 /* Provenance of a cell's type (test hook; no counterpart in the
@@ -1449,5 +1461,14 @@ int32_t og_kat_edge_border_inter(const double* p0, const double* p1, const doubl
     int r = -1;
     int t = edge_border_inter(mk(p0[0], p0[1], p0[2]), mk(p1[0], p1[1], p1[2]), mk(v0[0], v0[1], v0[2]), mk(v1[0], v1[1], v1[2]), &r);
     *reserved = r;
+    return t;
+}
+
+int32_t og_kat_cp_prop(const double* pv, const int32_t* pe, const int32_t* refs, int32_t rn, const double* center,
+                       int32_t* slot, int32_t* nv, double* mindist) {
+    int sl = -1, v = -1;
+    const int t = cp_prop_refs((const v3*)pv, pe, refs, rn, mk(center[0], center[1], center[2]), &sl, &v, mindist);
+    *slot = sl;
+    *nv = v;
     return t;
 }

@@ -130,6 +130,13 @@ int32_t og_kat_is_ee_inter(const double* p0, const double* p1,
 int32_t og_kat_edge_border_inter(const double* p0, const double* p1,
                                  const double* v0, const double* v1,
                                  int32_t* reserved);
+/* device_getCPProp (source/GPISPAHHG.cu:61-171) over refs[0, rn) of the
+ * polygon pv (3 doubles per vertex) / pe (2 vertex indices per edge):
+ * the type (1 out, 2 in); slot = the chosen edge's position in refs (-1:
+ * none chosen), nv = its nearest vertex (-1: a point inside the edge),
+ * mindist = that feature's squared distance. */
+int32_t og_kat_cp_prop(const double* pv, const int32_t* pe, const int32_t* refs, int32_t rn,
+                       const double* center, int32_t* slot, int32_t* nv, double* mindist);
 
 #ifdef __cplusplus
 }

@@ -88,8 +88,24 @@ def lib():
         L.og_kat_is_ee_inter.restype = ctypes.c_int32
         L.og_kat_edge_border_inter.argtypes = [dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int32)]
         L.og_kat_edge_border_inter.restype = ctypes.c_int32
+        L.og_kat_cp_prop.argtypes = [dp, ip, ip, ctypes.c_int32, dp, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]
+        L.og_kat_cp_prop.restype = ctypes.c_int32
         _lib = L
     return _lib
+
+
+def cp_prop(pv, pe, refs, center):
+    """getCPProp over refs at center: (type, slot of the chosen edge in
+    refs or -1, its nearest vertex or -1, squared distance)."""
+    L = lib()
+    sl, nv, md = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+    t = L.og_kat_cp_prop(np.ascontiguousarray(pv, dtype=np.float64).reshape(-1),
+                         np.ascontiguousarray(pe, dtype=np.int32).reshape(-1),
+                         np.ascontiguousarray(refs, dtype=np.int32), len(refs),
+                         np.ascontiguousarray(center, dtype=np.float64), ctypes.byref(sl), ctypes.byref(nv),
+                         ctypes.byref(md))
+    return int(t), sl.value, nv.value, md.value
 
 
 def sincos_cr(x):

@@ -72,6 +72,91 @@ def test_kat_bitwise(dev):
     assert bad == 0
 
 
+def _unit(x):
+    return x / np.linalg.norm(x, axis=-1, keepdims=True)
+
+
+def cp_prop_cases(pv, pe, rng, n_tie=5000, n_near=3000, n_vertex=500, n_arc=500):
+    """getCPProp inputs that reach every branch of its loop: centres near a
+    vertex with both edges at it among the refs (the shared-vertex tie rule),
+    centres near an edge with up to 150 nearby refs (more than a 64-lane
+    group), centres exactly at a vertex and exactly on an edge's great
+    circle (zero distances and zero sides)."""
+    N = len(pe)
+    nxt = np.empty(N, dtype=np.int64)
+    prv = np.empty(N, dtype=np.int64)
+    nxt[pe[:, 0]] = np.arange(N)
+    prv[pe[:, 1]] = np.arange(N)
+    mid = _unit(pv[pe[:, 0]] + pv[pe[:, 1]])
+    flat, rs_rn, centers, tie_vertex = [], [], [], []
+
+    def add(refs, c, k=-1):
+        rs_rn.append((len(flat), len(refs)))
+        flat.extend(int(r) for r in refs)
+        centers.append(c)
+        tie_vertex.append(k)
+
+    def nearest(c, m):
+        return np.argsort(-(mid @ c))[:m]
+
+    for _ in range(n_tie):
+        k = int(rng.integers(N))
+        c = _unit(pv[k] + 10 ** rng.uniform(-9, -2) * _unit(rng.normal(size=3)))
+        others = [e for e in nearest(c, 8) if e not in (prv[k], nxt[k])][:int(rng.integers(0, 7))]
+        refs = [prv[k], nxt[k]] + others
+        rng.shuffle(refs)
+        add(refs, c, k)
+    for _ in range(n_near):
+        e = int(rng.integers(N))
+        c = _unit(mid[e] + 10 ** rng.uniform(-8, -1) * _unit(rng.normal(size=3)))
+        refs = list(nearest(c, int(rng.integers(1, 151))))
+        rng.shuffle(refs)
+        add(refs, c)
+    for _ in range(n_vertex):
+        k = int(rng.integers(N))
+        refs = [prv[k], nxt[k]] + list(nearest(pv[k], int(rng.integers(0, 5))))
+        rng.shuffle(refs)
+        add(refs, pv[k].copy(), k)
+    for _ in range(n_arc):
+        e = int(rng.integers(N))
+        refs = list(nearest(mid[e], int(rng.integers(1, 9))))
+        if e not in refs:
+            refs.append(e)
+        rng.shuffle(refs)
+        add(refs, mid[e].copy())
+    return (np.array(flat, dtype=np.int32), np.array(rs_rn, dtype=np.int32), np.array(centers),
+            np.array(tie_vertex))
+
+
+def test_kat_cp_prop_bitwise(dev):
+    """getCPProp (device_getCPProp, source/GPISPAHHG.cu:61-171) as
+    setBottomNECell runs it on the GPU -- a lane group per cell replaying
+    the selection -- against the oracle, case by case: type, chosen edge,
+    nearest vertex and squared distance bit for bit, for every group size."""
+    say("kat: getCPProp, NA0, 9000 cases x 7 group sizes")
+    dev.load_polygon_txt(refpath("NA0.txt")).build()
+    o = ol.Oracle().load_txt(refpath("NA0.txt"))
+    pv, pe = o.polygon()
+    refs, rs_rn, centers, tie_vertex = cp_prop_cases(pv, pe, np.random.default_rng(11))
+    want = np.zeros((len(rs_rn), 3), dtype=np.int32)
+    want_md = np.zeros(len(rs_rn))
+    for k, (rs, rn) in enumerate(rs_rn):
+        t, sl, nv, md = ol.cp_prop(pv, pe, refs[rs:rs + rn], centers[k])
+        want[k] = (t, sl, nv)
+        want_md[k] = md
+    # the cases reach the branches: both types, vertex and interior
+    # features, the shared vertex of a tie case chosen
+    assert set(np.unique(want[:, 0])) == {1, 2}
+    assert np.mean(want[:, 2] == -1) > 0.2 and np.mean(want[:, 2] >= 0) > 0.2
+    tie = tie_vertex >= 0
+    assert np.mean(want[tie, 2] == tie_vertex[tie]) > 0.25
+    assert np.all(want[:, 1] >= 0)
+    for G in (1, 2, 4, 8, 16, 32, 64):
+        got, md = dev.kat_cp_prop(refs, rs_rn, centers, G)
+        bad = np.nonzero((got != want).any(axis=1) | (md.view(np.uint64) != want_md.view(np.uint64)))[0]
+        assert len(bad) == 0, (G, bad[:5], got[bad[:3]], want[bad[:3]])
+
+
 def compare_grids(gg, og, levels_o):
     assert gg["depth"] == len(levels_o)
     lv = [[L[k] for k in ("vstart", "vnum", "bstart", "bnum", "cstart", "cnum", "rnum")] for L in levels_o]
