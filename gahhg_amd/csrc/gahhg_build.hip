@@ -162,12 +162,14 @@ __device__ __forceinline__ int seg_exit_lazy(const GridView& g, const WalkRec* _
   return -1;
 }
 
-// Walk records of the cells [lo, lo + n): each thread fills one record in
-// LDS, then the block writes its records out as contiguous 16-byte chunks.
+// Walk records of the cells [lo + first, lo + first + kRecBlock) of the
+// range [lo, lo + n): each thread fills one record in LDS, then the block
+// writes its records out as contiguous 16-byte chunks.  Records are indexed
+// by global cell id.
 constexpr int kRecBlock = 128;
-__global__ void __launch_bounds__(kRecBlock) k_walk_records(GridView g, int lo, int n, WalkRec* __restrict__ rec) {
-  __shared__ WalkRec srec[kRecBlock];
-  const int t = blockIdx.x * kRecBlock + threadIdx.x;
+__device__ __forceinline__ void walk_records_tile(const GridView& g, int lo, int n, int first,
+                                                  WalkRec* __restrict__ rec, WalkRec* srec) {
+  const int t = first + threadIdx.x;
   if (t < n) {
     const int cid = lo + t;
     WalkRec& R = srec[threadIdx.x];
@@ -195,12 +197,28 @@ __global__ void __launch_bounds__(kRecBlock) k_walk_records(GridView g, int lo, 
     R.pad[0] = R.pad[1] = 0;
   }
   __syncthreads();
-  const int first = blockIdx.x * kRecBlock;
   const int cnt = min(kRecBlock, n - first);
   const int chunks = cnt * (int)(sizeof(WalkRec) / 16);
   const uint4* src = (const uint4*)srec;
   uint4* dst = (uint4*)(rec + lo + first);
   for (int c = threadIdx.x; c < chunks; c += kRecBlock) dst[c] = src[c];
+  __syncthreads();  // srec is refilled by the next tile
+}
+
+__global__ void __launch_bounds__(kRecBlock) k_walk_records(GridView g, int lo, int n, WalkRec* __restrict__ rec) {
+  __shared__ WalkRec srec[kRecBlock];
+  walk_records_tile(g, lo, n, blockIdx.x * kRecBlock, rec, srec);
+}
+
+// The same for level lv of the device-driven build, when its walks use
+// records (bit lv of st->rec_levels).
+__global__ void __launch_bounds__(kRecBlock) k_walk_records_dv(GridView g, const DevState* __restrict__ st, int lv,
+                                                               WalkRec* __restrict__ rec) {
+  __shared__ WalkRec srec[kRecBlock];
+  if (!((st->rec_levels >> lv) & 1u)) return;
+  const LevelDesc L = st->lv[lv];
+  for (int first = blockIdx.x * kRecBlock; first < L.cnum; first += gridDim.x * kRecBlock)
+    walk_records_tile(g, L.cstart, L.cnum, first, rec, srec);
 }
 
 struct WalkDiag {
@@ -377,13 +395,20 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 // its polygon's bottom level, starting from the level above it (buf_prev)
 // when refinement stopped by the density rule or the model after marking
 // that level (kCount then takes the count the mark walk kept).
+// dv != 0 (the device-driven build): the record levels come from
+// st->rec_levels, and the launch does nothing once the level loop has
+// stopped (mark walk) or the build stops short (dispatch walks).
 template <int MODE, bool REC>
 __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __restrict__ rec, uint32_t rec_levels,
                                                  const double* __restrict__ edge,
                                                  const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd,
                                                  int Ntot, int level, int32_t* buf, int32_t* buf_prev,
                                                  const int32_t* off, int32_t* cnt_out, int32_t* rcell,
-                                                 DevState* st) {
+                                                 DevState* st, int dv) {
+  if (dv) {
+    if (MODE == kMark ? st->run == 0 : st->abort != 0) return;  // uniform over the launch
+    rec_levels = st->rec_levels;
+  }
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   WalkDiag d = {0, 0, 0, 0, 0};
   bool mine = e < Ntot;
@@ -455,10 +480,7 @@ __global__ void k_edges(const double* __restrict__ pv, const int32_t* __restrict
 
 // kernel_DivGridFast_zeroRefCount, :1127-1150, over one level's cells of
 // every polygon (level 0 also resets the fields the reference resets there).
-__global__ void k_zero(GridView g, int lo, int n, int level0) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int cid = lo + t;
+__device__ __forceinline__ void zero_cell(const GridView& g, int cid, int level0) {
   g.c_rnum[cid] = 0;
   if (level0) {
     g.c_rstart[cid] = -1;
@@ -468,6 +490,21 @@ __global__ void k_zero(GridView g, int lo, int n, int level0) {
     const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
     for (int i = 0; i < bn; i++) g.b_subcell[bs + i] = -1;
   }
+}
+
+__global__ void k_zero(GridView g, int lo, int n, int level0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) zero_cell(g, lo + t, level0);
+}
+
+// Grid-stride loop over the cells of level lv of the device-driven build.
+#define FOR_LEVEL_CELLS(L, t) \
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < (L).cnum; t += gridDim.x * blockDim.x)
+
+__global__ void k_zero_dv(GridView g, const DevState* __restrict__ st, int lv) {
+  if (!st->run) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL_CELLS(L, t) zero_cell(g, L.cstart + t, lv == 0 ? 1 : 0);
 }
 
 // Kind of the border child produced for border i of refined cell P
@@ -488,9 +525,8 @@ __device__ __forceinline__ int child_kind(const GridView& g, int P, int bnum, in
 // Plan of one level (kernel_DivGrid_calcCenterOffsets :381-404 and the
 // border-child counts): per cell, new corners (= bnum when refined), the
 // centre child, border children and their borders.
-__global__ void k_plan(GridView g, int lo, int n, int32_t* fa, int32_t* fb, int32_t* fc, int32_t* fd) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+__device__ __forceinline__ void plan_cell(const GridView& g, int lo, int t, int32_t* fa, int32_t* fb, int32_t* fc,
+                                          int32_t* fd) {
   const int cid = lo + t;
   const bool r = g.c_rnum[cid] != 0;
   const int bn = g.c_bnum[cid];
@@ -507,10 +543,27 @@ __global__ void k_plan(GridView g, int lo, int n, int32_t* fa, int32_t* fb, int3
   fd[t] = nb;
 }
 
+__global__ void k_plan(GridView g, int lo, int n, int32_t* fa, int32_t* fb, int32_t* fc, int32_t* fd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) plan_cell(g, lo, t, fa, fb, fc, fd);
+}
+
+__global__ void k_plan_dv(GridView g, const DevState* __restrict__ st, int lv, int32_t* fa, int32_t* fb, int32_t* fc,
+                          int32_t* fd) {
+  if (!st->run) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL_CELLS(L, t) plan_cell(g, L.cstart, t, fa, fb, fc, fd);
+}
+
 // Per-polygon totals of the plan from the exclusive scans.
+// (st non-null: the device-driven build; lo is then its level's first cell)
 __global__ void k_plan_totals(PolyDev* pd, int npoly, int lo, const int32_t* fa, const int32_t* fb,
                               const int32_t* fc, const int32_t* fd, const int32_t* oa, const int32_t* ob,
-                              const int32_t* oc, const int32_t* od) {
+                              const int32_t* oc, const int32_t* od, const DevState* st, int lv) {
+  if (st) {
+    if (!st->run) return;
+    lo = st->lv[lv].cstart;
+  }
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npoly) return;
   PolyDev& P = pd[p];
@@ -523,10 +576,8 @@ __global__ void k_plan_totals(PolyDev* pd, int npoly, int lo, const int32_t* fa,
 }
 
 // kernel_DivGrid_createCenter, :406-470
-__global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oa,
-                                const int32_t* ob) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+__device__ __forceinline__ void create_center_cell(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t,
+                                                   const int32_t* oa, const int32_t* ob) {
   const int pid = lo + t;
   if (g.c_rnum[pid] == 0) return;
   const int p = g.c_poly[pid];
@@ -570,12 +621,23 @@ __global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int 
   g.c_subcell[pid] = child;
 }
 
+__global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oa,
+                                const int32_t* ob) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) create_center_cell(g, pd, lo, t, oa, ob);
+}
+
+__global__ void k_create_center_dv(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st, int lv,
+                                   const int32_t* oa, const int32_t* ob) {
+  if (!st->run) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL_CELLS(L, t) create_center_cell(g, pd, L.cstart, t, oa, ob);
+}
+
 // kernel_DivGrid_createBorderSC, :512-607: the border children of one
 // refined cell, in border order.
-__global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oc,
-                                  const int32_t* od) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+__device__ __forceinline__ void border_children_cell(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t,
+                                                     const int32_t* oc, const int32_t* od) {
   const int P = lo + t;
   if (g.c_rnum[P] == 0) return;
   const int p = g.c_poly[P];
@@ -636,6 +698,19 @@ __global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, in
   }
 }
 
+__global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oc,
+                                  const int32_t* od) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) border_children_cell(g, pd, lo, t, oc, od);
+}
+
+__global__ void k_border_children_dv(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st,
+                                     int lv, const int32_t* oc, const int32_t* od) {
+  if (!st->run) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL_CELLS(L, t) border_children_cell(g, pd, L.cstart, t, oc, od);
+}
+
 // device_setBorderPartner, :42-58: link `edge` with the border of `cell`
 // running the opposite way; the lower index writes both sides.
 __device__ __forceinline__ void link_border(const GridView& g, int cell, int edge) {
@@ -660,10 +735,7 @@ __device__ __forceinline__ void link_or_clear(const GridView& g, int te, int edg
 
 // kernel_DivGrid_setSCPartner, :610-682, over the new level's cells (the
 // centre children are skipped).
-__global__ void k_sc_partner(GridView g, const PolyDev* __restrict__ pd, int lo, int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int X = lo + t;
+__device__ __forceinline__ void sc_partner_cell(const GridView& g, const PolyDev* __restrict__ pd, int X) {
   const PolyDev& D = pd[g.c_poly[X]];
   if (X - D.ncb < D.U) return;
   const int Pu = g.c_upcell[X];
@@ -699,6 +771,18 @@ __global__ void k_sc_partner(GridView g, const PolyDev* __restrict__ pd, int lo,
     g.b_partner[te] = xb + 2;
     link_or_clear(g, ubs + (upid + ub - 1) % ub, xb + 3);
   }
+}
+
+__global__ void k_sc_partner(GridView g, const PolyDev* __restrict__ pd, int lo, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) sc_partner_cell(g, pd, lo + t);
+}
+
+// over the cells of level lv + 1, the level just created
+__global__ void k_sc_partner_dv(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st, int lv) {
+  if (!st->run) return;
+  const LevelDesc L = st->lv[lv + 1];
+  FOR_LEVEL_CELLS(L, t) sc_partner_cell(g, pd, L.cstart + t);
 }
 
 // --- dispatch (:1700-1784) ---
@@ -796,6 +880,346 @@ __global__ void k_fill(int32_t* p, int64_t n, int32_t v) {
 __global__ void k_copy(int32_t* d, const int32_t* s, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven build: the host loop's per-level decisions as kernels
+// ---------------------------------------------------------------------------
+constexpr int kCtlBlock = 256;
+constexpr int kDvGrid = 2048;  // blocks of the grid-stride launches
+
+// Block-wide exclusive prefix of an int64 per thread (kCtlBlock threads);
+// *total receives the block's sum.
+__device__ __forceinline__ long long block_excl64(long long v, long long* total) {
+  __shared__ long long ws[kCtlBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  long long inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const long long u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  long long base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kCtlBlock / 64; i++) {
+    if (i < w) base += ws[i];
+    tot += ws[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+// Build start: the state cleared (as a host-side memset of it would), the
+// polygon records from their template, level 0 of every polygon.
+__global__ void __launch_bounds__(kCtlBlock) k_dv_start(DevState* st, PolyDev* pd, const PolyDev* __restrict__ pd0,
+                                                        int np, LevelDesc* plv, int32_t* plvn, int32_t* pmarks,
+                                                        float* gamma) {
+  int32_t* w = (int32_t*)st;
+  for (int i = threadIdx.x; i < (int)(sizeof(DevState) / 4); i += kCtlBlock) w[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t V0 = (int64_t)kGrid0Verts * np, B0 = (int64_t)kGrid0Borders * np, C0 = (int64_t)kGrid0Cells * np;
+    st->lv[0] = LevelDesc{0, (int32_t)V0, 0, (int32_t)B0, 0, (int32_t)C0, 0};
+    st->gnum = 1;
+    st->tot_v = V0;
+    st->tot_b = B0;
+    st->tot_c = C0;
+    st->run = 1;
+  }
+  for (int p = threadIdx.x; p < np; p += kCtlBlock) {
+    pd[p] = pd0[p];
+    plv[(int64_t)p * kMaxLevels] = LevelDesc{kGrid0Verts * p, kGrid0Verts, kGrid0Borders * p, kGrid0Borders,
+                                             kGrid0Cells * p, kGrid0Cells, 0};
+    plvn[p] = 1;
+    pmarks[p] = 0;
+    gamma[p] = 0.f;
+  }
+}
+
+// One copy of the level-0 grid per polygon (block = polygon); borders are
+// (parent, partner, v0, v1, subcell, id), cells (bstart, bnum, rstart, rnum,
+// upcell, subcell, type), all local to the copy.
+__global__ void __launch_bounds__(kCtlBlock) k_dv_grid0(GridView g, const double* __restrict__ v0,
+                                                        const double* __restrict__ n0, const int32_t* __restrict__ b0,
+                                                        const int32_t* __restrict__ c0) {
+  const int p = blockIdx.x;
+  const int vb = kGrid0Verts * p, bb = kGrid0Borders * p, cb = kGrid0Cells * p;
+  double* vw = const_cast<double*>(g.v);
+  for (int i = threadIdx.x; i < 3 * kGrid0Verts; i += kCtlBlock) vw[3 * (int64_t)vb + i] = v0[i];
+  for (int i = threadIdx.x; i < 3 * kGrid0Borders; i += kCtlBlock) g.b_n[3 * (int64_t)bb + i] = n0[i];
+  for (int i = threadIdx.x; i < kGrid0Borders; i += kCtlBlock) {
+    const int32_t* r = b0 + 6 * i;
+    g.b_parent[bb + i] = r[0] + cb;
+    g.b_partner[bb + i] = r[1] < 0 ? -1 : r[1] + bb;
+    g.b_v0[bb + i] = r[2] + vb;
+    g.b_v1[bb + i] = r[3] + vb;
+    g.b_subcell[bb + i] = r[4];
+    g.b_id[bb + i] = r[5];
+  }
+  for (int i = threadIdx.x; i < kGrid0Cells; i += kCtlBlock) {
+    const int32_t* r = c0 + 7 * i;
+    g.c_bstart[cb + i] = r[0] + bb;
+    g.c_bnum[cb + i] = r[1];
+    g.c_rstart[cb + i] = r[2];
+    g.c_rnum[cb + i] = r[3];
+    g.c_upcell[cb + i] = r[4];
+    g.c_subcell[cb + i] = r[5];
+    g.c_type[cb + i] = r[6];
+    g.c_poly[cb + i] = p;
+    g.c_root[cb + i] = i;
+  }
+}
+
+// Top of level lv (createAHHG's loop, :1983-1992): whether this level's
+// walks use walk records, the stop of polygons with a fixed depth, the stop
+// of every polygon at the last level; the loop ends when none refines.
+__global__ void __launch_bounds__(kCtlBlock) k_lv_begin(DevState* st, PolyDev* pd, int np, int lv, int maxd,
+                                                        int32_t* pmarks, int64_t wrec_cap) {
+  if (!st->run) return;
+  __syncthreads();  // every thread has read run before thread 0 may clear it
+  if (threadIdx.x == 0) {
+    const LevelDesc GL = st->lv[lv];
+    if (GL.cnum > 0 && lv < 32 && (int64_t)GL.cstart + GL.cnum <= wrec_cap &&
+        st->extra_next >= kWalkRecStepsPerCell * (double)GL.cnum)
+      st->rec_levels |= 1u << lv;
+  }
+  int any = 0;
+  for (int p = threadIdx.x; p < np; p += kCtlBlock) {
+    PolyDev& P = pd[p];
+    if (P.refining && ((P.depth > 0 && lv == P.depth - 1) || lv == maxd - 1)) {
+      P.refining = 0;
+      P.L = lv;
+      P.use_prev = 0;
+    }
+    if (P.refining) {
+      any = 1;
+      pmarks[p] += 1;
+    }
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0 && !any) st->run = 0;
+}
+
+// After level lv's plan (divGrid's host part and the density rule,
+// :1595-1605): the mean walk length, the density rule per polygon, the
+// next level's offsets per polygon, the arena and int32 checks, the next
+// level's descriptor and the expected walk steps there.
+__global__ void __launch_bounds__(kCtlBlock) k_lv_step(PolyDev* pd, int np, DevState* st, int lv, int autod,
+                                                       float* gamma, int64_t vcap, int64_t bcap, int64_t ccap) {
+  if (!st->run) return;
+  __shared__ double sh_m;
+  __shared__ int sh_ok;
+  // walkers of this level's mark walk and its steps per walker
+  long long wk = 0;
+  for (int p = threadIdx.x; p < np; p += kCtlBlock) wk += pd[p].refining ? pd[p].N : 0;
+  long long walkers;
+  (void)block_excl64(wk, &walkers);
+  if (threadIdx.x == 0) {
+    const unsigned long long steps = st->steps_mark;
+    sh_m = walkers > 0 ? (double)(steps - st->steps_before) / (double)walkers : 0.0;
+    st->steps_before = steps;
+  }
+  // density rule, then offsets in polygon order (a block scan per 256 polygons)
+  const int64_t tv = st->tot_v, tb = st->tot_b, tc = st->tot_c;
+  long long cv = 0, cb = 0, cc = 0, wn = 0;
+  for (int base = 0; base < np; base += kCtlBlock) {
+    const int p = base + threadIdx.x;
+    long long a = 0, b = 0, d = 0;
+    if (p < np) {
+      PolyDev& P = pd[p];
+      if (P.refining && autod && P.depth == 0) {
+        const float k = P.U * 1.0 / P.N;
+        gamma[p] = k;
+        if (k > 0.1 || (k < 0.1 && (0.1 - k) < 0.01)) {
+          P.refining = 0;
+          P.L = lv;
+          P.use_prev = lv != 0 ? 1 : 0;
+        }
+      }
+      if (P.refining) {
+        a = P.nv;
+        b = (long long)P.nv + P.nbb;
+        d = (long long)P.U + P.nbc;
+        wn += P.N;
+      }
+    }
+    long long ta, tb2, td;
+    const long long ea = block_excl64(a, &ta);
+    const long long eb = block_excl64(b, &tb2);
+    const long long ed = block_excl64(d, &td);
+    if (p < np && pd[p].refining) {
+      PolyDev& P = pd[p];
+      P.nvb = (int32_t)(tv + cv + ea);
+      P.nbb0 = (int32_t)(tb + cb + eb);
+      P.ncb = (int32_t)(tc + cc + ed);
+    }
+    cv += ta;
+    cb += tb2;
+    cc += td;
+  }
+  long long wnext;
+  (void)block_excl64(wn, &wnext);
+  if (threadIdx.x == 0) {
+    sh_ok = 1;
+    if (cc == 0) {
+      st->run = 0;  // nothing to create: the loop ends here
+      sh_ok = 0;
+    } else if (tv + cv > INT32_MAX || tb + cb > INT32_MAX || tc + cc > INT32_MAX) {
+      st->limit = 1;
+      st->abort = 1;
+      st->run = 0;
+      sh_ok = 0;
+    } else if (tv + cv > vcap || tb + cb > bcap || tc + cc > ccap) {
+      st->overflow = 1;
+      st->need_v = tv + cv;
+      st->need_b = tb + cb;
+      st->need_c = tc + cc;
+      st->abort = 1;
+      st->run = 0;
+      sh_ok = 0;
+    }
+    if (sh_ok) {
+      st->lv[lv + 1] = LevelDesc{(int32_t)tv, (int32_t)cv, (int32_t)tb, (int32_t)cb, (int32_t)tc, (int32_t)cc, 0};
+      st->gnum = lv + 2;
+      st->tot_v = tv + cv;
+      st->tot_b = tb + cb;
+      st->tot_c = tc + cc;
+      st->extra_next = (double)wnext * 2.0 * fmax(sh_m - 1.0, 0.0);
+    }
+  }
+}
+
+// Every polygon still refining moves to the level just created.
+__global__ void k_lv_advance(PolyDev* pd, int np, const DevState* __restrict__ st, int lv, LevelDesc* plv,
+                             int32_t* plvn) {
+  if (!st->run) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  PolyDev& P = pd[p];
+  if (!P.refining) return;
+  LevelDesc* L = plv + (int64_t)p * kMaxLevels;
+  L[lv].rnum = P.U;
+  L[lv + 1] = LevelDesc{P.nvb, P.nv, P.nbb0, P.nv + P.nbb, P.ncb, P.U + P.nbc, 0};
+  plvn[p] = lv + 2;
+  P.pbs = P.bs;
+  P.plbs = P.lbs;
+  P.lbs = P.lbs + P.bn;
+  P.cs = P.ncb;
+  P.cn = P.U + P.nbc;
+  P.bs = P.nbb0;
+  P.bn = P.nv + P.nbb;
+}
+
+// End of the enqueued levels: a loop still running means levels are left.
+__global__ void k_dv_loop_end(DevState* st) {
+  if (st->run) st->abort = 1;
+  st->tot_c32 = (int32_t)st->tot_c;
+}
+
+// Bottom segments and the density ratio of polygons that never refine.
+__global__ void k_dv_bottom(PolyDev* pd, int np, DevState* st, const LevelDesc* __restrict__ plv, float* gamma,
+                            int maxd) {
+  if (st->abort) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  PolyDev& P = pd[p];
+  if ((P.depth == 0 && maxd == 1) || P.depth == 1) gamma[p] = (float)(42 * 1.0 / P.N);
+  const LevelDesc B = plv[(int64_t)p * kMaxLevels + P.L];
+  P.bcs = B.cstart;
+  P.bcn = B.cnum;
+  P.refining = 0;
+  atomicMax(&st->maxL, P.L);
+}
+
+// Refs of each polygon after the count walk, and the check that they fit.
+__global__ void k_dv_poly_refs(PolyDev* pd, int np, const DevState* __restrict__ st, const int32_t* off,
+                               const int32_t* cnt, LevelDesc* plv) {
+  if (st->abort) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  PolyDev& P = pd[p];
+  const int last = P.e0 + P.N - 1;
+  P.roff = off[P.e0];
+  P.R = off[last] + cnt[last] - P.roff;
+  plv[(int64_t)p * kMaxLevels + P.L].rnum = P.R;
+}
+
+__global__ void k_dv_check_refs(DevState* st, int64_t rcap) {
+  if (st->abort) return;
+  const int32_t R = st->R;
+  if (R < 0) {
+    st->limit = 1;
+    st->abort = 1;
+  } else if ((int64_t)R + 1 > rcap) {
+    st->overflow = 1;
+    st->need_r = (int64_t)R + 1;
+    st->abort = 1;
+  } else {
+    st->R_ok = R;
+  }
+}
+
+// p[0, *n) = v, with *n read on the device (grid-stride).
+__global__ void k_fill_dv(int32_t* p, const int32_t* __restrict__ n, int32_t v, const DevState* __restrict__ st) {
+  if (st->abort) return;
+  const int m = *n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// d[0, *n) = s[0, *n).
+__global__ void k_copy_dv(int32_t* d, const int32_t* __restrict__ s, const int32_t* __restrict__ n,
+                          const DevState* __restrict__ st) {
+  if (st->abort) return;
+  const int m = *n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__global__ void k_dv_edge_heads(const int32_t* off, const int32_t* epoly, const PolyDev* __restrict__ pd, int Ntot,
+                                int32_t* flag, const DevState* __restrict__ st) {
+  if (st->abort) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Ntot) return;
+  const PolyDev& P = pd[epoly[e]];
+  if (e == P.e0) return;
+  const int o = off[e];
+  if (o < P.roff + P.R) flag[o] = 1;
+}
+
+// ref -> global edge id (as k_ref_edges), and the ref's cell as a sort key.
+__global__ void k_dv_ref_edges(const int32_t* rcell, const int32_t* incl, const int32_t* cpoly,
+                               const PolyDev* __restrict__ pd, const DevState* __restrict__ st, int32_t* redge) {
+  if (st->abort) return;
+  const int R = st->R_ok;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
+    const PolyDev& P = pd[cpoly[rcell[i]]];
+    const int base = P.roff > 0 ? incl[P.roff - 1] : 0;
+    redge[i] = P.e0 + (incl[i] - base);
+  }
+}
+
+__global__ void k_dv_count_cells(const int32_t* rcell, const DevState* __restrict__ st, int32_t* ccnt) {
+  if (st->abort) return;
+  const int R = st->R_ok;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) atomicAdd(&ccnt[rcell[i]], 1);
+}
+
+// kernel_EC2CS_setCEInfo (:1030-1043) for bottom cells: the refs are in
+// cell-major order already (a stable sort by cell of the edge-major refs,
+// so ascending edge order within each cell).
+__global__ void k_dv_finish_cells(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st,
+                                  const int32_t* ccnt, const int32_t* coff) {
+  if (st->abort) return;
+  const int C = st->tot_c32;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < C; t += gridDim.x * blockDim.x) {
+    const PolyDev& P = pd[g.c_poly[t]];
+    if (t >= P.bcs && t < P.bcs + P.bcn) {
+      g.c_rnum[t] = ccnt[t];
+      g.c_rstart[t] = coff[t];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1219,12 +1643,12 @@ int build_hierarchy(gahhg_ctx* c) {
       hipLaunchKernelGGL((k_walk<kMark, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
     else
       hipLaunchKernelGGL((k_walk<kMark, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
     LAUNCH_CK("mark walk");
     hipLaunchKernelGGL(k_plan, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
                        c->d_f0, c->d_f1, c->d_f2, c->d_f3);
@@ -1236,7 +1660,7 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL(k_plan_totals, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
                        GL.cstart, (const int32_t*)c->d_f0, (const int32_t*)c->d_f1, (const int32_t*)c->d_f2,
                        (const int32_t*)c->d_f3, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1,
-                       (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
+                       (const int32_t*)c->d_o2, (const int32_t*)c->d_o3, (const DevState*)nullptr, 0);
     LAUNCH_CK("plan totals");
     // the model runs only at levels where some polygon may be inside its
     // window (crossed ratio within a factor 4 of the floor at the previous
@@ -1388,12 +1812,12 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL((k_walk<kCount, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                       c->d_ecnt, (int32_t*)nullptr, c->d_st);
+                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
   if (bottom_plain)
     hipLaunchKernelGGL((k_walk<kCount, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                       c->d_ecnt, (int32_t*)nullptr, c->d_st);
+                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
   LAUNCH_CK("count walk");
   CK(scan_i32(c, c->d_ecnt, c->d_o0, Nt, nullptr, &c->d_st->R, 0));
   hipLaunchKernelGGL(k_poly_refs, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
@@ -1410,12 +1834,12 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL((k_walk<kWrite, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                       (int32_t*)nullptr, c->d_rcell, c->d_st);
+                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0);
   if (bottom_plain)
     hipLaunchKernelGGL((k_walk<kWrite, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                       (int32_t*)nullptr, c->d_rcell, c->d_st);
+                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0);
   LAUNCH_CK("write walk");
   CK(fill_i32(c, c->d_rflag, R, 0));
   hipLaunchKernelGGL(k_edge_heads, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_o0,
@@ -1444,6 +1868,513 @@ int build_hierarchy(gahhg_ctx* c) {
   LAUNCH_CK("finish cells");
   for (int p = 0; p < np; p++) c->plv[p][c->pd[p].L].rnum = c->pd[p].R;
   CK(sync_state(c));
+  return GAHHG_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Device-driven build: host orchestration
+// ---------------------------------------------------------------------------
+void drop_build_graph(gahhg_ctx* c) {
+  DevBuildState& d = c->dvb;
+  if (d.exec) (void)hipGraphExecDestroy(d.exec);
+  if (d.graph) (void)hipGraphDestroy(d.graph);
+  d.exec = nullptr;
+  d.graph = nullptr;
+  d.key.clear();
+  d.graph_levels = 0;
+}
+
+void free_dev_build(gahhg_ctx* c) {
+  drop_build_graph(c);
+  DevBuildState& d = c->dvb;
+  void* ptrs[] = {d.g0_v, d.g0_n, d.g0_b, d.g0_c, d.pd0, d.plv, d.plvn, d.gamma, d.pmarks, d.hist, d.csums, d.sscr};
+  for (void* q : ptrs) dfree(q);
+  if (d.hbuf) (void)hipHostFree(d.hbuf);
+  d = DevBuildState();
+}
+
+namespace {
+
+// Arena of one device-driven build: what each array holds.
+struct DvArena {
+  int64_t v, b, c, r;
+  int end_bit;  // key bits of the ref sort (cells < 2^end_bit)
+};
+
+int dv_upload(gahhg_ctx* c, void* dst, const void* src, int64_t bytes) {
+  return hip_check(c, hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice), "device build upload");
+}
+
+// The build's scan scratch for scans of up to n elements.
+int dv_scan_scratch(gahhg_ctx* c, int64_t n) {
+  DevBuildState& d = c->dvb;
+  const int64_t need = scan_scratch_entries(n);
+  if (d.sscr_cap >= need) return GAHHG_OK;
+  CK_SYNC(c);
+  dfree(d.sscr);
+  d.sscr = nullptr;
+  d.sscr_cap = 0;
+  CK(dalloc(c, (void**)&d.sscr, need * 4));
+  d.sscr_cap = need;
+  return GAHHG_OK;
+}
+
+// Everything the enqueued build reads or writes, sized before anything is
+// enqueued (nothing is allocated while the build runs or is recorded).
+int dv_prepare(gahhg_ctx* c, const int64_t* want, DvArena* a) {
+  DevBuildState& d = c->dvb;
+  const int np = (int)c->polys.size();
+  const int64_t Nt = c->n_edges;
+  const int64_t V0 = (int64_t)kGrid0Verts * np, B0 = (int64_t)kGrid0Borders * np, C0 = (int64_t)kGrid0Cells * np;
+  // the reference's initial capacities (createAHHGInit :1949-1953, summed
+  // over the batch), or what the last build of this polygon set used plus
+  // an eighth, or what an overflowed attempt asked for
+  int64_t ev = std::max<int64_t>(Nt, 160) + V0, eb = std::max<int64_t>(4 * Nt, 480) + B0;
+  int64_t ec = std::max<int64_t>(Nt, 84) + C0, er = std::max<int64_t>(2 * Nt, 1024);
+  if (d.shape_n == Nt && d.shape_np == np) {
+    ev = std::max(ev, d.used_v + d.used_v / 8);
+    eb = std::max(eb, d.used_b + d.used_b / 8);
+    ec = std::max(ec, d.used_c + d.used_c / 8);
+    er = std::max(er, d.used_r + d.used_r / 8 + 1);
+  }
+  ev = std::max(ev, want[0]);
+  eb = std::max(eb, want[1]);
+  ec = std::max(ec, want[2]);
+  er = std::max(er, want[3]);
+  CK(ensure_v(c, 0, ev));
+  CK(ensure_b(c, 0, eb));
+  CK(ensure_c(c, 0, ec));
+  CK(ensure_flags(c, std::max<int64_t>(Nt, c->g.ccap) + 1));
+  CK(ensure_refs(c, er));
+  // walk records for at most 4M cells (1.4 GB): deeper levels of larger
+  // builds walk without them
+  CK(ensure_wrec(c, 0, std::min<int64_t>(c->g.ccap, 1 << 22)));
+  if (c->ecap < Nt) {
+    dfree(c->d_ecnt);
+    c->d_ecnt = nullptr;
+    CK(dalloc(c, (void**)&c->d_ecnt, Nt * 4));
+    c->ecap = Nt;
+  }
+  a->v = c->g.vcap;
+  a->b = c->g.bcap;
+  a->c = c->g.ccap;
+  a->r = std::min(c->r2cap, c->g.rcap);
+  a->end_bit = 1;
+  while ((1LL << a->end_bit) < a->c) a->end_bit++;
+  // radix histograms and every scan's scratch, reserved now
+  const int64_t hn = radix_hist_entries(a->r);
+  if (d.hist_cap < hn) {
+    CK_SYNC(c);
+    dfree(d.hist);
+    d.hist = nullptr;
+    d.hist_cap = 0;
+    CK(dalloc(c, (void**)&d.hist, hn * 4));
+    d.hist_cap = hn;
+  }
+  CK(dv_scan_scratch(c, std::max<int64_t>(Nt, hn)));
+  if (!d.csums) CK(dalloc(c, (void**)&d.csums, kScanColsSums * 4));
+  // per-polygon arrays
+  if (d.pcap < np) {
+    CK_SYNC(c);
+    void* ptrs[] = {d.pd0, d.plv, d.plvn, d.gamma, d.pmarks};
+    for (void* q : ptrs) dfree(q);
+    d.pd0 = nullptr, d.plv = nullptr, d.plvn = nullptr, d.gamma = nullptr, d.pmarks = nullptr;
+    d.pcap = 0;
+    d.pd0_host.clear();
+    CK(dalloc(c, (void**)&d.pd0, (int64_t)np * (int64_t)sizeof(PolyDev)));
+    CK(dalloc(c, (void**)&d.plv, (int64_t)np * kMaxLevels * (int64_t)sizeof(LevelDesc)));
+    CK(dalloc(c, (void**)&d.plvn, (int64_t)np * 4));
+    CK(dalloc(c, (void**)&d.gamma, (int64_t)np * 4));
+    CK(dalloc(c, (void**)&d.pmarks, (int64_t)np * 4));
+    d.pcap = np;
+  }
+  if (d.hcap < np) {
+    CK_SYNC(c);
+    if (d.hbuf) (void)hipHostFree(d.hbuf);
+    d.hbuf = nullptr;
+    d.hcap = 0;
+    const size_t hb = (size_t)np * (sizeof(PolyDev) + kMaxLevels * sizeof(LevelDesc) + 12);
+    if (hipHostMalloc((void**)&d.hbuf, hb, hipHostMallocDefault) != hipSuccess) {
+      d.hbuf = nullptr;
+      return set_err(c, GAHHG_E_NOMEM, "hipHostMalloc(%zu)", hb);
+    }
+    d.hcap = np;
+  }
+  // level-0 template, once per context
+  if (!d.g0_v) {
+    std::vector<double> v0(3 * kGrid0Verts), bn0(3 * kGrid0Borders);
+    std::vector<int32_t> bi0(6 * kGrid0Borders), ci0(7 * kGrid0Cells);
+    build_grid0(v0.data(), bi0.data(), bn0.data(), ci0.data());
+    CK(dalloc(c, (void**)&d.g0_v, (int64_t)v0.size() * 8));
+    CK(dalloc(c, (void**)&d.g0_n, (int64_t)bn0.size() * 8));
+    CK(dalloc(c, (void**)&d.g0_b, (int64_t)bi0.size() * 4));
+    CK(dalloc(c, (void**)&d.g0_c, (int64_t)ci0.size() * 4));
+    CK(dv_upload(c, d.g0_v, v0.data(), (int64_t)v0.size() * 8));
+    CK(dv_upload(c, d.g0_n, bn0.data(), (int64_t)bn0.size() * 8));
+    CK(dv_upload(c, d.g0_b, bi0.data(), (int64_t)bi0.size() * 4));
+    CK(dv_upload(c, d.g0_c, ci0.data(), (int64_t)ci0.size() * 4));
+  }
+  // polygon records at the start of a build (as build_hierarchy sets them)
+  const int maxd = c->prm.max_depth;
+  std::vector<PolyDev> t((size_t)np);
+  int64_t e0 = 0;
+  for (int p = 0; p < np; p++) {
+    PolyDev& P = t[(size_t)p];
+    memset(&P, 0, sizeof(P));
+    P.e0 = (int32_t)e0;
+    P.N = c->polys[p].N;
+    P.cell0 = kGrid0Cells * p;
+    P.refining = maxd > 1 ? 1 : 0;
+    P.cs = kGrid0Cells * p;
+    P.cn = kGrid0Cells;
+    P.bs = kGrid0Borders * p;
+    P.bn = kGrid0Borders;
+    P.tbl = -1;
+    P.depth = p < (int)c->depth_override.size() ? c->depth_override[p] : 0;
+    if (P.depth == 1) P.refining = 0;
+    e0 += P.N;
+  }
+  if (d.pd0_host.size() != t.size() || memcmp(d.pd0_host.data(), t.data(), t.size() * sizeof(PolyDev)) != 0) {
+    CK_SYNC(c);  // a replay in flight may still read the old template
+    CK(dv_upload(c, d.pd0, t.data(), (int64_t)t.size() * (int64_t)sizeof(PolyDev)));
+    d.pd0_host = std::move(t);
+  }
+  return GAHHG_OK;
+}
+
+unsigned dv_grid(int64_t n) { return (unsigned)std::min<int64_t>(blocks_for(n, kBlock), kDvGrid); }
+
+// Level 0 of every polygon, the state and the polygon records.
+int dv_enqueue_start(gahhg_ctx* c) {
+  DevBuildState& d = c->dvb;
+  const int np = (int)c->polys.size();
+  const int64_t Nt = c->n_edges;
+  hipLaunchKernelGGL(k_edges, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const double*)c->d_pv,
+                     (const int32_t*)c->d_pe, (int)Nt, c->d_edge);
+  hipLaunchKernelGGL(k_dv_start, dim3(1), dim3(kCtlBlock), 0, c->stream, c->d_st, c->d_pd, (const PolyDev*)d.pd0, np,
+                     d.plv, d.plvn, d.pmarks, d.gamma);
+  hipLaunchKernelGGL(k_dv_grid0, dim3(np), dim3(kCtlBlock), 0, c->stream, view(c), (const double*)d.g0_v,
+                     (const double*)d.g0_n, (const int32_t*)d.g0_b, (const int32_t*)d.g0_c);
+  CK(hip_check(c, hipGetLastError(), "device build start"));
+  CK(fill_i32(c, c->d_loc_cur, Nt, 0));
+  return fill_i32(c, c->d_loc_prev, Nt, 0);
+}
+
+// Iteration lv of createAHHG's level loop (mark, plan, density rule,
+// create), every launch a no-op once the loop has ended.
+int dv_enqueue_level(gahhg_ctx* c, const DvArena& a, int lv) {
+  DevBuildState& d = c->dvb;
+  const int np = (int)c->polys.size();
+  const int64_t Nt = c->n_edges;
+  const GridView g = view(c);
+  const unsigned cg = dv_grid(a.c);
+  hipLaunchKernelGGL(k_lv_begin, dim3(1), dim3(kCtlBlock), 0, c->stream, c->d_st, c->d_pd, np, lv,
+                     (int)c->prm.max_depth, d.pmarks, c->wrec_cap);
+  hipLaunchKernelGGL(k_walk_records_dv, dim3((unsigned)std::min<int64_t>(blocks_for(a.c, kRecBlock), kDvGrid)),
+                     dim3(kRecBlock), 0, c->stream, g, (const DevState*)c->d_st, lv, c->d_wrec);
+  hipLaunchKernelGGL(k_zero_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const DevState*)c->d_st, lv);
+  hipLaunchKernelGGL((k_walk<kMark, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                     (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                     (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
+                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+  hipLaunchKernelGGL((k_walk<kMark, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                     (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                     (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
+                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+  hipLaunchKernelGGL(k_plan_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const DevState*)c->d_st, lv, c->d_f0,
+                     c->d_f1, c->d_f2, c->d_f3);
+  CK(hip_check(c, hipGetLastError(), "device build level"));
+  ScanCols sc = {{c->d_f0, c->d_f1, c->d_f2, c->d_f3}, {c->d_o0, c->d_o1, c->d_o2, c->d_o3}, {nullptr, nullptr, nullptr, nullptr}};
+  CK(scan_cols_dev(c, sc, 4, &c->d_st->lv[lv].cnum, 0, d.csums));
+  hipLaunchKernelGGL(k_plan_totals, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np, 0,
+                     (const int32_t*)c->d_f0, (const int32_t*)c->d_f1, (const int32_t*)c->d_f2,
+                     (const int32_t*)c->d_f3, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1,
+                     (const int32_t*)c->d_o2, (const int32_t*)c->d_o3, (const DevState*)c->d_st, lv);
+  hipLaunchKernelGGL(k_lv_step, dim3(1), dim3(kCtlBlock), 0, c->stream, c->d_pd, np, c->d_st, lv,
+                     c->prm.auto_depth != 0 ? 1 : 0, d.gamma, a.v, a.b, a.c);
+  hipLaunchKernelGGL(k_create_center_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, lv, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
+  hipLaunchKernelGGL(k_border_children_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, lv, (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
+  hipLaunchKernelGGL(k_sc_partner_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, lv);
+  hipLaunchKernelGGL(k_lv_advance, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
+                     (const DevState*)c->d_st, lv, d.plv, d.plvn);
+  return hip_check(c, hipGetLastError(), "device build level");
+}
+
+// dispatchSELast (:1700-1784) for every polygon at its bottom level, first
+// part: bottom segments, the count walk and the refs of each polygon (their
+// total in st->R).
+int dv_enqueue_count(gahhg_ctx* c) {
+  DevBuildState& d = c->dvb;
+  const int np = (int)c->polys.size();
+  const int64_t Nt = c->n_edges;
+  const GridView g = view(c);
+  hipLaunchKernelGGL(k_dv_loop_end, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  hipLaunchKernelGGL(k_dv_bottom, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np, c->d_st,
+                     (const LevelDesc*)d.plv, d.gamma, (int)c->prm.max_depth);
+  for (int w = 0; w < 2; w++) {
+    if (w == 0)
+      hipLaunchKernelGGL((k_walk<kCount, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                         (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                         (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+    else
+      hipLaunchKernelGGL((k_walk<kCount, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                         (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                         (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+  }
+  CK(hip_check(c, hipGetLastError(), "count walk"));
+  CK(scan_i32_with(c, c->d_ecnt, c->d_o0, Nt, nullptr, &c->d_st->R, 0, d.sscr));
+  hipLaunchKernelGGL(k_dv_poly_refs, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
+                     (const DevState*)c->d_st, (const int32_t*)c->d_o0, (const int32_t*)c->d_ecnt, d.plv);
+  return hip_check(c, hipGetLastError(), "polygon refs");
+}
+
+// Second part: the check that the refs fit, the write walk, the refs in
+// cell-major order, then the classification of `levels` levels.
+int dv_enqueue_write(gahhg_ctx* c, const DvArena& a, int levels) {
+  DevBuildState& d = c->dvb;
+  const int64_t Nt = c->n_edges;
+  const GridView g = view(c);
+  const unsigned rg = dv_grid(a.r), cg = dv_grid(a.c);
+  hipLaunchKernelGGL(k_dv_check_refs, dim3(1), dim3(1), 0, c->stream, c->d_st, a.r);
+  for (int w = 0; w < 2; w++) {
+    if (w == 0)
+      hipLaunchKernelGGL((k_walk<kWrite, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                         (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                         (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
+                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1);
+    else
+      hipLaunchKernelGGL((k_walk<kWrite, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
+                         (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
+                         (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
+                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1);
+  }
+  hipLaunchKernelGGL(k_fill_dv, dim3(rg), dim3(kBlock), 0, c->stream, c->d_rflag, (const int32_t*)&c->d_st->R_ok, 0,
+                     (const DevState*)c->d_st);
+  hipLaunchKernelGGL(k_dv_edge_heads, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream,
+                     (const int32_t*)c->d_o0, (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt,
+                     c->d_rflag, (const DevState*)c->d_st);
+  CK(hip_check(c, hipGetLastError(), "write walk"));
+  ScanCols s1 = {{c->d_rflag, nullptr, nullptr, nullptr}, {c->d_rscan, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  CK(scan_cols_dev(c, s1, 1, &c->d_st->R_ok, 1, d.csums));
+  hipLaunchKernelGGL(k_dv_ref_edges, dim3(rg), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
+                     (const int32_t*)c->d_rscan, (const int32_t*)c->g.c_poly, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, c->d_redge);
+  // refs per cell, their offsets, and the refs in cell-major order by a
+  // stable sort of (cell, edge) over the edge-major refs
+  hipLaunchKernelGGL(k_fill_dv, dim3(cg), dim3(kBlock), 0, c->stream, c->d_f1, (const int32_t*)&c->d_st->tot_c32, 0,
+                     (const DevState*)c->d_st);
+  hipLaunchKernelGGL(k_dv_count_cells, dim3(rg), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
+                     (const DevState*)c->d_st, c->d_f1);
+  CK(hip_check(c, hipGetLastError(), "ref counts"));
+  ScanCols s2 = {{c->d_f1, nullptr, nullptr, nullptr}, {c->d_o1, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  CK(scan_cols_dev(c, s2, 1, &c->d_st->tot_c32, 0, d.csums));
+  uint32_t *kres = nullptr, *vres = nullptr;
+  CK(radix_sort_pairs(c, (uint32_t*)c->d_rcell, (uint32_t*)c->d_redge, (uint32_t*)c->d_rflag, (uint32_t*)c->d_rscan,
+                      &c->d_st->R_ok, a.r, a.end_bit, d.hist, &kres, &vres, d.sscr));
+  hipLaunchKernelGGL(k_copy_dv, dim3(rg), dim3(kBlock), 0, c->stream, c->g.refs, (const int32_t*)vres,
+                     (const int32_t*)&c->d_st->R_ok, (const DevState*)c->d_st);
+  hipLaunchKernelGGL(k_dv_finish_cells, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+                     (const DevState*)c->d_st, (const int32_t*)c->d_f1, (const int32_t*)c->d_o1);
+  CK(hip_check(c, hipGetLastError(), "cell refs"));
+  return classify_dev_enqueue(c, levels);
+}
+
+// What a recorded build depends on: every pointer and size baked into it.
+std::vector<uintptr_t> dv_key(gahhg_ctx* c, const DvArena& a, int levels) {
+  const DevBuildState& d = c->dvb;
+  const DeviceGrid& g = c->g;
+  std::vector<uintptr_t> k = {
+      (uintptr_t)c->polys.size(), (uintptr_t)c->n_edges, (uintptr_t)c->prm.max_depth, (uintptr_t)c->prm.auto_depth,
+      (uintptr_t)levels, (uintptr_t)a.v, (uintptr_t)a.b, (uintptr_t)a.c, (uintptr_t)a.r, (uintptr_t)a.end_bit,
+      (uintptr_t)c->wrec_cap, (uintptr_t)c->fcap, (uintptr_t)d.hist_cap,
+      (uintptr_t)c->stream, (uintptr_t)c->d_pv, (uintptr_t)c->d_pe, (uintptr_t)c->d_edge, (uintptr_t)c->d_epoly,
+      (uintptr_t)c->d_pd, (uintptr_t)c->d_st, (uintptr_t)c->d_loc_cur, (uintptr_t)c->d_loc_prev,
+      (uintptr_t)c->d_f0, (uintptr_t)c->d_f1, (uintptr_t)c->d_f2, (uintptr_t)c->d_f3, (uintptr_t)c->d_o0,
+      (uintptr_t)c->d_o1, (uintptr_t)c->d_o2, (uintptr_t)c->d_o3, (uintptr_t)c->d_ecnt,
+      (uintptr_t)c->d_rcell, (uintptr_t)c->d_rflag, (uintptr_t)c->d_redge, (uintptr_t)c->d_rscan,
+      (uintptr_t)c->d_wrec, (uintptr_t)d.g0_v, (uintptr_t)d.g0_n, (uintptr_t)d.g0_b, (uintptr_t)d.g0_c,
+      (uintptr_t)d.pd0, (uintptr_t)d.plv, (uintptr_t)d.plvn, (uintptr_t)d.gamma, (uintptr_t)d.pmarks,
+      (uintptr_t)d.hist, (uintptr_t)d.csums, (uintptr_t)d.sscr, (uintptr_t)d.sscr_cap, (uintptr_t)g.v, (uintptr_t)g.b_parent, (uintptr_t)g.b_partner,
+      (uintptr_t)g.b_v0, (uintptr_t)g.b_v1, (uintptr_t)g.b_subcell, (uintptr_t)g.b_id, (uintptr_t)g.b_n,
+      (uintptr_t)g.c_bstart, (uintptr_t)g.c_bnum, (uintptr_t)g.c_rstart, (uintptr_t)g.c_rnum, (uintptr_t)g.c_upcell,
+      (uintptr_t)g.c_subcell, (uintptr_t)g.c_type, (uintptr_t)g.c_poly, (uintptr_t)g.c_root, (uintptr_t)g.refs};
+  return k;
+}
+
+// Levels to enqueue before the first look at whether the loop still runs:
+// those of the last build of this polygon set, else an estimate (a level
+// per factor 4 of edges per polygon; the density rule stops near there).
+int dv_depth_hint(gahhg_ctx* c) {
+  const DevBuildState& d = c->dvb;
+  const int maxd = c->prm.max_depth;
+  if (d.shape_n == c->n_edges && d.shape_np == (int)c->polys.size() && d.used_levels > 0)
+    return std::min(maxd, d.used_levels);
+  int64_t nmax = 1;
+  for (const Polygon& p : c->polys) nmax = std::max<int64_t>(nmax, p.N);
+  int h = 2;
+  while ((1LL << (2 * (h - 1))) < nmax) h++;
+  return std::min(maxd, h);
+}
+
+}  // namespace
+
+int build_classify_dev(gahhg_ctx* c, bool* replayed, int32_t* attempts) {
+  DevBuildState& d = c->dvb;
+  const int np = (int)c->polys.size();
+  const int maxd = c->prm.max_depth;
+  *replayed = false;
+  *attempts = 0;
+  int64_t want[4] = {0, 0, 0, 0};
+  DvArena a{};
+  int levels = 0;
+  bool done = false;
+  for (int attempt = 0; attempt < 16 && !done; attempt++) {
+    *attempts = attempt + 1;
+    *replayed = false;
+    CK(dv_prepare(c, want, &a));
+    const int glv = d.graph_levels;
+    std::vector<uintptr_t> key = dv_key(c, a, glv);
+    if (d.exec && glv > 0 && key == d.key) {
+      CK(hip_check(c, hipGraphLaunch(d.exec, c->stream), "build graph launch"));
+      levels = glv;
+      *replayed = true;
+    } else {
+      // eager: levels up to the hint, then one word read back per further
+      // pair of levels until the loop has ended
+      CK(dv_enqueue_start(c));
+      int lv = 0, upto = dv_depth_hint(c);
+      for (;;) {
+        for (; lv < upto; lv++) CK(dv_enqueue_level(c, a, lv));
+        if (lv >= maxd) break;
+        int32_t run = 0;
+        CK(hip_check(c, hipMemcpyAsync(&c->h_st->run, &c->d_st->run, 4, hipMemcpyDeviceToHost, c->stream),
+                     "loop state"));
+        CK_SYNC(c);
+        run = c->h_st->run;
+        if (!run) break;
+        upto = std::min(maxd, lv + 2);
+      }
+      levels = lv;
+      CK(dv_enqueue_count(c));
+      // the refs' total is known now: grow the ref arrays to it here rather
+      // than let the device stop the build short
+      CK(hip_check(c, hipMemcpyAsync(&c->h_st->R, &c->d_st->R, 4, hipMemcpyDeviceToHost, c->stream), "ref count"));
+      CK(hip_check(c, hipMemcpyAsync(&c->h_st->abort, &c->d_st->abort, 4, hipMemcpyDeviceToHost, c->stream),
+                   "ref count"));
+      CK_SYNC(c);
+      if (!c->h_st->abort && c->h_st->R >= 0 && (int64_t)c->h_st->R + 1 > a.r) {
+        const int64_t R = (int64_t)c->h_st->R + 1;
+        CK(ensure_refs(c, R + R / 8));
+        a.r = std::min(c->r2cap, c->g.rcap);
+        const int64_t hn = radix_hist_entries(a.r);
+        if (d.hist_cap < hn) {
+          dfree(d.hist);
+          d.hist = nullptr;
+          d.hist_cap = 0;
+          CK(dalloc(c, (void**)&d.hist, hn * 4));
+          d.hist_cap = hn;
+        }
+        CK(dv_scan_scratch(c, std::max<int64_t>(c->n_edges, hn)));
+      }
+      CK(dv_enqueue_write(c, a, levels));
+    }
+    // the state and the per-polygon results, one wait for all
+    char* hb = d.hbuf;
+    CK(hip_check(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream), "readback"));
+    CK(hip_check(c, hipMemcpyAsync(hb, c->d_pd, (size_t)np * sizeof(PolyDev), hipMemcpyDeviceToHost, c->stream),
+                 "readback"));
+    hb += (size_t)np * sizeof(PolyDev);
+    CK(hip_check(c, hipMemcpyAsync(hb, d.plv, (size_t)np * kMaxLevels * sizeof(LevelDesc), hipMemcpyDeviceToHost,
+                                   c->stream),
+                 "readback"));
+    hb += (size_t)np * kMaxLevels * sizeof(LevelDesc);
+    CK(hip_check(c, hipMemcpyAsync(hb, d.plvn, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream), "readback"));
+    CK(hip_check(c, hipMemcpyAsync(hb + (size_t)np * 4, d.pmarks, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream),
+                 "readback"));
+    CK(hip_check(c, hipMemcpyAsync(hb + (size_t)np * 8, d.gamma, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream),
+                 "readback"));
+    CK_SYNC(c);
+    const DevState& s = *c->h_st;
+    if (s.limit) return set_err(c, GAHHG_E_LIMIT, "build: a level or the refs exceed int32 indexing");
+    if (s.overflow) {
+      // grow what the attempt ran out of and start again
+      drop_build_graph(c);
+      // (a level's need is a lower bound of the hierarchy's: the deeper
+      // levels grow it further, so the arena grows four-fold past it)
+      if (s.need_v > a.v || s.need_b > a.b || s.need_c > a.c) {
+        want[0] = std::max(want[0], 4 * s.need_v);
+        want[1] = std::max(want[1], 4 * s.need_b);
+        want[2] = std::max(want[2], 4 * s.need_c);
+      }
+      want[3] = std::max(want[3], s.need_r + s.need_r / 8);
+      continue;
+    }
+    if (s.abort) {
+      // a replay whose level loop needed more levels than it recorded
+      drop_build_graph(c);
+      d.used_levels = maxd;
+      continue;
+    }
+    done = true;
+  }
+  if (!done) return set_err(c, GAHHG_E_STATE, "build: the arena did not converge");
+  // host mirrors: levels, polygon records and their level segments
+  const DevState& s = *c->h_st;
+  const PolyDev* hpd = (const PolyDev*)d.hbuf;
+  const LevelDesc* hplv = (const LevelDesc*)(d.hbuf + (size_t)np * sizeof(PolyDev));
+  const int32_t* hplvn = (const int32_t*)((const char*)hplv + (size_t)np * kMaxLevels * sizeof(LevelDesc));
+  c->pd.assign(hpd, hpd + np);
+  c->plv.assign((size_t)np, std::vector<LevelDesc>());
+  for (int p = 0; p < np; p++) c->plv[p].assign(hplv + (int64_t)p * kMaxLevels, hplv + (int64_t)p * kMaxLevels + hplvn[p]);
+  c->pmarks.assign(hplvn + np, hplvn + 2 * np);
+  const float* hg = (const float*)(hplvn + 2 * np);
+  c->pgamma.assign(hg, hg + np);
+  c->glv.assign(s.lv, s.lv + s.gnum);
+  c->gnum = s.gnum;
+  c->tot_v = s.tot_v;
+  c->tot_b = s.tot_b;
+  c->tot_c = s.tot_c;
+  c->tot_r = s.R;
+  c->und_snap = (int64_t)s.und_first;
+  c->und_final = (int64_t)s.undefined_leaves;
+  d.used_v = s.tot_v;
+  d.used_b = s.tot_b;
+  d.used_c = s.tot_c;
+  d.used_r = (int64_t)s.R + 1;
+  d.used_levels = s.gnum;
+  d.shape_n = c->n_edges;
+  d.shape_np = np;
+  // record the build for the next one of this polygon set (the levels it
+  // needed; a polygon set that needs more stops the replay short, and the
+  // next build runs eagerly and records again)
+  if (!*replayed && c->prm.build_graph) {
+    drop_build_graph(c);
+    const int gl = s.gnum;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ex = nullptr;
+    CK(hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "build capture"));
+    int rc = dv_enqueue_start(c);
+    for (int lv = 0; lv < gl && !rc; lv++) rc = dv_enqueue_level(c, a, lv);
+    if (!rc) rc = dv_enqueue_count(c);
+    if (!rc) rc = dv_enqueue_write(c, a, gl);
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (!rc) rc = hip_check(c, e, "build capture end");
+    if (!rc) rc = hip_check(c, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0), "build graph instantiate");
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    d.graph = gr;
+    d.exec = ex;
+    d.graph_levels = gl;
+    d.key = dv_key(c, a, gl);
+  }
   return GAHHG_OK;
 }
 

@@ -269,6 +269,8 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_xcd = 32;
   p->query_table_budget = 8;
   p->query_graph = 0;
+  p->build_device = 1;
+  p->build_graph = 1;
   return GAHHG_OK;
 }
 
@@ -289,7 +291,8 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       c->prm.query_sort > 2 || c->prm.query_xcd < 0 || c->prm.query_xcd > 4096 ||
       c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14 || c->prm.query_index < 0 ||
       c->prm.query_index > 1 || c->prm.query_table_budget < 0 || c->prm.query_graph < 0 ||
-      c->prm.query_graph > 1) {
+      c->prm.query_graph > 1 || c->prm.build_device < 0 || c->prm.build_device > 1 || c->prm.build_graph < 0 ||
+      c->prm.build_graph > 1) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -321,6 +324,7 @@ void gahhg_destroy(gahhg_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_dev_build(c);
   free_grid(c);
   free_scratch(c);
   dfree(c->qsort.buf);
@@ -522,11 +526,22 @@ int gahhg_build(gahhg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   reset_grid(c);
   const double t0 = now_ms();
-  RCK(build_hierarchy(c));
-  const double t1 = now_ms();
-  RCK(classify_hierarchy(c));
-  RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
-  const double tl = now_ms();
+  double t1 = t0, tl = t0;
+  if (c->prm.build_device && !c->learn.on) {
+    // level loop, dispatch and classification decided on the device
+    bool replayed = false;
+    RCK(build_classify_dev(c, &replayed, &c->build_attempts));
+    t1 = tl = now_ms();
+    c->build_replayed = replayed;
+  } else {
+    RCK(build_hierarchy(c));
+    t1 = now_ms();
+    RCK(classify_hierarchy(c));
+    RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
+    tl = now_ms();
+    c->build_replayed = false;
+    c->build_attempts = 0;
+  }
   RCK(pack_query_layout(c));
   RCK(hip_check(c, hipStreamSynchronize(c->stream), "sync"));
   const double t2 = now_ms();
@@ -560,6 +575,8 @@ int gahhg_build(gahhg_ctx* c) {
   c->st.walk_unlocated = (int64_t)s.unlocated;
   c->st.undefined_leaves_snapshot = c->und_snap;
   c->st.undefined_leaves_final = c->und_final;
+  c->st.build_replayed = c->build_replayed ? 1 : 0;
+  c->st.build_attempts = c->build_attempts;
   return GAHHG_OK;
 }
 

@@ -36,7 +36,7 @@ static inline unsigned gblocks(int64_t n, int G) {
 // Lanes per cell for the per-ref loops of a level: the smallest power of two
 // >= a quarter of the mean refs per bottom cell, in [1, 64] (small groups
 // keep many cells per wave where cells hold few refs).
-int group_for_refs(double mean_refs) {
+__host__ __device__ int group_for_refs(double mean_refs) {
   int G = 1;
   while (G < 64 && G < 0.25 * mean_refs) G *= 2;
   return G;
@@ -263,12 +263,9 @@ __device__ int parity_label_group(const GridView& g, const double* __restrict__ 
 // setBottomDECell (a lane group per cell): every border whose neighbour was
 // labelled in the snapshot decides in turn; the last one wins, as in the
 // reference loop.
-__global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev* __restrict__ pd,
-                                                       const double* edge, int cstart, int cnum, int lv,
-                                                       const int32_t* snap, int G) {
-  const Group gr = group_of(G);
-  const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
-  if (t >= cnum || snap[t] != kUndefined) return;
+__device__ __forceinline__ void de_bottom_cell(const GridView& g, const PolyDev* __restrict__ pd, const double* edge,
+                                               int cstart, int t, int lv, const int32_t* snap, const Group& gr) {
+  if (snap[t] != kUndefined) return;
   const int cid = cstart + t;
   if (cell_phase(g, pd, cid, lv) != 0) return;
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
@@ -286,12 +283,17 @@ __global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev
   if (set && gr.gl == 0) g.c_type[cid] = lab;
 }
 
+__global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev* __restrict__ pd,
+                                                       const double* edge, int cstart, int cnum, int lv,
+                                                       const int32_t* snap, int G) {
+  const Group gr = group_of(G);
+  const int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  if (t < cnum) de_bottom_cell(g, pd, edge, cstart, t, lv, snap, gr);
+}
+
 // setIECell round: an unlabelled leaf adopts the label of the first
 // neighbour that is an empty labelled cell.
-__global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum, const int32_t* snap,
-                                                DevState* st) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnum) return;
+__device__ __forceinline__ void ie_cell(const GridView& g, int cstart, int t, const int32_t* snap, DevState* st) {
   const int cid = cstart + t;
   if (snap[t] != kUndefined || g.c_subcell[cid] != -1) return;
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
@@ -308,13 +310,16 @@ __global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum
   }
 }
 
+__global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum, const int32_t* snap,
+                                                DevState* st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < cnum) ie_cell(g, cstart, t, snap, st);
+}
+
 // setDECell: an empty unlabelled cell above the bottom takes its label from
 // the leaf reached through the first refined neighbour.
-__global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const PolyDev* __restrict__ pd,
-                                                      const double* edge, int cstart, int cnum, int lv) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnum) return;
-  const int cid = cstart + t;
+__device__ __forceinline__ void de_upper_cell(const GridView& g, const PolyDev* __restrict__ pd, const double* edge,
+                                              int cid, int lv) {
   if (cell_phase(g, pd, cid, lv) != 1) return;
   if (!(g.c_rnum[cid] == 0 && g.c_type[cid] == kUndefined)) return;
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
@@ -336,11 +341,18 @@ __global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const PolyDev*
     g.c_type[cid] = parity_label(g, edge, target, pe_, g.c_type[target]);
 }
 
+__global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const PolyDev* __restrict__ pd,
+                                                      const double* edge, int cstart, int cnum, int lv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < cnum) de_upper_cell(g, pd, edge, cstart + t, lv);
+}
+
 // setTopECell on one polygon's 42 level-0 cells (block = polygon),
 // repeated to a fixpoint.
-__global__ void k_top(GridView g, const PolyDev* __restrict__ pd) {
+__global__ void k_top(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ gate) {
   __shared__ int snap[64];
   __shared__ int changed;
+  if (gate && gate->abort) return;
   const int base = pd[blockIdx.x].cell0;
   const int cnum = kGrid0Cells;
   const int t = threadIdx.x;
@@ -372,6 +384,149 @@ __global__ void k_count_undef(GridView g, int cstart, int cnum, DevState* st) {
   if (t >= cnum) return;
   const int cid = cstart + t;
   if (g.c_subcell[cid] == -1 && g.c_type[cid] == kUndefined) atomicAdd(&st->undefined_leaves, 1ULL);
+}
+
+// ---- device-driven classification: sizes, lane groups and the fixpoint on
+// the device (every kernel returns at once when the build stopped short or
+// the level is below every polygon's bottom) ----
+__device__ __forceinline__ bool cls_skip(const DevState* st, int lv) { return st->abort || lv > st->maxL; }
+
+#define FOR_LEVEL(L, t) \
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < (L).cnum; t += gridDim.x * blockDim.x)
+
+// Lane-group size of level lv (as the host version chooses it) and the
+// fixpoint list cleared.
+__global__ void __launch_bounds__(256) k_cls_level(const PolyDev* __restrict__ pd, int np, DevState* st, int lv) {
+  if (cls_skip(st, lv)) return;
+  __shared__ long long sr[256], sl[256];
+  long long r = 0, l = 0;
+  for (int p = threadIdx.x; p < np; p += 256)
+    if (pd[p].L == lv) {
+      r += pd[p].R;
+      l += pd[p].bcn;
+    }
+  sr[threadIdx.x] = r;
+  sl[threadIdx.x] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long refs = 0, leaves = 0;
+    for (int i = 0; i < 256; i++) refs += sr[i], leaves += sl[i];
+    st->cls_G[lv] = group_for_refs(leaves > 0 ? (double)refs / leaves : 1.0);
+    st->ie_n = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kCBlock) k_ne_dv(GridView g, const PolyDev* __restrict__ pd, const double* pv,
+                                                   const int32_t* pe, const double* edge, const DevState* st, int lv) {
+  if (cls_skip(st, lv)) return;
+  const int G = st->cls_G[lv];
+  const Group gr = group_of(G);
+  const LevelDesc L = st->lv[lv];
+  const int groups = (int)((int64_t)gridDim.x * blockDim.x / G);
+  for (int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G); w < L.cnum; w += groups) {
+    const int cid = L.cstart + w;
+    if (cell_phase(g, pd, cid, lv) != 0) continue;
+    const int rn = g.c_rnum[cid];
+    if (rn == 0) continue;
+    const V3 center = mean_center(g, cid);
+    const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pv, pe, edge, center, gr);
+    if (gr.gl == 0) g.c_type[cid] = cp_type(ch);
+  }
+}
+
+__global__ void k_snap_dv(const int32_t* type, const DevState* st, int lv, int32_t* snap) {
+  if (cls_skip(st, lv)) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL(L, t) snap[t] = type[L.cstart + t];
+}
+
+__global__ void __launch_bounds__(kCBlock) k_de_bottom_dv(GridView g, const PolyDev* __restrict__ pd,
+                                                          const double* edge, const DevState* st, int lv,
+                                                          const int32_t* snap) {
+  if (cls_skip(st, lv)) return;
+  const int G = st->cls_G[lv];
+  const Group gr = group_of(G);
+  const LevelDesc L = st->lv[lv];
+  const int groups = (int)((int64_t)gridDim.x * blockDim.x / G);
+  for (int t = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G); t < L.cnum; t += groups)
+    de_bottom_cell(g, pd, edge, L.cstart, t, lv, snap, gr);
+}
+
+__global__ void __launch_bounds__(kCBlock) k_de_upper_dv(GridView g, const PolyDev* __restrict__ pd,
+                                                         const double* edge, const DevState* st, int lv) {
+  if (cls_skip(st, lv)) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL(L, t) de_upper_cell(g, pd, edge, L.cstart + t, lv);
+}
+
+// setIECell's first round over the level.
+__global__ void __launch_bounds__(kCBlock) k_ie_dv(GridView g, DevState* st, int lv, const int32_t* snap) {
+  if (cls_skip(st, lv)) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL(L, t) ie_cell(g, L.cstart, t, snap, st);
+}
+
+// The level's leaves still unlabelled after that round: counted (as the
+// host version counts them) and listed for the fixpoint.
+__global__ void k_ie_list(GridView g, DevState* st, int lv, int32_t* list) {
+  if (cls_skip(st, lv)) return;
+  const LevelDesc L = st->lv[lv];
+  FOR_LEVEL(L, t) {
+    const int cid = L.cstart + t;
+    if (g.c_subcell[cid] == -1 && g.c_type[cid] == kUndefined) {
+      list[atomicAdd(&st->ie_n, 1)] = cid;
+      atomicAdd(&st->und_first, 1ULL);
+    }
+  }
+}
+
+// setIECell's remaining rounds to the fixpoint, by one workgroup over the
+// listed leaves: each round reads every listed leaf's neighbours (phase 1:
+// nothing is written, so the reads see the types at the round's start, the
+// snapshot of the host version's rounds) and then writes the new types
+// (phase 2); it ends after a round that changed nothing.
+constexpr int kFixBlock = 1024;
+__global__ void __launch_bounds__(kFixBlock) k_ie_fix(GridView g, const DevState* st, int lv, const int32_t* list,
+                                                      int32_t* nt) {
+  if (cls_skip(st, lv)) return;
+  const int n = st->ie_n;
+  __shared__ int changed;
+  for (;;) {
+    if (threadIdx.x == 0) changed = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kFixBlock) {
+      const int cid = list[i];
+      int ty = g.c_type[cid];
+      if (ty == kUndefined) {
+        const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+        for (int k = 0; k < bn; k++) {
+          const int pe_ = g.b_partner[bs + k];
+          if (pe_ == -1) continue;
+          const int pc = g.b_parent[pe_];
+          const int pt = g.c_type[pc];
+          if (g.c_rnum[pc] == 0 && pt != kUndefined) {
+            ty = pt;
+            changed = 1;
+            break;
+          }
+        }
+      }
+      nt[i] = ty;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kFixBlock) g.c_type[list[i]] = nt[i];
+    __syncthreads();
+    const int ch = changed;
+    __syncthreads();
+    if (!ch) break;
+  }
+}
+
+__global__ void k_count_undef_dv(GridView g, DevState* st) {
+  if (st->abort) return;
+  const int C = st->tot_c32;
+  for (int cid = blockIdx.x * blockDim.x + threadIdx.x; cid < C; cid += gridDim.x * blockDim.x)
+    if (g.c_subcell[cid] == -1 && g.c_type[cid] == kUndefined) atomicAdd(&st->undefined_leaves, 1ULL);
 }
 
 #define CCK(x)                       \
@@ -425,6 +580,43 @@ int run_kat_cp_prop(gahhg_ctx* c, const int32_t* d_refs, const int32_t* d_rs_rn,
   return hip_check(c, hipGetLastError(), "kat cp_prop kernel");
 }
 
+// classifyAHHG (:2018-2059) with every size and lane-group choice on the
+// device: levels [0, levels) from the deepest up (the ones below every
+// polygon's bottom return at once), then setTopECell and the count of
+// leaves left unlabelled.  Enqueued only; nothing waits.
+int classify_dev_enqueue(gahhg_ctx* c, int levels) {
+  const int np = (int)c->polys.size();
+  const GridView g = view(c);
+  const PolyDev* pd = c->d_pd;
+  DevState* st = c->d_st;
+  int32_t* snap = c->d_f0;
+  int32_t* list = c->d_f2;
+  int32_t* nt = c->d_f3;
+  const unsigned cg = (unsigned)std::min<int64_t>(std::max<int64_t>((c->g.ccap + kCBlock - 1) / kCBlock, 1), 2048);
+  const unsigned gg = (unsigned)std::min<int64_t>(gblocks(c->g.ccap, 64), 2048);  // lane-group launches
+  for (int lv = levels - 1; lv >= 0; lv--) {
+    hipLaunchKernelGGL(k_cls_level, dim3(1), dim3(256), 0, c->stream, pd, np, st, lv);
+    hipLaunchKernelGGL(k_ne_dv, dim3(gg), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_pv,
+                       (const int32_t*)c->d_pe, (const double*)c->d_edge, (const DevState*)st, lv);
+    hipLaunchKernelGGL(k_snap_dv, dim3(cg), dim3(kCBlock), 0, c->stream, (const int32_t*)c->g.c_type,
+                       (const DevState*)st, lv, snap);
+    hipLaunchKernelGGL(k_de_bottom_dv, dim3(gg), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_edge,
+                       (const DevState*)st, lv, (const int32_t*)snap);
+    hipLaunchKernelGGL(k_de_upper_dv, dim3(cg), dim3(kCBlock), 0, c->stream, g, pd, (const double*)c->d_edge,
+                       (const DevState*)st, lv);
+    hipLaunchKernelGGL(k_snap_dv, dim3(cg), dim3(kCBlock), 0, c->stream, (const int32_t*)c->g.c_type,
+                       (const DevState*)st, lv, snap);
+    hipLaunchKernelGGL(k_ie_dv, dim3(cg), dim3(kCBlock), 0, c->stream, g, st, lv, (const int32_t*)snap);
+    hipLaunchKernelGGL(k_ie_list, dim3(cg), dim3(kCBlock), 0, c->stream, g, st, lv, list);
+    hipLaunchKernelGGL(k_ie_fix, dim3(1), dim3(kFixBlock), 0, c->stream, g, (const DevState*)st, lv,
+                       (const int32_t*)list, nt);
+    CCK(hip_check(c, hipGetLastError(), "classify level"));
+  }
+  hipLaunchKernelGGL(k_top, dim3(np), dim3(64), 0, c->stream, g, pd, (const DevState*)st);
+  hipLaunchKernelGGL(k_count_undef_dv, dim3(cg), dim3(kCBlock), 0, c->stream, g, st);
+  return hip_check(c, hipGetLastError(), "classify top");
+}
+
 int classify_hierarchy(gahhg_ctx* c) {
   const int np = (int)c->polys.size();
   int maxL = 0;
@@ -461,7 +653,7 @@ int classify_hierarchy(gahhg_ctx* c) {
     CCK(ie_fixpoint(c, lv, snap, &und1));
     und_snap += und1;
   }
-  hipLaunchKernelGGL(k_top, dim3(np), dim3(64), 0, c->stream, g, pd);
+  hipLaunchKernelGGL(k_top, dim3(np), dim3(64), 0, c->stream, g, pd, (const DevState*)nullptr);
   hipLaunchKernelGGL(k_clear_undef, dim3(1), dim3(1), 0, c->stream, c->d_st);
   hipLaunchKernelGGL(k_count_undef, dim3(cblocks(c->tot_c)), dim3(kCBlock), 0, c->stream, g, 0, (int)c->tot_c,
                      c->d_st);

@@ -134,6 +134,23 @@ struct DevState {
   int32_t bad_poly;                  // a device polygon id was out of range
   int32_t bad_nodes;                 // node records of cells with more than 6 borders
   LevelDesc lv[kMaxLevels];
+  // device-driven build (every per-level decision taken on the device):
+  int32_t run;        // 1 while the level loop goes on
+  int32_t abort;      // the enqueued work stops short: levels left, or an arena array too small
+  int32_t overflow;   // an arena array is too small; need_* hold what it must hold
+  int32_t limit;      // a count exceeds int32 indexing
+  int32_t gnum;       // levels created
+  int32_t maxL;       // deepest bottom level
+  int32_t tot_c32;    // total cells (the dispatch's scans read it as their length)
+  int32_t R_ok;       // R once it fits the ref arrays, else 0 (the length the ref passes read)
+  uint32_t rec_levels;  // levels whose cells have walk records (bit l)
+  int64_t tot_v, tot_b, tot_c;
+  int64_t need_v, need_b, need_c, need_r;
+  double extra_next;  // expected walk steps of the next level beyond each edge's first cell
+  unsigned long long steps_before;  // steps_mark before this level's mark walk
+  unsigned long long und_first;     // leaves unlabelled after each level's first setIECell round
+  int32_t ie_n;       // leaves of the level being classified left unlabelled by that round
+  int32_t cls_G[kMaxLevels];        // lanes per cell of each level's getCPProp / parity groups
 };
 
 // Structure-of-arrays hierarchy in HBM.  Vertices and border normals are
@@ -456,6 +473,44 @@ int lonlat_to_xyz_gpu(gahhg_ctx* c, const double* lon, const double* lat, int64_
 // Vertex counts from which the TXT loaders convert on the device.
 constexpr int64_t kGpuConvertMin = 16384;
 void free_text_load(gahhg_ctx* c);
+// Device-driven build (build_hierarchy_dev): the level loop, the dispatch and
+// the classification take every per-level decision on the device, so the
+// whole build is one stream of launches with no host synchronisation in it
+// and can be recorded into a HIP graph and replayed.
+struct DevBuildState {
+  // level-0 grid template (80 vertices, 240 borders, 42 cells) on the device
+  double* g0_v = nullptr;
+  double* g0_n = nullptr;
+  int32_t* g0_b = nullptr;  // 6 per border
+  int32_t* g0_c = nullptr;  // 7 per cell
+  PolyDev* pd0 = nullptr;   // polygon records at the start of a build
+  int32_t pd0_n = 0;
+  LevelDesc* plv = nullptr;  // per polygon kMaxLevels level segments
+  int32_t* plvn = nullptr;   // per polygon: levels created
+  float* gamma = nullptr;    // per polygon density ratio
+  int32_t* pmarks = nullptr; // per polygon levels marked
+  int32_t* hist = nullptr;   // radix histograms of the ref sort
+  int64_t hist_cap = 0;
+  int32_t* csums = nullptr;  // chunk sums of the chunked scans
+  int32_t* sscr = nullptr;   // the build's own scan scratch (queries regrow the context's)
+  int64_t sscr_cap = 0;
+  char* hbuf = nullptr;      // pinned: the build's results read back in one copy each, one wait
+  int32_t hcap = 0;          // polygons hbuf holds
+  std::vector<PolyDev> pd0_host;  // what pd0 holds
+  int32_t pcap = 0;          // polygons the per-polygon arrays hold
+  // arena the last build ran with (cells, borders, vertices, refs) and what
+  // it used; the next build sizes from these
+  int64_t used_v = 0, used_b = 0, used_c = 0, used_r = 0;
+  int32_t used_levels = 0;
+  int64_t shape_n = -1;      // edges of the polygon set those figures belong to
+  int32_t shape_np = -1;
+  // the recorded build and what it was recorded against
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> key;
+  int32_t graph_levels = 0;
+};
+
 }  // namespace gahhg
 
 struct gahhg_ctx {
@@ -504,6 +559,7 @@ struct gahhg_ctx {
   int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr, *d_rscan = nullptr;
   gahhg::WalkRec* d_wrec = nullptr;  // walk record of every cell (build scratch)
   int64_t wrec_cap = 0;
+  gahhg::DevBuildState dvb;          // device-driven build: templates, arena sizes, graph
 
   gahhg::QueryLayout q;
   gahhg::QuerySort qsort;
@@ -517,6 +573,8 @@ struct gahhg_ctx {
   int64_t mlp_part_cap = 0;
   int64_t und_snap = 0, und_final = 0;
   bool built = false;
+  bool build_replayed = false;  // the last build ran from the recorded graph
+  int32_t build_attempts = 0;   // device-driven runs of the last build
   gahhg_stats st;
   double last_query_ms = 0.0;
   std::vector<void*> user_allocs;
@@ -531,6 +589,17 @@ namespace gahhg {
 // sum instead.
 int scan_i32(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound, const int32_t* d_n,
              int32_t* d_total, int inclusive);
+// Exclusive (inclusive != 0: inclusive) prefix sums of up to four int32
+// columns of the same length *d_n, read on the device: a fixed grid whatever
+// the length, two launches for all columns.  total[k] (non-null) receives
+// column k's sum; sums holds kScanColsSums int32.
+struct ScanCols {
+  const int32_t* in[4];
+  int32_t* out[4];
+  int32_t* total[4];
+};
+constexpr int kScanColsSums = 4 * 256;
+int scan_cols_dev(gahhg_ctx* c, const ScanCols& cols, int ncols, const int32_t* d_n, int inclusive, int32_t* sums);
 // Grows the scan scratch for n_bound elements now (a later scan of at most
 // that many then neither allocates nor synchronises).
 int scan_reserve(gahhg_ctx* c, int64_t n_bound);
@@ -544,6 +613,16 @@ int scan_i32_with(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound
 // ---- build / classify / query (gahhg_build.hip etc.) ----
 int build_hierarchy(gahhg_ctx* c);
 int classify_hierarchy(gahhg_ctx* c);
+// The device-driven build and classification (every per-level decision on
+// the device; replayed from a HIP graph when the polygon set, the arena and
+// the parameters are those it was recorded with).  *replayed tells whether
+// the graph ran.
+int build_classify_dev(gahhg_ctx* c, bool* replayed, int32_t* attempts);
+// Enqueues the device-driven classification of levels [0, levels) on the
+// context's stream (no synchronisation).
+int classify_dev_enqueue(gahhg_ctx* c, int levels);
+void drop_build_graph(gahhg_ctx* c);
+void free_dev_build(gahhg_ctx* c);
 int pack_query_layout(gahhg_ctx* c);
 int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t n, int32_t* d_labels);
 // GAHHG_E_ARG unless every d_poly[0..n) is a polygon of the batch (a

@@ -162,6 +162,89 @@ int scan_i32_with(gahhg_ctx* c, const int32_t* in, int32_t* out, int64_t n_bound
   return scan_rec(c, in, out, n_bound, d_n, d_total, inclusive, scratch);
 }
 
+// Chunked scan of up to four columns whose length lives in device memory
+// (*d_n, the same for every column).  The grid is fixed -- kChunkBlocks
+// blocks per column, block b owning a contiguous chunk of whole tiles --
+// so the launch does not depend on any bound of the length and costs the
+// same two launches for 10 elements as for 10 million; columns run side by
+// side (blockIdx.y).  Pass 1 reduces each chunk; pass 2 has every block sum
+// the chunk totals before its own (one per thread) and scan its chunk tile
+// by tile.
+constexpr int kChunkBlocks = kScanThreads;
+
+__device__ __forceinline__ int64_t chunk_len(int64_t n) {
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  return (tiles + kChunkBlocks - 1) / kChunkBlocks * kScanTile;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_chunk_reduce(ScanCols s, const int32_t* __restrict__ d_n,
+                                                               int32_t* __restrict__ sums) {
+  const int k = blockIdx.y;
+  const int64_t n = *d_n;
+  const int64_t ch = chunk_len(n);
+  const int64_t lo = (int64_t)blockIdx.x * ch, hi = min(n, lo + ch);
+  const int32_t* in = s.in[k];
+  int v = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) v += in[i];
+  int tot;
+  (void)block_exclusive(v, &tot);
+  if (threadIdx.x == 0) sums[k * kChunkBlocks + blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_chunk_scan(ScanCols s, const int32_t* __restrict__ d_n,
+                                                             const int32_t* __restrict__ sums, int inclusive) {
+  __shared__ int tile[kScanTile];
+  const int k = blockIdx.y;
+  const int t = threadIdx.x;
+  const int64_t n = *d_n;
+  const int64_t ch = chunk_len(n);
+  const int64_t lo = (int64_t)blockIdx.x * ch, hi = min(n, lo + ch);
+  int run;
+  (void)block_exclusive(t < (int)blockIdx.x ? sums[k * kChunkBlocks + t] : 0, &run);  // this chunk's offset
+  const int32_t* in = s.in[k];
+  int32_t* out = s.out[k];
+  for (int64_t base = lo; base < hi; base += kScanTile) {
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+      const int j = q * kScanThreads + t;
+      const int64_t i = base + j;
+      tile[j] = i < hi ? in[i] : 0;
+    }
+    __syncthreads();
+    int loc[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+      loc[q] = tile[t * kScanItems + q];
+      sum += loc[q];
+    }
+    int ttot;
+    int r = run + block_exclusive(sum, &ttot);
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+      tile[t * kScanItems + q] = inclusive ? r + loc[q] : r;
+      r += loc[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+      const int j = q * kScanThreads + t;
+      const int64_t i = base + j;
+      if (i < hi) out[i] = tile[j];
+    }
+    __syncthreads();
+    run += ttot;
+  }
+  if (s.total[k] && blockIdx.x == kChunkBlocks - 1 && t == 0) *s.total[k] = run;  // the last chunk ends the column
+}
+
+int scan_cols_dev(gahhg_ctx* c, const ScanCols& cols, int ncols, const int32_t* d_n, int inclusive, int32_t* sums) {
+  hipLaunchKernelGGL(k_chunk_reduce, dim3(kChunkBlocks, ncols), dim3(kScanThreads), 0, c->stream, cols, d_n, sums);
+  hipLaunchKernelGGL(k_chunk_scan, dim3(kChunkBlocks, ncols), dim3(kScanThreads), 0, c->stream, cols, d_n,
+                     (const int32_t*)sums, inclusive);
+  return hip_check(c, hipGetLastError(), "chunked scan");
+}
+
 int scan_reserve(gahhg_ctx* c, int64_t n_bound) {
   const int64_t need = scratch_need(std::max<int64_t>(n_bound, 1));
   if (need <= c->scap) return GAHHG_OK;

@@ -96,14 +96,29 @@ typedef struct {
                               points the table leaves open are counted on
                               the device only.  0 (default): launched
                               kernel by kernel.  Labels are unchanged      */
+  int32_t build_device;    /* 1 (default): the build takes every per-level
+                              decision on the device (density rule, next
+                              level's layout, dispatch sizes, classification
+                              fixpoint), one stream of launches with no host
+                              synchronisation in it; 0: the host decides
+                              between levels (also the path of the learned
+                              hierarchy depth).  Grids are the same          */
+  int32_t build_graph;     /* with build_device: 1 (default) records the
+                              build as a HIP graph and replays it on later
+                              builds of the same polygon set (same sizes,
+                              parameters and buffers); a replay that runs
+                              out of room or levels is redone eagerly and
+                              recorded again.  Grids are the same           */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by
  * GPISP::getStat (source/GPISPAHHG.cu:2088-2113), plus diagnostics. */
 typedef struct {
   double pre_ms;          /* create_ms + classify_ms                           */
-  double create_ms;       /* createAHHG: level loop + edge dispatch            */
-  double classify_ms;     /* classifyAHHG                                      */
+  double create_ms;       /* createAHHG: level loop + edge dispatch (with
+                             build_device also classifyAHHG: one stream)     */
+  double classify_ms;     /* classifyAHHG and the query layout (with
+                             build_device the query layout alone)            */
   double query_ms;        /* last query: kernel time only                      */
   double query_unit_s;    /* query seconds per point                           */
   int32_t depth;          /* levels built (gnum)                               */
@@ -132,6 +147,10 @@ typedef struct {
   double query_kernel_ms; /* last query: the query kernel alone                */
   double query_sort_ms;   /* last query: grouping of the points before it      */
   double layout_ms;       /* last build: query layout (part of classify_ms)    */
+  int32_t build_replayed; /* last build: 1 when it ran from the recorded graph */
+  int32_t build_attempts; /* last build: device-driven runs it took (more than
+                             one when the arena or the recorded levels were
+                             too small and it ran again; 0: host loop)       */
 } gahhg_stats;
 
 /* Fills defaults (GPISPParam::GPISPParam, source/GPISPParam.h:66-77). */
