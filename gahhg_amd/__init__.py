@@ -42,7 +42,7 @@ class Params(ctypes.Structure):
         "max_depth", "auto_depth", "block_create", "block_classify", "block_query",
         "device", "verbose", "query_layout", "query_bins", "query_group", "query_sub",
     "query_sub_budget", "query_sort", "query_sort_bits", "query_index", "query_xcd",
-    "query_table_budget", "query_graph", "build_device", "build_graph")]
+    "query_table_budget", "query_graph", "build_device", "build_graph", "query_persistent")]
 
 
 _STAT_FIELDS = [
@@ -325,7 +325,7 @@ class GPISP:
     def __init__(self, max_depth=15, auto_depth=True, device=0, verbose=False, block_query=256,
                  query_layout=0, query_bins=256, query_group=0, query_sub=8, query_sub_budget=4,
                  query_sort=1, query_sort_bits=14, query_index=1, query_xcd=32, query_table_budget=8,
-                 query_graph=0, build_device=1, build_graph=1):
+                 query_graph=0, build_device=1, build_graph=1, query_persistent=0):
         self.L = lib()
         prm = default_params(max_depth=max_depth, auto_depth=1 if auto_depth else 0, device=device,
                              verbose=1 if verbose else 0, block_query=block_query,
@@ -333,7 +333,8 @@ class GPISP:
                              query_sub=query_sub, query_sub_budget=query_sub_budget, query_sort=query_sort,
                              query_sort_bits=query_sort_bits, query_index=query_index, query_xcd=query_xcd,
                              query_table_budget=query_table_budget, query_graph=query_graph,
-                             build_device=build_device, build_graph=build_graph)
+                             build_device=build_device, build_graph=build_graph,
+                             query_persistent=query_persistent)
         h = ctypes.c_void_p()
         rc = self.L.gahhg_create(ctypes.byref(h), ctypes.byref(prm))
         if rc:

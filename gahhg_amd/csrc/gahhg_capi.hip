@@ -271,6 +271,7 @@ int gahhg_params_default(gahhg_params* p) {
   p->query_graph = 0;
   p->build_device = 1;
   p->build_graph = 1;
+  p->query_persistent = 0;
   return GAHHG_OK;
 }
 
@@ -292,7 +293,7 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
       c->prm.query_sort_bits < 1 || c->prm.query_sort_bits > 14 || c->prm.query_index < 0 ||
       c->prm.query_index > 1 || c->prm.query_table_budget < 0 || c->prm.query_graph < 0 ||
       c->prm.query_graph > 1 || c->prm.build_device < 0 || c->prm.build_device > 1 || c->prm.build_graph < 0 ||
-      c->prm.build_graph > 1) {
+      c->prm.build_graph > 1 || c->prm.query_persistent < 0 || c->prm.query_persistent > 1) {
     delete c;
     return GAHHG_E_ARG;
   }
@@ -325,6 +326,8 @@ void gahhg_destroy(gahhg_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_dev_build(c);
+  dfree(c->q_queue);
+  c->q_queue = nullptr;
   free_grid(c);
   free_scratch(c);
   dfree(c->qsort.buf);

@@ -565,6 +565,8 @@ struct gahhg_ctx {
   gahhg::QuerySort qsort;
   gahhg::QueryGraph qgraph;
   uint64_t qepoch = 1;              // bumped when a buffer a query pass reads or writes moves
+  unsigned long long* q_queue = nullptr;  // persistent query launch: per-XCD queue counters
+  int32_t q_resident = 0;                 // its grid: blocks resident at once
   int32_t* h_cnt = nullptr;         // pinned: points the start table leaves open
   hipEvent_t ev_cnt = nullptr;      // h_cnt written
   gahhg::LearnState learn;

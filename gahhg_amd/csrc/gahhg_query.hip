@@ -1028,30 +1028,20 @@ __device__ __forceinline__ void leaf_tests(int lane, int leaf, V3 p, int type, i
 // its top-level cell, its level, level of the table start cell (-1 = level-0
 // scan, -2 = resolved by the table), border planes evaluated, leaf edge
 // records scanned}; labels are unchanged.
+// The points of one wave: lane l takes position t (the wave's 64
+// consecutive positions), point order[t] (points grouped by polygon and
+// start bin, so that a wave's gathers share cache lines) or point t.
 template <bool TRACE>
-__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
-                                                   const int32_t* __restrict__ table,
-                                                   const int32_t* __restrict__ sub, int B,
-                                                   const double* __restrict__ qc, const int32_t* __restrict__ hdr,
-                                                   const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
-                                                   const double* __restrict__ sv, const double* __restrict__ sn,
-                                                   const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
-                                                   const int32_t* __restrict__ qpoly,
-                                                   const int32_t* __restrict__ order, int64_t n,
-                                                   const int32_t* __restrict__ d_n,
-                                                   int32_t* __restrict__ labels, LevelTab lt,
-                                                   int32_t* __restrict__ trace, int G_override, int xcd,
-                                                   const int32_t* __restrict__ nodes) {
-  // thread t handles point order[t] (points grouped by polygon and start
-  // bin, so that a wave's gathers share cache lines), or point t; with xcd,
-  // each XCD takes a contiguous run of t.  With d_n the count is *d_n (at
-  // most n, the launch bound): blocks past it leave at once.
-  const int64_t blk = xcd_block(xcd);
-  if (d_n) {
-    n = *d_n;
-    if (blk * blockDim.x >= n) return;
-  }
-  const int64_t t = blk * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void query_lanes(int64_t t, int64_t n, const GridView& g, const PolyDev* __restrict__ pd,
+                                            const int32_t* __restrict__ table, const int32_t* __restrict__ sub,
+                                            int B, const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                            const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
+                                            const double* __restrict__ sv, const double* __restrict__ sn,
+                                            const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
+                                            const int32_t* __restrict__ qpoly, const int32_t* __restrict__ order,
+                                            int32_t* __restrict__ labels, const LevelTab& lt,
+                                            int32_t* __restrict__ trace, int G_override,
+                                            const int32_t* __restrict__ nodes) {
   const int64_t q = (order && t < n) ? (int64_t)order[t] : t;
   const int lane = threadIdx.x & 63;
   V3 p = v3(0.0, 0.0, 0.0);
@@ -1143,6 +1133,75 @@ __global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __
     case 16: leaf_tests<16>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
     case 32: leaf_tests<32>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
     default: leaf_tests<64>(lane, leaf, p, type, q, qc, hdr, ib, idx, sv, sn, sf, labels); break;
+  }
+}
+
+// Queue counters of the persistent launch: one per XCD, 256 bytes apart.
+constexpr int kQueueStride = 32;  // uint64 entries
+constexpr int kQueueWords = 8 * kQueueStride;
+
+__global__ void k_zero_queue(unsigned long long* queue) {
+  if (threadIdx.x < kQueueWords) queue[threadIdx.x] = 0ull;
+}
+
+// Thread t of the launch takes position t; with xcd > 0 each XCD takes
+// runs of xcd consecutive blocks (xcd_block).  With d_n the count is *d_n
+// (at most n, the launch bound): blocks past it leave at once.
+template <bool TRACE>
+__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
+                                                   const int32_t* __restrict__ table,
+                                                   const int32_t* __restrict__ sub, int B,
+                                                   const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+                                                   const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
+                                                   const double* __restrict__ sv, const double* __restrict__ sn,
+                                                   const uint8_t* __restrict__ sf, const double* __restrict__ xyz,
+                                                   const int32_t* __restrict__ qpoly,
+                                                   const int32_t* __restrict__ order, int64_t n,
+                                                   const int32_t* __restrict__ d_n,
+                                                   int32_t* __restrict__ labels, LevelTab lt,
+                                                   int32_t* __restrict__ trace, int G_override, int xcd,
+                                                   const int32_t* __restrict__ nodes) {
+  if (d_n) n = *d_n;
+  const int64_t blk = xcd_block(xcd);
+  if (blk * blockDim.x >= n) return;
+  query_lanes<TRACE>(blk * blockDim.x + threadIdx.x, n, g, pd, table, sub, B, qc, hdr, ib, idx, sv, sn, sf, xyz,
+                     qpoly, order, labels, lt, trace, G_override, nodes);
+}
+
+// The persistent launch (queue counters cleared before it): a grid of
+// resident blocks whose waves take tiles of 64 positions from queues.  The
+// positions are cut into the runs of the launch above (xcd blocks' worth of
+// tiles, run r belonging to XCD r mod 8); queue x hands out XCD x's runs
+// tile by tile in order; a wave takes from its own XCD's queue (blockIdx
+// mod 8: workgroups reach the XCDs round-robin) and, once that is empty,
+// from the others in turn.  Every tile is taken exactly once, so labels do
+// not depend on the order.
+__global__ void __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+k_query_persist(GridView g, const PolyDev* __restrict__ pd, const int32_t* __restrict__ table,
+                const int32_t* __restrict__ sub, int B, const double* __restrict__ qc,
+                const int32_t* __restrict__ hdr, const int32_t* __restrict__ ib, const int32_t* __restrict__ idx,
+                const double* __restrict__ sv, const double* __restrict__ sn, const uint8_t* __restrict__ sf,
+                const double* __restrict__ xyz, const int32_t* __restrict__ qpoly, const int32_t* __restrict__ order,
+                int64_t n, const int32_t* __restrict__ d_n, int32_t* __restrict__ labels, LevelTab lt, int G_override,
+                int xcd, const int32_t* __restrict__ nodes, unsigned long long* __restrict__ queue) {
+  if (d_n) n = *d_n;
+  const int lane = threadIdx.x & 63;
+  const int64_t tiles = (n + 63) / 64;
+  const int64_t K = xcd > 0 ? (int64_t)xcd * (blockDim.x / 64) : 1;  // tiles per run
+  const int home = blockIdx.x & 7;
+  for (int v = 0; v < 8; v++) {
+    const int x = (home + v) & 7;
+    for (;;) {
+      unsigned long long k = 0;
+      if (lane == 0) k = atomicAdd(queue + (int64_t)x * kQueueStride, 1ull);
+      k = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(k >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)k);
+      const int64_t run = (int64_t)k / K, j = (int64_t)k - run * K;
+      const int64_t tile = (run * 8 + x) * K + j;
+      if (tile >= tiles) break;  // tiles of queue x increase with k: x is empty
+      query_lanes<false>(tile * 64 + lane, n, g, pd, table, sub, B, qc, hdr, ib, idx, sv, sn, sf, xyz, qpoly,
+                         order, labels, lt, nullptr, G_override, nodes);
+    }
   }
 }
 
@@ -1729,23 +1788,61 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
 
 // Scratch a grouped pass over n points needs, allocated now (a captured
 // pass must neither allocate nor synchronise).
+static int ensure_queue(gahhg_ctx* c);
+
 static int reserve_query(gahhg_ctx* c, int64_t n) {
   const int64_t nb = (n + 255) / 256;
+  if (c->prm.query_persistent) {
+    const int qrc = ensure_queue(c);
+    if (qrc) return qrc;
+  }
   const int rc = scan_reserve(c, std::max<int64_t>(nb, radix_hist_entries(n)));
   return rc ? rc : sort_buffers(c, n);
 }
 
+// The persistent launch's queue counters and its grid (the blocks of
+// k_query<false> resident at once on the device), set up once per context.
+static int ensure_queue(gahhg_ctx* c) {
+  if (c->q_queue) return GAHHG_OK;
+  int per_cu = 0, cus = 0;
+  int rc = hip_check(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_query_persist, c->prm.block_query, 0),
+                     "query occupancy");
+  if (!rc) rc = hip_check(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device), "CUs");
+  if (rc) return rc;
+  if (dalloc(c, (void**)&c->q_queue, kQueueWords * 8)) return GAHHG_E_NOMEM;
+  c->q_resident = std::max(1, per_cu) * std::max(1, cus);
+  c->qepoch++;
+  return GAHHG_OK;
+}
+
+// allow_persistent: the persistent launch may be used (its queue exists);
+// the allocation-free entry point passes false and keeps the static one.
 template <bool TRACE>
 static void launch_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, const int32_t* d_order,
-                         int64_t n, const int32_t* d_n, int32_t* d_labels, int32_t* d_trace) {
-  const unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
+                         int64_t n, const int32_t* d_n, int32_t* d_labels, int32_t* d_trace,
+                         bool allow_persistent = false) {
+  unsigned blocks = (unsigned)((n + c->prm.block_query - 1) / c->prm.block_query);
   const QueryLayout& L = c->q;
-  hipLaunchKernelGGL((k_query<TRACE>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
-                     (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
-                     L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
-                     (const int32_t*)L.idx, (const double*)L.sv,
-                     (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_n, d_labels,
-                     level_tab(c), d_trace, c->prm.query_group, c->prm.query_xcd, (const int32_t*)L.nodes);
+  unsigned long long* queue = nullptr;
+  if (!TRACE && allow_persistent && c->prm.query_persistent && c->q_queue) {
+    queue = c->q_queue;
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(kQueueWords), 0, c->stream, queue);
+    blocks = std::min<unsigned>(blocks, (unsigned)c->q_resident);
+  }
+  if (queue)
+    hipLaunchKernelGGL(k_query_persist, dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                       (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
+                       L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
+                       (const int32_t*)L.idx, (const double*)L.sv, (const double*)L.sn, (const uint8_t*)L.sf, d_xyz,
+                       d_poly, d_order, n, d_n, d_labels, level_tab(c), c->prm.query_group, c->prm.query_xcd,
+                       (const int32_t*)L.nodes, queue);
+  else
+    hipLaunchKernelGGL((k_query<TRACE>), dim3(blocks), dim3(c->prm.block_query), 0, c->stream, view(c),
+                       (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub,
+                       L.table ? L.bins : 0, (const double*)L.qc, (const int32_t*)L.hdr, (const int32_t*)L.ib,
+                       (const int32_t*)L.idx, (const double*)L.sv,
+                       (const double*)L.sn, (const uint8_t*)L.sf, d_xyz, d_poly, d_order, n, d_n, d_labels,
+                       level_tab(c), d_trace, c->prm.query_group, c->prm.query_xcd, (const int32_t*)L.nodes);
 }
 
 // The work of one accelerated query pass, enqueued with no host wait when
@@ -1760,10 +1857,11 @@ static int enqueue_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_pol
     if (rc) return rc;
   }
   if (host_count) (void)hipEventRecord(c->ev[2], c->stream);
+  const bool pers = ext == nullptr;  // the allocation-free path keeps the static launch
   if (m < 0)
-    launch_query<false>(c, d_xyz, d_poly, order, n, d_m, d_labels, nullptr);
+    launch_query<false>(c, d_xyz, d_poly, order, n, d_m, d_labels, nullptr, pers);
   else if (m > 0)
-    launch_query<false>(c, d_xyz, d_poly, order, m, nullptr, d_labels, nullptr);
+    launch_query<false>(c, d_xyz, d_poly, order, m, nullptr, d_labels, nullptr, pers);
   if (host_count) (void)hipEventRecord(c->ev[3], c->stream);
   return hip_check(c, hipGetLastError(), "query kernel");
 }
@@ -1829,6 +1927,10 @@ int run_query(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64_t 
   } else if (c->prm.query_graph) {
     return graph_query(c, d_xyz, d_poly, n, d_labels);
   } else {
+    if (c->prm.query_persistent) {
+      const int qrc = ensure_queue(c);
+      if (qrc) return qrc;
+    }
     return enqueue_query(c, d_xyz, d_poly, n, d_labels, true);
   }
   return hip_check(c, hipGetLastError(), "query kernel");

@@ -109,6 +109,12 @@ typedef struct {
                               parameters and buffers); a replay that runs
                               out of room or levels is redone eagerly and
                               recorded again.  Grids are the same           */
+  int32_t query_persistent;/* accelerated query: 1 runs the walk as a
+                              persistent, work-stealing launch (a resident
+                              grid whose waves take 64-point tiles from
+                              per-XCD queues, then from the other XCDs'),
+                              0 (default) one thread per point.  Labels
+                              are unchanged                                 */
 } gahhg_params;
 
 /* Statistics: GPISPStat fields (source/GPISPParam.h:80-100) as filled by

@@ -331,6 +331,40 @@ def test_query_paths_agree(layout, bins, group, sub, sort):
     g.close()
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_persistent_launch_agrees(graph):
+    """The persistent work-stealing walk (query_persistent=1: a resident
+    grid whose waves take 64-point tiles from per-XCD queues and steal from
+    the others) labels every point as the oracle does: a single polygon with
+    grouped points (more tiles than the grid has waves, a count known only
+    on the device) and a batch, recorded as a graph or launched directly."""
+    say("persistent launch, graph", graph)
+    w = datasets.fractal_100k(n_queries=1_200_000, n_outer=30_000)
+    o = ol.Oracle().set_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+    xyz = gahhg_amd.query_lonlat_to_xyz(w["qlon"], w["qlat"])
+    want = o.query_xyz(xyz)
+    with gahhg_amd.GPISP(device=0, query_persistent=1, query_graph=graph, query_sort=2) as g:
+        g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"]).build()
+        for _ in range(2):
+            assert np.count_nonzero(g.query_xyz(xyz) != want) == 0
+        # fewer points than one tile, and an empty query
+        assert np.array_equal(g.query_xyz(xyz[:37]), want[:37])
+        assert g.query_xyz(np.zeros((0, 3))).size == 0
+    ps = datasets.batch_polygons(n_poly=6, seed=33, n_min=50, n_max=20_000)
+    items = [(p["lon"], p["lat"], p["ring_off"]) for p in ps]
+    rng = np.random.default_rng(8)
+    lon, lat = datasets.uniform_queries(rng, 400_000)
+    bx = gahhg_amd.query_lonlat_to_xyz(lon, lat)
+    poly = rng.integers(0, len(items), len(lon)).astype(np.int32)
+    with gahhg_amd.GPISP(device=0, query_persistent=1, query_graph=graph, query_sort=2) as g:
+        g.set_polygons_lonlat(items).build()
+        got = g.query_batch_xyz(bx, poly)
+    for k, it in enumerate(items):
+        sel = poly == k
+        ok = ol.Oracle().set_lonlat(*it).build()
+        assert np.count_nonzero(got[sel] != ok.query_xyz(np.ascontiguousarray(bx[sel]))) == 0, k
+
+
 def test_cli_matches_oracle(tmp_path):
     say("cli")
     import subprocess
