@@ -409,6 +409,9 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __re
     if (MODE == kMark ? st->run == 0 : st->abort != 0) return;  // uniform over the launch
     rec_levels = st->rec_levels;
   }
+  // a mark walk walks one level: the launch whose record mode is not that
+  // level's has nothing to do
+  if (MODE == kMark && ((((rec_levels >> level) & 1u) != 0) != REC)) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   WalkDiag d = {0, 0, 0, 0, 0};
   bool mine = e < Ntot;
@@ -636,79 +639,89 @@ __global__ void k_create_center_dv(GridView g, const PolyDev* __restrict__ pd, c
 
 // kernel_DivGrid_createBorderSC, :512-607: the border children of one
 // refined cell, in border order.
-__device__ __forceinline__ void border_children_cell(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t,
-                                                     const int32_t* oc, const int32_t* od) {
+// Lanes per refined cell of the border-children kernels: lane i creates the
+// child of border i (its slot follows from the kinds of borders 0..i-1), so
+// a cell's children are built side by side.
+constexpr int kBcLanes = 8;
+
+__device__ __forceinline__ void border_child(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t, int i,
+                                             const int32_t* oc, const int32_t* od) {
   const int P = lo + t;
   if (g.c_rnum[P] == 0) return;
+  const int bnP = g.c_bnum[P];
+  if (i >= bnP) return;
+  const int kind = child_kind(g, P, bnP, i);
+  if (!kind) return;
   const int p = g.c_poly[P];
   const PolyDev& D = pd[p];
   if (!D.refining) return;
   const int f = D.cs - lo;
   int co = D.ncb + D.U + (oc[t] - oc[f]);
   int eo = D.nbb0 + D.nv + (od[t] - od[f]);
-  const int bnP = g.c_bnum[P];
+  for (int j = 0; j < i; j++) {
+    const int k = child_kind(g, P, bnP, j);
+    co += k ? 1 : 0;
+    eo += k;
+  }
   const int pbs = g.c_bstart[P];
   const int cbs = g.c_bstart[g.c_subcell[P]];  // centre child's borders
-  for (int i = 0; i < bnP; i++) {
-    const int kind = child_kind(g, P, bnP, i);
-    if (!kind) continue;
-    const int ue = pbs + i;
-    const int b = cbs + i;
-    int vv[6];
-    if (kind == 6) {
-      const int pt = g.b_partner[ue];
-      const int psub = g.c_subcell[g.b_parent[pt]];
-      const int pse = g.c_bstart[psub] + g.b_id[pt];
-      vv[0] = g.b_v0[ue];
-      vv[1] = g.b_v1[pse];
-      vv[2] = g.b_v0[pse];
-      vv[3] = g.b_v1[ue];
-      vv[4] = g.b_v1[b];
-      vv[5] = g.b_v0[b];
-    } else {
-      vv[0] = g.b_v0[ue];
-      vv[1] = g.b_v1[ue];
-      vv[2] = g.b_v1[b];
-      vv[3] = g.b_v0[b];
-    }
-    for (int k = 0; k < kind; k++) {
-      const int nb = eo + k;
-      const int a = vv[k], z = vv[(k + 1) % kind];
-      g.b_id[nb] = k;
-      g.b_v0[nb] = a;
-      g.b_v1[nb] = z;
-      g.b_parent[nb] = co;
-      g.b_partner[nb] = -1;
-      g.b_subcell[nb] = -1;
-      store3(g.b_n + 3 * (int64_t)nb, arc_normal(vtx(g, a), vtx(g, z)));
-    }
-    g.c_bnum[co] = kind;
-    g.c_bstart[co] = eo;
-    g.c_rstart[co] = -1;
-    g.c_rnum[co] = -1;
-    g.c_subcell[co] = -1;
-    g.c_upcell[co] = P;
-    g.c_type[co] = kUndefined;
-    g.c_poly[co] = p;
-    g.c_root[co] = g.c_root[P];
-    g.b_subcell[ue] = co;
-    if (kind == 6) g.b_subcell[g.b_partner[ue]] = co;
-    co += 1;
-    eo += kind;
+  const int ue = pbs + i;
+  const int b = cbs + i;
+  int vv[6];
+  if (kind == 6) {
+    const int pt = g.b_partner[ue];
+    const int psub = g.c_subcell[g.b_parent[pt]];
+    const int pse = g.c_bstart[psub] + g.b_id[pt];
+    vv[0] = g.b_v0[ue];
+    vv[1] = g.b_v1[pse];
+    vv[2] = g.b_v0[pse];
+    vv[3] = g.b_v1[ue];
+    vv[4] = g.b_v1[b];
+    vv[5] = g.b_v0[b];
+  } else {
+    vv[0] = g.b_v0[ue];
+    vv[1] = g.b_v1[ue];
+    vv[2] = g.b_v1[b];
+    vv[3] = g.b_v0[b];
   }
+  for (int k = 0; k < kind; k++) {
+    const int nb = eo + k;
+    const int a = vv[k], z = vv[(k + 1) % kind];
+    g.b_id[nb] = k;
+    g.b_v0[nb] = a;
+    g.b_v1[nb] = z;
+    g.b_parent[nb] = co;
+    g.b_partner[nb] = -1;
+    g.b_subcell[nb] = -1;
+    store3(g.b_n + 3 * (int64_t)nb, arc_normal(vtx(g, a), vtx(g, z)));
+  }
+  g.c_bnum[co] = kind;
+  g.c_bstart[co] = eo;
+  g.c_rstart[co] = -1;
+  g.c_rnum[co] = -1;
+  g.c_subcell[co] = -1;
+  g.c_upcell[co] = P;
+  g.c_type[co] = kUndefined;
+  g.c_poly[co] = p;
+  g.c_root[co] = g.c_root[P];
+  g.b_subcell[ue] = co;
+  if (kind == 6) g.b_subcell[g.b_partner[ue]] = co;
 }
 
 __global__ void k_border_children(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oc,
                                   const int32_t* od) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) border_children_cell(g, pd, lo, t, oc, od);
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)(u / kBcLanes);
+  if (t < n) border_child(g, pd, lo, t, (int)(u % kBcLanes), oc, od);
 }
 
 __global__ void k_border_children_dv(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st,
                                      int lv, const int32_t* oc, const int32_t* od) {
   if (!st->run) return;
   const LevelDesc L = st->lv[lv];
-  FOR_LEVEL_CELLS(L, t) border_children_cell(g, pd, L.cstart, t, oc, od);
+  const int64_t n = (int64_t)L.cnum * kBcLanes;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x)
+    border_child(g, pd, L.cstart, (int)(u / kBcLanes), (int)(u % kBcLanes), oc, od);
 }
 
 // device_setBorderPartner, :42-58: link `edge` with the border of `cell`
@@ -1203,7 +1216,13 @@ __global__ void k_dv_ref_edges(const int32_t* rcell, const int32_t* incl, const 
 __global__ void k_dv_count_cells(const int32_t* rcell, const DevState* __restrict__ st, int32_t* ccnt) {
   if (st->abort) return;
   const int R = st->R_ok;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) atomicAdd(&ccnt[rcell[i]], 1);
+  // consecutive refs often share a cell: one atomic per distinct cell of a
+  // wave (wave_add); the loop bound is rounded up so that whole waves take
+  // part (lanes past R add nothing)
+  const int stride = gridDim.x * blockDim.x;
+  const int end = (R + 63) & ~63;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += stride)
+    wave_add(ccnt, i < R ? rcell[i] : -1, 1);
 }
 
 // kernel_EC2CS_setCEInfo (:1030-1043) for bottom cells: the refs are in
@@ -1749,7 +1768,8 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL(k_create_center, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const PolyDev*)c->d_pd, GL.cstart, GL.cnum, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
     LAUNCH_CK("create center");
-    hipLaunchKernelGGL(k_border_children, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
+    hipLaunchKernelGGL(k_border_children, dim3(blocks_for((int64_t)GL.cnum * kBcLanes, kBlock)), dim3(kBlock), 0,
+                       c->stream, g,
                        (const PolyDev*)c->d_pd, GL.cstart, GL.cnum, (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
     LAUNCH_CK("border children");
     const LevelDesc NL{(int32_t)c->tot_v, (int32_t)nv_all, (int32_t)c->tot_b, (int32_t)nb_all, (int32_t)c->tot_c,
@@ -2095,7 +2115,8 @@ int dv_enqueue_level(gahhg_ctx* c, const DvArena& a, int lv) {
                      c->prm.auto_depth != 0 ? 1 : 0, d.gamma, a.v, a.b, a.c);
   hipLaunchKernelGGL(k_create_center_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, lv, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
-  hipLaunchKernelGGL(k_border_children_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+  hipLaunchKernelGGL(k_border_children_dv, dim3(dv_grid(a.c * kBcLanes)), dim3(kBlock), 0, c->stream, g,
+                     (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, lv, (const int32_t*)c->d_o2, (const int32_t*)c->d_o3);
   hipLaunchKernelGGL(k_sc_partner_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, lv);
