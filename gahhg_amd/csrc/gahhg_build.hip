@@ -579,71 +579,84 @@ __global__ void k_plan_totals(PolyDev* pd, int npoly, int lo, const int32_t* fa,
 }
 
 // kernel_DivGrid_createCenter, :406-470
-__device__ __forceinline__ void create_center_cell(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t,
-                                                   const int32_t* oa, const int32_t* ob) {
+// Lanes per refined cell of the centre-child and border-children kernels
+// (a cell has at most 6 borders).
+constexpr int kBcLanes = 8;
+
+// Lane i of a refined cell's group (kBcLanes lanes per cell) creates the
+// centre child's corner i and border i; it computes the next corner itself
+// (the same operations, so the same bits as the stored vertex) for the
+// border's normal.  Lane 0 writes the child's cell record.
+__device__ __forceinline__ void create_center_lane(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t,
+                                                   int i, const int32_t* oa, const int32_t* ob) {
   const int pid = lo + t;
   if (g.c_rnum[pid] == 0) return;
+  const int bn = g.c_bnum[pid];
+  if (i >= bn) return;
   const int p = g.c_poly[pid];
   const PolyDev& P = pd[p];
   if (!P.refining) return;
   const int f = P.cs - lo;
-  const int bs = g.c_bstart[pid], bn = g.c_bnum[pid];
+  const int bs = g.c_bstart[pid];
   V3 center;
   if (bn == 4) {
     center = mul(add(vtx(g, g.b_v0[bs]), vtx(g, g.b_v1[bs])), 0.5);
   } else {
     center = v3(0.0, 0.0, 0.0);
-    for (int i = 0; i < bn; i++) center = add(center, vtx(g, g.b_v0[bs + i]));
+    for (int k = 0; k < bn; k++) center = add(center, vtx(g, g.b_v0[bs + k]));
     center = divk(center, bn);
   }
   center = unit(center);
   const int vo = P.nvb + (oa[t] - oa[f]);
   const int bo = P.nbb0 + (oa[t] - oa[f]);
   const int child = P.ncb + (ob[t] - ob[f]);
-  double* vw = const_cast<double*>(g.v);
-  for (int i = 0; i < bn; i++) store3(vw + 3 * (int64_t)(vo + i), unit(mul(add(vtx(g, g.b_v0[bs + i]), center), 0.5)));
-  for (int i = 0; i < bn; i++) {
-    const int b = bo + i;
-    g.b_id[b] = i;
-    g.b_parent[b] = child;
-    g.b_partner[b] = -1;
-    g.b_subcell[b] = -1;
-    g.b_v0[b] = vo + i;
-    g.b_v1[b] = vo + (i + 1) % bn;
-    store3(g.b_n + 3 * (int64_t)b, arc_normal(vtx(g, vo + i), vtx(g, vo + (i + 1) % bn)));
+  const int i1 = (i + 1) % bn;
+  const V3 a = unit(mul(add(vtx(g, g.b_v0[bs + i]), center), 0.5));
+  const V3 z = unit(mul(add(vtx(g, g.b_v0[bs + i1]), center), 0.5));
+  store3(const_cast<double*>(g.v) + 3 * (int64_t)(vo + i), a);
+  const int b = bo + i;
+  g.b_id[b] = i;
+  g.b_parent[b] = child;
+  g.b_partner[b] = -1;
+  g.b_subcell[b] = -1;
+  g.b_v0[b] = vo + i;
+  g.b_v1[b] = vo + i1;
+  store3(g.b_n + 3 * (int64_t)b, arc_normal(a, z));
+  if (i == 0) {
+    g.c_bstart[child] = bo;
+    g.c_bnum[child] = bn;
+    g.c_rstart[child] = -1;
+    g.c_rnum[child] = -1;
+    g.c_upcell[child] = pid;
+    g.c_subcell[child] = -1;
+    g.c_type[child] = kUndefined;
+    g.c_poly[child] = p;
+    g.c_root[child] = g.c_root[pid];
+    g.c_subcell[pid] = child;
   }
-  g.c_bstart[child] = bo;
-  g.c_bnum[child] = bn;
-  g.c_rstart[child] = -1;
-  g.c_rnum[child] = -1;
-  g.c_upcell[child] = pid;
-  g.c_subcell[child] = -1;
-  g.c_type[child] = kUndefined;
-  g.c_poly[child] = p;
-  g.c_root[child] = g.c_root[pid];
-  g.c_subcell[pid] = child;
 }
 
 __global__ void k_create_center(GridView g, const PolyDev* __restrict__ pd, int lo, int n, const int32_t* oa,
                                 const int32_t* ob) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) create_center_cell(g, pd, lo, t, oa, ob);
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)(u / kBcLanes);
+  if (t < n) create_center_lane(g, pd, lo, t, (int)(u % kBcLanes), oa, ob);
 }
 
 __global__ void k_create_center_dv(GridView g, const PolyDev* __restrict__ pd, const DevState* __restrict__ st, int lv,
                                    const int32_t* oa, const int32_t* ob) {
   if (!st->run) return;
   const LevelDesc L = st->lv[lv];
-  FOR_LEVEL_CELLS(L, t) create_center_cell(g, pd, L.cstart, t, oa, ob);
+  const int64_t n = (int64_t)L.cnum * kBcLanes;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x)
+    create_center_lane(g, pd, L.cstart, (int)(u / kBcLanes), (int)(u % kBcLanes), oa, ob);
 }
 
 // kernel_DivGrid_createBorderSC, :512-607: the border children of one
 // refined cell, in border order.
-// Lanes per refined cell of the border-children kernels: lane i creates the
-// child of border i (its slot follows from the kinds of borders 0..i-1), so
-// a cell's children are built side by side.
-constexpr int kBcLanes = 8;
-
+// Border children: lane i of a refined cell's group creates the child of
+// border i (its slot follows from the kinds of borders 0..i-1), so a cell's
+// children are built side by side.
 __device__ __forceinline__ void border_child(const GridView& g, const PolyDev* __restrict__ pd, int lo, int t, int i,
                                              const int32_t* oc, const int32_t* od) {
   const int P = lo + t;
@@ -1765,7 +1778,8 @@ int build_hierarchy(gahhg_ctx* c) {
     CK(ensure_c(c, c->tot_c, c->tot_c + nc_all));
     CK(pd_upload(c));
     g = view(c);
-    hipLaunchKernelGGL(k_create_center, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g,
+    hipLaunchKernelGGL(k_create_center, dim3(blocks_for((int64_t)GL.cnum * kBcLanes, kBlock)), dim3(kBlock), 0,
+                       c->stream, g,
                        (const PolyDev*)c->d_pd, GL.cstart, GL.cnum, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
     LAUNCH_CK("create center");
     hipLaunchKernelGGL(k_border_children, dim3(blocks_for((int64_t)GL.cnum * kBcLanes, kBlock)), dim3(kBlock), 0,
@@ -2113,7 +2127,8 @@ int dv_enqueue_level(gahhg_ctx* c, const DvArena& a, int lv) {
                      (const int32_t*)c->d_o2, (const int32_t*)c->d_o3, (const DevState*)c->d_st, lv);
   hipLaunchKernelGGL(k_lv_step, dim3(1), dim3(kCtlBlock), 0, c->stream, c->d_pd, np, c->d_st, lv,
                      c->prm.auto_depth != 0 ? 1 : 0, d.gamma, a.v, a.b, a.c);
-  hipLaunchKernelGGL(k_create_center_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
+  hipLaunchKernelGGL(k_create_center_dv, dim3(dv_grid(a.c * kBcLanes)), dim3(kBlock), 0, c->stream, g,
+                     (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, lv, (const int32_t*)c->d_o0, (const int32_t*)c->d_o1);
   hipLaunchKernelGGL(k_border_children_dv, dim3(dv_grid(a.c * kBcLanes)), dim3(kBlock), 0, c->stream, g,
                      (const PolyDev*)c->d_pd,

@@ -268,19 +268,21 @@ __device__ __forceinline__ void de_bottom_cell(const GridView& g, const PolyDev*
   if (snap[t] != kUndefined) return;
   const int cid = cstart + t;
   if (cell_phase(g, pd, cid, lv) != 0) return;
+  // the reference loop lets every labelled neighbour decide in border
+  // order, the last one's label standing: the last labelled neighbour
+  // alone gives it (found from the end; one parity count, not one per
+  // labelled neighbour)
   const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
-  int lab = kUndefined;
-  bool set = false;
-  for (int i = 0; i < bn; i++) {
+  for (int i = bn - 1; i >= 0; i--) {
     const int pe_ = g.b_partner[bs + i];
     if (pe_ == -1) continue;
     const int nb = g.b_parent[pe_];
     const int nt = snap[nb - cstart];
     if (nt == kUndefined) continue;
-    lab = parity_label_group(g, edge, nb, pe_, nt, gr);
-    set = true;
+    const int lab = parity_label_group(g, edge, nb, pe_, nt, gr);
+    if (gr.gl == 0) g.c_type[cid] = lab;
+    return;
   }
-  if (set && gr.gl == 0) g.c_type[cid] = lab;
 }
 
 __global__ void __launch_bounds__(kCBlock) k_de_bottom(GridView g, const PolyDev* __restrict__ pd,
