@@ -169,7 +169,10 @@ def main():
     g = gahhg_amd.GPISP(device=0, query_sub=16, query_graph=graph)
     g.set_polygon_lonlat(w["lon"], w["lat"], w["ring_off"])
     if model is not None:
-        learned_tables.apply(g, model, xyz, None, Q)
+        # the hierarchy answers every pass of the run: the model weighs the
+        # table's build against that query volume
+        volume = Q * (args.steps + args.warmup + 1)
+        learned_tables.apply(g, model, xyz, None, volume)
     cold, warm = builds(g, args.builds)
     st = g.stats()
     tab = g.table_layout()[0].tolist()
@@ -320,7 +323,8 @@ def main():
                    "parallelism": "1 GPU"},
         "build_ms": round(warm[0], 3), "build_ms_cold": round(cold[0], 3), "create_ms": round(warm[1], 3),
         "classify_ms": round(warm[2], 3), "layout_ms": round(warm[3], 3), "depth": int(st["depth"]),
-        "start_table": {"bins": tab[0], "sub": tab[1], "chosen_by": "learned model" if model else "library default"},
+        "start_table": {"bins": tab[0], "sub": tab[1], "chosen_by": "learned model" if model else "library default",
+                        "query_volume": int(Q * (args.steps + args.warmup + 1)) if model else None},
         "grid_bytes": Sg, "parity": parity,
         "query_gpu_ms": round(pass_ms, 4),
         "plain_path": plain,
