@@ -1226,16 +1226,25 @@ __global__ void k_dv_ref_edges(const int32_t* rcell, const int32_t* incl, const 
   }
 }
 
-__global__ void k_dv_count_cells(const int32_t* rcell, const DevState* __restrict__ st, int32_t* ccnt) {
+// Runs of equal cells in the sorted refs: the run of cell k spans
+// [start[k], end[k]) (both left 0 for a cell without refs).
+__global__ void k_dv_cell_runs(const uint32_t* __restrict__ key, const DevState* __restrict__ st,
+                               int32_t* __restrict__ start, int32_t* __restrict__ end) {
   if (st->abort) return;
   const int R = st->R_ok;
-  // consecutive refs often share a cell: one atomic per distinct cell of a
-  // wave (wave_add); the loop bound is rounded up so that whole waves take
-  // part (lanes past R add nothing)
-  const int stride = gridDim.x * blockDim.x;
-  const int end = (R + 63) & ~63;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += stride)
-    wave_add(ccnt, i < R ? rcell[i] : -1, 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
+    const uint32_t k = key[i];
+    if (i == 0 || key[i - 1] != k) start[k] = i;
+    if (i == R - 1 || key[i + 1] != k) end[k] = i + 1;
+  }
+}
+
+// Refs per cell from its run: cnt[k] = end[k] - start[k], in place of end.
+__global__ void k_dv_run_counts(const int32_t* __restrict__ start, int32_t* __restrict__ end,
+                                const DevState* __restrict__ st) {
+  if (st->abort) return;
+  const int C = st->tot_c32;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < C; k += gridDim.x * blockDim.x) end[k] -= start[k];
 }
 
 // kernel_EC2CS_setCEInfo (:1030-1043) for bottom cells: the refs are in
@@ -2201,18 +2210,23 @@ int dv_enqueue_write(gahhg_ctx* c, const DvArena& a, int levels) {
   hipLaunchKernelGGL(k_dv_ref_edges, dim3(rg), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
                      (const int32_t*)c->d_rscan, (const int32_t*)c->g.c_poly, (const PolyDev*)c->d_pd,
                      (const DevState*)c->d_st, c->d_redge);
-  // refs per cell, their offsets, and the refs in cell-major order by a
-  // stable sort of (cell, edge) over the edge-major refs
-  hipLaunchKernelGGL(k_fill_dv, dim3(cg), dim3(kBlock), 0, c->stream, c->d_f1, (const int32_t*)&c->d_st->tot_c32, 0,
-                     (const DevState*)c->d_st);
-  hipLaunchKernelGGL(k_dv_count_cells, dim3(rg), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_rcell,
-                     (const DevState*)c->d_st, c->d_f1);
-  CK(hip_check(c, hipGetLastError(), "ref counts"));
-  ScanCols s2 = {{c->d_f1, nullptr, nullptr, nullptr}, {c->d_o1, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
-  CK(scan_cols_dev(c, s2, 1, &c->d_st->tot_c32, 0, d.csums));
+  // the refs in cell-major order by a stable sort of (cell, edge) over the
+  // edge-major refs, then refs per cell from the sorted runs and their
+  // offsets by a scan
   uint32_t *kres = nullptr, *vres = nullptr;
   CK(radix_sort_pairs(c, (uint32_t*)c->d_rcell, (uint32_t*)c->d_redge, (uint32_t*)c->d_rflag, (uint32_t*)c->d_rscan,
                       &c->d_st->R_ok, a.r, a.end_bit, d.hist, &kres, &vres, d.sscr));
+  hipLaunchKernelGGL(k_fill_dv, dim3(cg), dim3(kBlock), 0, c->stream, c->d_f0, (const int32_t*)&c->d_st->tot_c32, 0,
+                     (const DevState*)c->d_st);
+  hipLaunchKernelGGL(k_fill_dv, dim3(cg), dim3(kBlock), 0, c->stream, c->d_f1, (const int32_t*)&c->d_st->tot_c32, 0,
+                     (const DevState*)c->d_st);
+  hipLaunchKernelGGL(k_dv_cell_runs, dim3(rg), dim3(kBlock), 0, c->stream, (const uint32_t*)kres,
+                     (const DevState*)c->d_st, c->d_f0, c->d_f1);
+  hipLaunchKernelGGL(k_dv_run_counts, dim3(cg), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_f0, c->d_f1,
+                     (const DevState*)c->d_st);
+  CK(hip_check(c, hipGetLastError(), "ref counts"));
+  ScanCols s2 = {{c->d_f1, nullptr, nullptr, nullptr}, {c->d_o1, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  CK(scan_cols_dev(c, s2, 1, &c->d_st->tot_c32, 0, d.csums));
   hipLaunchKernelGGL(k_copy_dv, dim3(rg), dim3(kBlock), 0, c->stream, c->g.refs, (const int32_t*)vres,
                      (const int32_t*)&c->d_st->R_ok, (const DevState*)c->d_st);
   hipLaunchKernelGGL(k_dv_finish_cells, dim3(cg), dim3(kBlock), 0, c->stream, g, (const PolyDev*)c->d_pd,
