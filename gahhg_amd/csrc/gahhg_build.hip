@@ -99,22 +99,26 @@ __device__ int seg_exit_border(const GridView& g, V3 ne, V3 p0, V3 p1, int cid, 
   return -1;
 }
 
-// device_isPointInsideCell on a walk record (the same planes, same order).
+// device_isPointInsideCell on a walk record (the same planes; all of them
+// are fetched before any is tested, the result is the same).
 __device__ __forceinline__ int inside_rec(const WalkRec* __restrict__ rec, V3 p, int cid) {
   const WalkRec& R = rec[cid];
   const int bn = R.bnum & 0xff;
-  for (int k = 0; k < bn; k++) {
-    const double* n = R.b[k].n;
-    double r = n[0] * p.x + n[1] * p.y + n[2] * p.z;
-    if (r < -kEps) return -1;
+  bool in = true;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (k < bn) {
+      const double* n = R.b[k].n;
+      const double r = n[0] * p.x + n[1] * p.y + n[2] * p.z;
+      in = in && !(r < -kEps);
+    }
   }
-  return cid;
+  return in ? cid : -1;
 }
 
-// seg_exit_border on the record of cell cid, reading each border's fields
-// when the test reaches it (the record's address is known, so each border
-// costs one fetch, not the index-then-vertex chain of the grid arrays).
-// Returns the exit border's slot k or -1; *bs_out, *bn_out: the cell's
+// seg_exit_border on the record of cell cid (the record's address is known,
+// so a border costs one fetch, not the index-then-vertex chain of the grid
+// arrays).  Returns the exit border's slot k or -1; *bs_out, *bn_out: the cell's
 // border range.
 __device__ __forceinline__ int seg_exit_lazy(const GridView& g, const WalkRec* __restrict__ rec, int cid, V3 ne,
                                              V3 p0, V3 p1, int in, int* intertype, int* info, int* bs_out,
@@ -123,19 +127,29 @@ __device__ __forceinline__ int seg_exit_lazy(const GridView& g, const WalkRec* _
   const int bs = R.bstart, bn = R.bnum & 0xff;
   *bs_out = bs;
   *bn_out = bn;
-  double s = dot(ne, V3{R.b[0].v0[0], R.b[0].v0[1], R.b[0].v0[2]});
-  const double s_first = s;
-  for (int k = 0; k < bn; k++) {
-    const WalkBorder& B = R.b[k];
+  // every plane value of the record first: the loads of all borders are
+  // independent and go out together (testing border by border would wait
+  // for each border's fetch in turn)
+  double r0[6], r1[6], sv[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (k < bn) {
+      const WalkBorder& B = R.b[k];
+      const V3 nb = V3{B.n[0], B.n[1], B.n[2]};
+      r0[k] = dot(nb, p0);
+      r1[k] = dot(nb, p1);
+      sv[k] = dot(ne, V3{B.v0[0], B.v0[1], B.v0[2]});
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (k >= bn) break;
     const int kn = k + 1 < bn ? k + 1 : 0;
-    const double s1 = kn == 0 ? s_first : dot(ne, V3{R.b[kn].v0[0], R.b[kn].v0[1], R.b[kn].v0[2]});
-    const double s0 = s;
-    s = s1;
+    const double s0 = sv[k], s1 = sv[kn];
     const int i = bs + k;
     if (i == in) continue;
-    const V3 nb = V3{B.n[0], B.n[1], B.n[2]};
     int temp = 0;
-    const int inter = edge_border_class_r(dot(nb, p0), dot(nb, p1), s0, s1, &temp);
+    const int inter = edge_border_class_r(r0[k], r1[k], s0, s1, &temp);
     if (inter == kInterX) {
       *intertype = kInterX;
       return k;
