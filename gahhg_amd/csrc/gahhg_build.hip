@@ -39,6 +39,13 @@ constexpr int kBlock = 256;
 // levels 7-10 have 20 to 80 extra steps per cell and gain 0.6 ms in all; the
 // 256-polygon batch has at most 12 and loses 0.15 ms with records).
 constexpr double kWalkRecStepsPerCell = 16.0;
+// The mark walk keeps the first kSlotK cells of each edge's walk (walk
+// slots, device-driven build): the dispatch's write walk of a polygon that
+// stopped after marking its bottom level repeats that walk, so the edges
+// whose whole walk fits are copied from the slots and only the others are
+// walked again (fractal_1m: 89% of the edges at its bottom level).
+constexpr int kSlotK = 8;
+constexpr int32_t kSlotDirty = INT32_MIN;  // slot 0 of a walk that raised a diagnostic
 
 static inline unsigned blocks_for(int64_t n, int bs) {
   int64_t b = (n + bs - 1) / bs;
@@ -267,7 +274,8 @@ __device__ __forceinline__ int overlap_in_index(const GridView& g, const PolyDev
 // MODE kMark), calcRefNum (:685-818, kCount), writeRef (:820-955, kWrite).
 template <int MODE, bool REC>
 __device__ void walk_edge(const GridView& g, const WalkRec* __restrict__ rec, const double* __restrict__ edge, int e,
-                          int level, const PolyDev& P, int32_t* buf, int32_t* out_cells, WalkDiag& d) {
+                          int level, const PolyDev& P, int32_t* buf, int32_t* out_cells, WalkDiag& d,
+                          int32_t* __restrict__ slot = nullptr, int64_t slot_n = 0) {
   // REC false: this level has no walk records; the steps read the grid
   // arrays (the same values) and the step state stays small
   const double* er = edge + 9 * (int64_t)e;
@@ -316,7 +324,10 @@ __device__ void walk_edge(const GridView& g, const WalkRec* __restrict__ rec, co
       d.cap_hits++;
       break;
     }
-    if (MODE == kMark) g.c_rnum[cur] = kMarked;
+    if (MODE == kMark) {
+      g.c_rnum[cur] = kMarked;
+      if (slot && steps < kSlotK) slot[steps * slot_n + e] = cur;
+    }
     if (MODE == kWrite) out_cells[steps] = cur;
     steps++;
     int intertype = 0, info = 0;
@@ -418,7 +429,8 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __re
                                                  const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd,
                                                  int Ntot, int level, int32_t* buf, int32_t* buf_prev,
                                                  const int32_t* off, int32_t* cnt_out, int32_t* rcell,
-                                                 DevState* st, int dv) {
+                                                 DevState* st, int dv, int32_t* __restrict__ slot,
+                                                 const int32_t* __restrict__ elist) {
   if (dv) {
     if (MODE == kMark ? st->run == 0 : st->abort != 0) return;  // uniform over the launch
     rec_levels = st->rec_levels;
@@ -426,9 +438,13 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __re
   // a mark walk walks one level: the launch whose record mode is not that
   // level's has nothing to do
   if (MODE == kMark && ((((rec_levels >> level) & 1u) != 0) != REC)) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
   WalkDiag d = {0, 0, 0, 0, 0};
   bool mine = e < Ntot;
+  if (elist) {  // the edges listed in elist (st->nwalk of them)
+    mine = e < st->nwalk;
+    if (mine) e = elist[e];
+  }
   PolyDev P;
   if (mine) {
     P = pd[epoly[e]];
@@ -441,13 +457,15 @@ __global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __re
     if (MODE == kMark) {
       if (P.refining) {
         buf_prev[e] = buf[e];
-        walk_edge<MODE, REC>(g, rec, edge, e, level, P, buf, nullptr, d);
+        walk_edge<MODE, REC>(g, rec, edge, e, level, P, buf, nullptr, d, slot, Ntot);
         // the count walk of a polygon that stops at this level repeats this
         // walk exactly (same start, same cells): keep its count, unless
         // the walk raised a diagnostic (then the count walk runs and
-        // counts it as the reference's dispatch does)
+        // counts it as the reference's dispatch does, and the write walk
+        // walks it again)
         const bool clean = d.partner_missing == 0 && d.cap_hits == 0 && d.unlocated == 0 && d.bad_index == 0;
         cnt_out[e] = clean ? (int32_t)d.steps : -1;
+        if (slot && !clean) slot[e] = kSlotDirty;
       }
     } else {
       int32_t* b = P.use_prev ? buf_prev : buf;
@@ -1187,6 +1205,35 @@ __global__ void k_dv_poly_refs(PolyDev* pd, int np, const DevState* __restrict__
   plv[(int64_t)p * kMaxLevels + P.L].rnum = P.R;
 }
 
+// Write phase of the device-driven dispatch, first part: an edge of a
+// polygon that stopped after marking its bottom level (use_prev), whose
+// clean mark walk there fitted the walk slots, gets its cells (in walk
+// order, the write walk's) from the slots; every other edge is flagged for
+// the write walk.
+__global__ void k_dv_slot_copy(const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd, int Ntot,
+                               const int32_t* __restrict__ slot, const int32_t* __restrict__ cnt,
+                               const int32_t* __restrict__ off, int32_t* __restrict__ rcell,
+                               int32_t* __restrict__ flag, const DevState* __restrict__ st) {
+  if (st->abort) return;  // the refs may not fit: nothing is written
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Ntot) return;
+  const int n = cnt[e];
+  const bool copy = pd[epoly[e]].use_prev && n <= kSlotK && slot[e] != kSlotDirty;
+  flag[e] = copy ? 0 : 1;
+  if (copy) {
+    int32_t* o = rcell + off[e];
+    for (int k = 0; k < n; k++) o[k] = slot[(int64_t)k * Ntot + e];
+  }
+}
+
+// The flagged edges in edge order (pos: exclusive scan of the flags).
+__global__ void k_dv_walk_list(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int Ntot,
+                               int32_t* __restrict__ list, const DevState* __restrict__ st) {
+  if (st->abort) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < Ntot && flag[e]) list[pos[e]] = e;
+}
+
 __global__ void k_dv_check_refs(DevState* st, int64_t rcap) {
   if (st->abort) return;
   const int32_t R = st->R;
@@ -1698,12 +1745,12 @@ int build_hierarchy(gahhg_ctx* c) {
       hipLaunchKernelGGL((k_walk<kMark, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
     else
       hipLaunchKernelGGL((k_walk<kMark, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
     LAUNCH_CK("mark walk");
     hipLaunchKernelGGL(k_plan, dim3(blocks_for(GL.cnum, kBlock)), dim3(kBlock), 0, c->stream, g, GL.cstart, GL.cnum,
                        c->d_f0, c->d_f1, c->d_f2, c->d_f3);
@@ -1869,12 +1916,12 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL((k_walk<kCount, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
+                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
   if (bottom_plain)
     hipLaunchKernelGGL((k_walk<kCount, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0);
+                       c->d_ecnt, (int32_t*)nullptr, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
   LAUNCH_CK("count walk");
   CK(scan_i32(c, c->d_ecnt, c->d_o0, Nt, nullptr, &c->d_st->R, 0));
   hipLaunchKernelGGL(k_poly_refs, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, c->stream, c->d_pd, np,
@@ -1891,12 +1938,12 @@ int build_hierarchy(gahhg_ctx* c) {
     hipLaunchKernelGGL((k_walk<kWrite, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0);
+                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
   if (bottom_plain)
     hipLaunchKernelGGL((k_walk<kWrite, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                        (const WalkRec*)c->d_wrec, rec_levels, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                        (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0);
+                       (int32_t*)nullptr, c->d_rcell, c->d_st, 0, (int32_t*)nullptr, (const int32_t*)nullptr);
   LAUNCH_CK("write walk");
   CK(fill_i32(c, c->d_rflag, R, 0));
   hipLaunchKernelGGL(k_edge_heads, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, (const int32_t*)c->d_o0,
@@ -2012,6 +2059,13 @@ int dv_prepare(gahhg_ctx* c, const int64_t* want, DvArena* a) {
     c->d_ecnt = nullptr;
     CK(dalloc(c, (void**)&c->d_ecnt, Nt * 4));
     c->ecap = Nt;
+  }
+  if (c->wslot_cap < Nt) {
+    CK_SYNC(c);
+    dfree(c->d_wslot);
+    c->d_wslot = nullptr;
+    CK(dalloc(c, (void**)&c->d_wslot, std::max<int64_t>(Nt, 1) * kSlotK * 4));
+    c->wslot_cap = Nt;
   }
   a->v = c->g.vcap;
   a->b = c->g.bcap;
@@ -2134,11 +2188,11 @@ int dv_enqueue_level(gahhg_ctx* c, const DvArena& a, int lv) {
   hipLaunchKernelGGL((k_walk<kMark, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                      (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                      (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1, c->d_wslot, (const int32_t*)nullptr);
   hipLaunchKernelGGL((k_walk<kMark, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                      (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                      (const PolyDev*)c->d_pd, (int)Nt, lv, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+                     c->d_ecnt, (int32_t*)nullptr, c->d_st, 1, c->d_wslot, (const int32_t*)nullptr);
   hipLaunchKernelGGL(k_plan_dv, dim3(cg), dim3(kBlock), 0, c->stream, g, (const DevState*)c->d_st, lv, c->d_f0,
                      c->d_f1, c->d_f2, c->d_f3);
   CK(hip_check(c, hipGetLastError(), "device build level"));
@@ -2179,12 +2233,12 @@ int dv_enqueue_count(gahhg_ctx* c) {
       hipLaunchKernelGGL((k_walk<kCount, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1, (int32_t*)nullptr, (const int32_t*)nullptr);
     else
       hipLaunchKernelGGL((k_walk<kCount, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)nullptr,
-                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1);
+                         c->d_ecnt, (int32_t*)nullptr, c->d_st, 1, (int32_t*)nullptr, (const int32_t*)nullptr);
   }
   CK(hip_check(c, hipGetLastError(), "count walk"));
   CK(scan_i32_with(c, c->d_ecnt, c->d_o0, Nt, nullptr, &c->d_st->R, 0, d.sscr));
@@ -2201,17 +2255,26 @@ int dv_enqueue_write(gahhg_ctx* c, const DvArena& a, int levels) {
   const GridView g = view(c);
   const unsigned rg = dv_grid(a.r), cg = dv_grid(a.c);
   hipLaunchKernelGGL(k_dv_check_refs, dim3(1), dim3(1), 0, c->stream, c->d_st, a.r);
+  // edges whose cells the walk slots hold are copied; the write walk walks
+  // the others, listed in edge order
+  hipLaunchKernelGGL(k_dv_slot_copy, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream,
+                     (const int32_t*)c->d_epoly, (const PolyDev*)c->d_pd, (int)Nt, (const int32_t*)c->d_wslot,
+                     (const int32_t*)c->d_ecnt, (const int32_t*)c->d_o0, c->d_rcell, c->d_f0,
+                     (const DevState*)c->d_st);
+  CK(scan_i32_with(c, c->d_f0, c->d_f1, Nt, nullptr, &c->d_st->nwalk, 0, d.sscr));
+  hipLaunchKernelGGL(k_dv_walk_list, dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream,
+                     (const int32_t*)c->d_f0, (const int32_t*)c->d_f1, (int)Nt, c->d_f2, (const DevState*)c->d_st);
   for (int w = 0; w < 2; w++) {
     if (w == 0)
       hipLaunchKernelGGL((k_walk<kWrite, true>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1);
+                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1, (int32_t*)nullptr, (const int32_t*)c->d_f2);
     else
       hipLaunchKernelGGL((k_walk<kWrite, false>), dim3(blocks_for(Nt, kBlock)), dim3(kBlock), 0, c->stream, g,
                          (const WalkRec*)c->d_wrec, 0u, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const PolyDev*)c->d_pd, (int)Nt, 0, c->d_loc_cur, c->d_loc_prev, (const int32_t*)c->d_o0,
-                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1);
+                         (int32_t*)nullptr, c->d_rcell, c->d_st, 1, (int32_t*)nullptr, (const int32_t*)c->d_f2);
   }
   hipLaunchKernelGGL(k_fill_dv, dim3(rg), dim3(kBlock), 0, c->stream, c->d_rflag, (const int32_t*)&c->d_st->R_ok, 0,
                      (const DevState*)c->d_st);
@@ -2260,7 +2323,7 @@ std::vector<uintptr_t> dv_key(gahhg_ctx* c, const DvArena& a, int levels) {
       (uintptr_t)c->stream, (uintptr_t)c->d_pv, (uintptr_t)c->d_pe, (uintptr_t)c->d_edge, (uintptr_t)c->d_epoly,
       (uintptr_t)c->d_pd, (uintptr_t)c->d_st, (uintptr_t)c->d_loc_cur, (uintptr_t)c->d_loc_prev,
       (uintptr_t)c->d_f0, (uintptr_t)c->d_f1, (uintptr_t)c->d_f2, (uintptr_t)c->d_f3, (uintptr_t)c->d_o0,
-      (uintptr_t)c->d_o1, (uintptr_t)c->d_o2, (uintptr_t)c->d_o3, (uintptr_t)c->d_ecnt,
+      (uintptr_t)c->d_o1, (uintptr_t)c->d_o2, (uintptr_t)c->d_o3, (uintptr_t)c->d_ecnt, (uintptr_t)c->d_wslot,
       (uintptr_t)c->d_rcell, (uintptr_t)c->d_rflag, (uintptr_t)c->d_redge, (uintptr_t)c->d_rscan,
       (uintptr_t)c->d_wrec, (uintptr_t)d.g0_v, (uintptr_t)d.g0_n, (uintptr_t)d.g0_b, (uintptr_t)d.g0_c,
       (uintptr_t)d.pd0, (uintptr_t)d.plv, (uintptr_t)d.plvn, (uintptr_t)d.gamma, (uintptr_t)d.pmarks,

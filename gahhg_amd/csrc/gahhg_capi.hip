@@ -79,7 +79,9 @@ void free_scratch(gahhg_ctx* c) {
   c->d_rcell = c->d_rflag = c->d_redge = c->d_rscan = c->d_loc_cur = c->d_loc_prev = nullptr;
   dfree(c->d_wrec);
   c->d_wrec = nullptr;
-  c->fcap = c->scap = c->ecap = c->r2cap = c->wrec_cap = 0;
+  dfree(c->d_wslot);
+  c->d_wslot = nullptr;
+  c->fcap = c->scap = c->ecap = c->r2cap = c->wrec_cap = c->wslot_cap = 0;
 }
 
 int upload_polygons(gahhg_ctx* c) {

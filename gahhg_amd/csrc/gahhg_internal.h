@@ -151,6 +151,7 @@ struct DevState {
   unsigned long long und_first;     // leaves unlabelled after each level's first setIECell round
   int32_t ie_n;       // leaves of the level being classified left unlabelled by that round
   int32_t cls_G[kMaxLevels];        // lanes per cell of each level's getCPProp / parity groups
+  int32_t nwalk;      // edges the dispatch's write walk walks (the others are copied from walk slots)
 };
 
 // Structure-of-arrays hierarchy in HBM.  Vertices and border normals are
@@ -555,6 +556,8 @@ struct gahhg_ctx {
   int32_t* d_scan = nullptr;
   int64_t ecap = 0;
   int32_t* d_ecnt = nullptr;  // per edge
+  int64_t wslot_cap = 0;      // edges the walk slots hold
+  int32_t* d_wslot = nullptr; // per edge: the first cells of its last mark walk (device-driven build)
   int64_t r2cap = 0;
   int32_t *d_rcell = nullptr, *d_rflag = nullptr, *d_redge = nullptr, *d_rscan = nullptr;
   gahhg::WalkRec* d_wrec = nullptr;  // walk record of every cell (build scratch)

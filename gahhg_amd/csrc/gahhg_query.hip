@@ -824,7 +824,7 @@ __global__ void k_index_list(const int32_t* __restrict__ flag, const int32_t* __
 __global__ void __launch_bounds__(256) k_index_size(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
                                                     const double* __restrict__ sv, const uint8_t* __restrict__ sf,
                                                     const int32_t* __restrict__ list, int nlist,
-                                                    int32_t* __restrict__ isize) {
+                                                    int32_t* __restrict__ isize, uint16_t* __restrict__ code) {
   const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= nlist) return;
@@ -836,9 +836,11 @@ __global__ void __launch_bounds__(256) k_index_size(const double* __restrict__ q
   for (int base = 0; base < m - 1; base += 64) {
     const int k = base + lane;
     int nb = 0;
-    if (k < m - 1 && (sf[r0 + k] & 4)) {
-      int b0;
-      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+    if (k < m - 1) {
+      int b0 = 0;
+      if (sf[r0 + k] & 4)
+        edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+      code[r0 + k] = (uint16_t)(b0 | (nb << 8));  // the buckets again for k_index_fill
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o, 64);
@@ -886,9 +888,10 @@ __global__ void __launch_bounds__(256) k_index_poly_sums(const PolyDev* __restri
 
 // Index contents at ioff[cell] (leaves sized by k_index_size): kIdxB + 1
 // offsets (relative to the entries), then per bucket its records in
-// ascending order; ib[cell] = ioff[cell].
-__global__ void __launch_bounds__(256) k_index_fill(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
-                                                    const double* __restrict__ sv, const uint8_t* __restrict__ sf,
+// ascending order; ib[cell] = ioff[cell].  The buckets of each record are
+// the codes k_index_size left (first bucket | count << 8).
+__global__ void __launch_bounds__(256) k_index_fill(const int32_t* __restrict__ hdr,
+                                                    const uint16_t* __restrict__ code,
                                                     const int32_t* __restrict__ list, int nlist,
                                                     const int32_t* __restrict__ ioff, int32_t* __restrict__ ib,
                                                     int32_t* __restrict__ idx) {
@@ -900,8 +903,6 @@ __global__ void __launch_bounds__(256) k_index_fill(const double* __restrict__ q
   const int base = ioff[cell];
   int32_t* off = idx + base;
   int32_t* ent = off + kIdxB + 1;
-  V3 u, v;
-  leaf_basis(load3(qc + 3 * (int64_t)cell), u, v);
   const unsigned long long below = (1ull << lane) - 1ull;
   // pass 1: entries per bucket
   int cnt[kIdxB];
@@ -909,9 +910,8 @@ __global__ void __launch_bounds__(256) k_index_fill(const double* __restrict__ q
   for (int b = 0; b < kIdxB; b++) cnt[b] = 0;
   for (int c0 = 0; c0 < m - 1; c0 += 64) {
     const int k = c0 + lane;
-    int b0 = 0, nb = 0;
-    if (k < m - 1 && (sf[r0 + k] & 4))
-      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+    const int cb = k < m - 1 ? code[r0 + k] : 0;
+    const int b0 = cb & 0xff, nb = cb >> 8;
 #pragma unroll
     for (int b = 0; b < kIdxB; b++) {
       const bool in = nb > 0 && ((b - b0 + kIdxB) % kIdxB) < nb;
@@ -933,9 +933,8 @@ __global__ void __launch_bounds__(256) k_index_fill(const double* __restrict__ q
   // pass 2: entries, in record order within each bucket
   for (int c0 = 0; c0 < m - 1; c0 += 64) {
     const int k = c0 + lane;
-    int b0 = 0, nb = 0;
-    if (k < m - 1 && (sf[r0 + k] & 4))
-      edge_buckets(load3(sv + 3 * (int64_t)(r0 + k)), load3(sv + 3 * (int64_t)(r0 + k + 1)), u, v, b0, nb);
+    const int cb = k < m - 1 ? code[r0 + k] : 0;
+    const int b0 = cb & 0xff, nb = cb >> 8;
 #pragma unroll
     for (int b = 0; b < kIdxB; b++) {
       const bool in = nb > 0 && ((b - b0 + kIdxB) % kIdxB) < nb;
@@ -1293,8 +1292,9 @@ static int build_leaf_index(gahhg_ctx* c) {
   if (nc <= 0 || c->prm.query_index == 0) return GAHHG_OK;
   const int np = (int)c->polys.size();
   int32_t* scratch = nullptr;
-  // sizes, offsets, total (+pad), per-polygon sums, flags, positions, count, list
-  const int64_t words = 2 * nc + 2 + 4 * (int64_t)np + 3 * nc + 2;
+  // sizes, offsets, total (+pad), per-polygon sums, flags, positions, count,
+  // list, bucket codes (one uint16 per record)
+  const int64_t words = 2 * nc + 2 + 4 * (int64_t)np + 3 * nc + 2 + ((int64_t)q.nrec + 1) / 2 + 1;
   if (use_buf(c, q.b_ib, nc * 4, (void**)&q.ib) || use_buf(c, q.b_iscr, words * 4, (void**)&scratch))
     return GAHHG_E_NOMEM;
   int32_t* isize = scratch;
@@ -1305,6 +1305,7 @@ static int build_leaf_index(gahhg_ctx* c) {
   int32_t* pos = flag + nc;
   int32_t* nlist_d = pos + nc;
   int32_t* list = nlist_d + 2;
+  uint16_t* code = (uint16_t*)(list + nc);
   const unsigned cb = (unsigned)((nc + 255) / 256);
   hipLaunchKernelGGL(k_index_flag, dim3(cb), dim3(256), 0, c->stream, (const int32_t*)q.hdr, (int)nc, flag, isize,
                      q.ib);
@@ -1320,7 +1321,7 @@ static int build_leaf_index(gahhg_ctx* c) {
   const unsigned lb = (unsigned)(((int64_t)std::max(nlist, 1) * 64 + 255) / 256);
   if (!rc && nlist > 0) {
     hipLaunchKernelGGL(k_index_size, dim3(lb), dim3(256), 0, c->stream, (const double*)q.qc, (const int32_t*)q.hdr,
-                       (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, isize);
+                       (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, isize, code);
     rc = hip_check(c, hipGetLastError(), "index size");
   }
   if (!rc) rc = fill_i32(c, (int32_t*)pstat, 4 * (int64_t)np, 0);
@@ -1370,9 +1371,8 @@ static int build_leaf_index(gahhg_ctx* c) {
   q.nidx = h;
   q.bytes += nc * 4 + (int64_t)h * 4;
   if (nlist > 0)
-    hipLaunchKernelGGL(k_index_fill, dim3(lb), dim3(256), 0, c->stream, (const double*)q.qc, (const int32_t*)q.hdr,
-                       (const double*)q.sv, (const uint8_t*)q.sf, (const int32_t*)list, nlist, (const int32_t*)ioff,
-                       q.ib, q.idx);
+    hipLaunchKernelGGL(k_index_fill, dim3(lb), dim3(256), 0, c->stream, (const int32_t*)q.hdr,
+                       (const uint16_t*)code, (const int32_t*)list, nlist, (const int32_t*)ioff, q.ib, q.idx);
   return hip_check(c, hipGetLastError(), "index fill");
 }
 
