@@ -1618,39 +1618,64 @@ __device__ __forceinline__ uint32_t group_key(V3 p, uint32_t poly, int r) {
 // counts[block] = the block's points left for the walk.
 constexpr uint32_t kNoKey = 0xffffffffu;
 
+// Each block takes kTableSub consecutive 256-point tiles (counts per tile,
+// as the compaction reads them), with every tile's loads in flight
+// together: a tile's chain (point, table entry, sub-table entry) is a few
+// dependent loads, and four chains per thread hide each other's latency.
+constexpr int kTableSub = 4;
 __global__ void __launch_bounds__(256) k_query_table(const PolyDev* __restrict__ pd, const int32_t* __restrict__ table,
                                                      const int32_t* __restrict__ sub, int B,
                                                      const double* __restrict__ xyz,
                                                      const int32_t* __restrict__ qpoly, int64_t n, int r,
                                                      int32_t* __restrict__ labels, uint32_t* __restrict__ key,
-                                                     int32_t* __restrict__ counts) {
-  __shared__ int part[4];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool keep = false;
-  if (t < n) {
-    const V3 p = load3(xyz + 3 * t);
-    const uint32_t poly = qpoly ? (uint32_t)qpoly[t] : 0u;
-    const PolyDev& P = pd[poly];
-    double fx = 0.0, fy = 0.0;
-    const int bin = P.tbins > 0 ? bin_of(p, P.tbins, fx, fy) : -1;
-    int e = bin >= 0 ? table[(int64_t)P.tbl + bin] : -1;
-    if (e <= -kSubCode) {
-      const int S = P.tsub;
-      const int si = min(max((int)(fx * S), 0), S - 1), sj = min(max((int)(fy * S), 0), S - 1);
-      e = sub[(int64_t)(-e - kSubCode) + si * S + sj];
-    }
-    if (e <= -2) {
-      labels[t] = -e - 2;
-      key[t] = kNoKey;
-    } else {
-      keep = true;
-      key[t] = group_key(p, poly, r);
+                                                     int32_t* __restrict__ counts, int64_t nblk) {
+  __shared__ int part[kTableSub][4];
+  V3 p[kTableSub];
+  uint32_t poly[kTableSub];
+  int64_t t[kTableSub];
+  bool ok[kTableSub];
+#pragma unroll
+  for (int j = 0; j < kTableSub; j++) {
+    t[j] = ((int64_t)blockIdx.x * kTableSub + j) * 256 + threadIdx.x;
+    ok[j] = t[j] < n;
+    p[j] = ok[j] ? load3(xyz + 3 * t[j]) : v3(1.0, 0.0, 0.0);
+    poly[j] = (ok[j] && qpoly) ? (uint32_t)qpoly[t[j]] : 0u;
+  }
+  int e[kTableSub];
+  double fx[kTableSub], fy[kTableSub];
+#pragma unroll
+  for (int j = 0; j < kTableSub; j++) {
+    const PolyDev& P = pd[poly[j]];
+    fx[j] = fy[j] = 0.0;
+    const int bin = P.tbins > 0 ? bin_of(p[j], P.tbins, fx[j], fy[j]) : -1;
+    e[j] = (ok[j] && bin >= 0) ? table[(int64_t)P.tbl + bin] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kTableSub; j++) {
+    if (e[j] <= -kSubCode) {
+      const int S = pd[poly[j]].tsub;
+      const int si = min(max((int)(fx[j] * S), 0), S - 1), sj = min(max((int)(fy[j] * S), 0), S - 1);
+      e[j] = sub[(int64_t)(-e[j] - kSubCode) + si * S + sj];
     }
   }
-  const int c = __popcll(__ballot(keep));
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+#pragma unroll
+  for (int j = 0; j < kTableSub; j++) {
+    bool keep = false;
+    if (ok[j]) {
+      // every point's label slot is written, so that the stores fill whole
+      // lines; the walk overwrites the placeholder of each kept point
+      keep = e[j] > -2;
+      labels[t[j]] = keep ? kUndefined : -e[j] - 2;
+      key[t[j]] = keep ? group_key(p[j], poly[j], r) : kNoKey;
+    }
+    const int c = __popcll(__ballot(keep));
+    if ((threadIdx.x & 63) == 0) part[j][threadIdx.x >> 6] = c;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+  if (threadIdx.x < kTableSub) {
+    const int64_t sb = (int64_t)blockIdx.x * kTableSub + threadIdx.x;
+    if (sb < nblk) counts[sb] = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
+  }
 }
 
 // Second pass: the kept points of table-pass block sb, in point order, to
@@ -1750,9 +1775,9 @@ int sort_queries(gahhg_ctx* c, const double* d_xyz, const int32_t* d_poly, int64
   const unsigned blocks = (unsigned)nb;
   if (c->q.table) {
     const QueryLayout& L = c->q;
-    hipLaunchKernelGGL(k_query_table, dim3(blocks), dim3(256), 0, c->stream, (const PolyDev*)c->d_pd,
-                       (const int32_t*)L.table, (const int32_t*)L.sub, L.bins, d_xyz, d_poly, n, r, d_labels, k1,
-                       bcount);
+    hipLaunchKernelGGL(k_query_table, dim3((unsigned)((nb + kTableSub - 1) / kTableSub)), dim3(256), 0, c->stream,
+                       (const PolyDev*)c->d_pd, (const int32_t*)L.table, (const int32_t*)L.sub, L.bins, d_xyz, d_poly,
+                       n, r, d_labels, k1, bcount, nb);
     rc = hip_check(c, hipGetLastError(), "table pass");
     if (rc) return rc;
     rc = scan_i32_with(c, bcount, boffs, nb, nullptr, btotal, 0, s.scan);
