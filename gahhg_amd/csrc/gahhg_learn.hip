@@ -170,19 +170,48 @@ __global__ void k_level_stats_last(GridView g, const PolyDev* __restrict__ pd, L
   wave_add<int32_t>(st, key >= 0 ? key + 1 : -1, crossed);
 }
 
+// descend() on the query layout's node records (the same planes and child
+// order, one record fetch per candidate cell).
+__device__ __forceinline__ bool node_in(const int32_t* __restrict__ nodes, int cid, V3 q) {
+  const int32_t* r = nodes + (int64_t)cid * kNodeInts;
+  const int bn = r[1] & 0xff;
+  const double* nr = reinterpret_cast<const double*>(r + 10);
+  for (int k = 0; k < bn; k++)
+    if (lt0(dot(v3(nr[3 * k], nr[3 * k + 1], nr[3 * k + 2]), q))) return false;
+  return true;
+}
+__device__ __forceinline__ int descend_nodes_n(const int32_t* __restrict__ nodes, V3 q, int cur, int steps) {
+  for (int s = 0; s < steps && cur >= 0; s++) {
+    const int32_t* r = nodes + (int64_t)cur * kNodeInts;
+    const int sc = r[0];
+    if (sc == -1) return -1;
+    int next = node_in(nodes, sc, q) ? sc : -1;
+    const int nch = (r[1] >> 8) & 0xff;
+    for (int i = 0; i < nch && next < 0; i++) {
+      const int tc = r[2 + i];
+      if (node_in(nodes, tc, q)) next = tc;
+    }
+    cur = next;
+  }
+  return cur;
+}
+
 // k_hint_band after a build: each hint point descends from its top-level
 // cell (loc) to its polygon's last refined level and counts when the cell
-// it reaches there is refined.
-__global__ void k_hint_band_last(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ xyz,
-                                 const int32_t* __restrict__ hpoly, const int32_t* __restrict__ loc, int64_t n,
-                                 int npoly, int64_t nbins, uint32_t* __restrict__ band) {
+// it reaches there is refined.  nodes: the query layout's node records,
+// when packed (else the grid arrays).
+__global__ void k_hint_band_last(GridView g, const int32_t* __restrict__ nodes, const PolyDev* __restrict__ pd,
+                                 const double* __restrict__ xyz, const int32_t* __restrict__ hpoly,
+                                 const int32_t* __restrict__ loc, int64_t n, int npoly, int64_t nbins,
+                                 uint32_t* __restrict__ band) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   if (t < n) {
     const int p = hpoly ? hpoly[t] : 0;
     int cur = loc[t];
     if (cur >= 0 && p >= 0 && p < npoly) {
-      cur = descend(g, load3(xyz + 3 * t), cur, last_refined(pd[p]));
+      const V3 q = load3(xyz + 3 * t);
+      cur = nodes ? descend_nodes_n(nodes, q, cur, last_refined(pd[p])) : descend(g, q, cur, last_refined(pd[p]));
       if (cur >= 0 && g.c_subcell[cur] != -1) key = g.c_poly[cur] * kGrid0Cells + g.c_root[cur];
     }
   }
@@ -513,6 +542,7 @@ int table_features_after_build(gahhg_ctx* c, float* h_x) {
     rc = fill_i32(c, (int32_t*)L.d_stripes, pr * kStripes, 0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_hint_band_last, dim3(blocks(L.hint_n, 256)), dim3(256), 0, c->stream, view(c),
+                       c->prm.query_layout == 1 ? (const int32_t*)nullptr : (const int32_t*)c->q.nodes,
                        (const PolyDev*)c->d_pd, (const double*)L.d_hint_xyz, (const int32_t*)L.d_hint_poly,
                        (const int32_t*)L.d_hloc0, L.hint_n, np, pr, L.d_stripes);
     rc = hip_check(c, hipGetLastError(), "hint band");

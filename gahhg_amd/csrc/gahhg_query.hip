@@ -404,32 +404,89 @@ __device__ void bin_cap(int f, int B, int i, int j, int S, int si, int sj, V3& c
 // starting from cell `from` (-1: the level-0 scan of the polygon's cells
 // cell0 .. cell0 + 41).  Codes: >= 0 start cell, -1 start with the level-0
 // scan, -(2 + label) resolved leaf without refs.
+// cell_state_cap on a node record: the same normals, in the same order,
+// from one fetch.
+__device__ int node_state_cap(const int32_t* __restrict__ nodes, int cid, V3 u, double cr, double sr) {
+  const int32_t* r = nodes + (int64_t)cid * kNodeInts;
+  const int bn = r[1] & 0xff;
+  const double* nr = reinterpret_cast<const double*>(r + 10);
+  bool all_in = true;
+  for (int i = 0; i < bn; i++) {
+    const V3 n = v3(nr[3 * i], nr[3 * i + 1], nr[3 * i + 2]);
+    const double nn = sqrt(dot(n, n));
+    const double ct = fmin(1.0, fmax(-1.0, dot(n, u) / nn));
+    const double st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+    const double lo = (ct >= -cr ? nn * (ct * cr - st * sr) : -nn) - 1e-10;
+    const double hi = (ct <= cr ? nn * (ct * cr + st * sr) : nn) + 1e-10;
+    if (hi < -kEps) return 0;
+    if (!(lo >= -kEps)) all_in = false;
+  }
+  return all_in ? 1 : 2;
+}
+
 // Level-0 part of cap_entry: the level-0 cell (0..41, relative to cell0)
 // that the scan certainly matches first for every point of the cap, or -1.
-__device__ int cap_level0(const GridView& g, int cell0, V3 cen, double cr, double sr) {
+// With node records (nodes != nullptr) every cell test is one record fetch.
+__device__ int cap_level0(const GridView& g, const int32_t* __restrict__ nodes, int cell0, V3 cen, double cr,
+                          double sr) {
   for (int c = 0; c < kGrid0Cells; c++) {
-    const int s = cell_state_cap(g, cell0 + c, cen, cr, sr);
+    const int s = nodes ? node_state_cap(nodes, cell0 + c, cen, cr, sr) : cell_state_cap(g, cell0 + c, cen, cr, sr);
     if (s == 0) continue;
     return s == 1 ? c : -1;  // certain first match, or undecided: stop either way
   }
   return -1;
 }
 
-__device__ int32_t cap_descend(const GridView& g, int cur, V3 cen, double cr, double sr);
+__device__ int32_t cap_descend(const GridView& g, const int32_t* __restrict__ nodes, int cur, V3 cen, double cr,
+                               double sr);
 
-__device__ int32_t cap_entry(const GridView& g, int cell0, int from, V3 cen, double rho) {
+__device__ int32_t cap_entry(const GridView& g, const int32_t* __restrict__ nodes, int cell0, int from, V3 cen,
+                             double rho) {
   const double cr = cos(rho), sr = sin(rho);
   int cur = from;
   if (cur < 0) {
-    const int c = cap_level0(g, cell0, cen, cr, sr);
+    const int c = cap_level0(g, nodes, cell0, cen, cr, sr);
     if (c < 0) return -1;
     cur = cell0 + c;
   }
-  return cap_descend(g, cur, cen, cr, sr);
+  return cap_descend(g, nodes, cur, cen, cr, sr);
 }
 
-// From cell cur: descend while the first-match child is certain.
-__device__ int32_t cap_descend(const GridView& g, int cur, V3 cen, double cr, double sr) {
+// From cell cur: descend while the first-match child is certain.  With node
+// records a level reads the record of cur (centre child, border children)
+// and one record per candidate child.
+__device__ int32_t cap_descend(const GridView& g, const int32_t* __restrict__ nodes, int cur, V3 cen, double cr,
+                               double sr) {
+  if (nodes) {
+    for (;;) {
+      const int32_t* r = nodes + (int64_t)cur * kNodeInts;
+      const int sc = r[0];
+      if (sc == -1) break;
+      const int nch = (r[1] >> 8) & 0xff;
+      int next = -1;
+      bool stop = false;
+      int s = node_state_cap(nodes, sc, cen, cr, sr);
+      if (s == 1) {
+        next = sc;
+      } else if (s == 2) {
+        stop = true;
+      } else {
+        for (int k = 0; k < nch; k++) {
+          const int tc = r[2 + k];
+          s = node_state_cap(nodes, tc, cen, cr, sr);
+          if (s == 0) continue;
+          if (s == 1) next = tc;
+          else stop = true;
+          break;
+        }
+      }
+      if (stop || next < 0) break;
+      cur = next;
+    }
+    const int2 h = *reinterpret_cast<const int2*>(nodes + (int64_t)cur * kNodeInts);
+    if (h.x == -1 && (h.y >> 24) == 0) return -(2 + ((h.y >> 16) & 0xff));
+    return cur;
+  }
   for (;;) {
     const int sc = g.c_subcell[cur];
     if (sc == -1) break;
@@ -470,7 +527,8 @@ __host__ __device__ __forceinline__ int64_t l0_offset(int B) {
 // Level-0 part of every bin's entry for 16, 32, ..., Bmax bins per side.
 // Level-0 geometry is the same for every polygon (polygon 0's cells are
 // used), so a batch computes it once per resolution, not once per polygon.
-__global__ void k_level0_table(GridView g, int cell0, int Bmax, int32_t* __restrict__ out) {
+__global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, int cell0, int Bmax,
+                               int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= l0_offset(2 * Bmax)) return;
   int B = 16;
@@ -482,14 +540,14 @@ __global__ void k_level0_table(GridView g, int cell0, int Bmax, int32_t* __restr
   V3 cen;
   double rho;
   bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
-  out[t] = cap_level0(g, cell0, cen, cos(rho), sin(rho));
+  out[t] = cap_level0(g, nodes, cell0, cen, cos(rho), sin(rho));
 }
 
 // Entry tt of all polygons' tables: the polygon owning it is found by a
 // binary search over the tables' first entries (tstart, np + 1 values).
 // With l0 (a batch), the level-0 part comes from k_level0_table.
-__global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int npoly,
-                              const int32_t* __restrict__ tstart, const int32_t* __restrict__ l0,
+__global__ void k_build_table(GridView g, const int32_t* __restrict__ nodes, const PolyDev* __restrict__ pd,
+                              int npoly, const int32_t* __restrict__ tstart, const int32_t* __restrict__ l0,
                               int32_t* __restrict__ table_all) {
   const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tt >= tstart[npoly]) return;
@@ -510,9 +568,9 @@ __global__ void k_build_table(GridView g, const PolyDev* __restrict__ pd, int np
   bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
   if (l0) {
     const int c = l0[l0_offset(B) + t];
-    table_all[tt] = c < 0 ? -1 : cap_descend(g, pd[poly].cell0 + c, cen, cos(rho), sin(rho));
+    table_all[tt] = c < 0 ? -1 : cap_descend(g, nodes, pd[poly].cell0 + c, cen, cos(rho), sin(rho));
   } else {
-    table_all[tt] = cap_entry(g, pd[poly].cell0, -1, cen, rho);
+    table_all[tt] = cap_entry(g, nodes, pd[poly].cell0, -1, cen, rho);
   }
 }
 
@@ -555,7 +613,8 @@ __global__ void k_sub_list(int64_t n, const int32_t* __restrict__ has, const int
 
 // One 64-lane block per bin with a sub-table: its S^2 entries, each
 // descending from the bin's own entry.
-__global__ void __launch_bounds__(64) k_sub_build(GridView g, const PolyDev* __restrict__ pd, int B, int64_t per,
+__global__ void __launch_bounds__(64) k_sub_build(GridView g, const int32_t* __restrict__ nodes,
+                                                  const PolyDev* __restrict__ pd, int B, int64_t per,
                                                   const int32_t* __restrict__ list,
                                                   const int32_t* __restrict__ table,
                                                   const int32_t* __restrict__ off, int32_t* __restrict__ sub) {
@@ -573,7 +632,7 @@ __global__ void __launch_bounds__(64) k_sub_build(GridView g, const PolyDev* __r
     V3 cen;
     double rho;
     bin_cap(f, B, i, j, S, s / S, s % S, cen, rho);
-    o[s] = cap_entry(g, P.cell0, from, cen, rho);
+    o[s] = cap_entry(g, nodes, P.cell0, from, cen, rho);
   }
 }
 
@@ -1268,7 +1327,8 @@ static int build_sub_tables(gahhg_ctx* c, int64_t per, int64_t nb, int sub_cap, 
   if (use_buf(c, q.b_sub, total * 4, (void**)&q.sub)) return GAHHG_E_NOMEM;
   q.nsub = total;
   q.bytes += total * 4;
-  hipLaunchKernelGGL(k_sub_build, dim3((unsigned)nbin), dim3(64), 0, c->stream, g, (const PolyDev*)c->d_pd, B, per,
+  hipLaunchKernelGGL(k_sub_build, dim3((unsigned)nbin), dim3(64), 0, c->stream, g, (const int32_t*)q.nodes,
+                     (const PolyDev*)c->d_pd, B, per,
                      (const int32_t*)list, (const int32_t*)q.table, (const int32_t*)off, q.sub);
   rc = hip_check(c, hipGetLastError(), "sub build");
   if (rc) return rc;
@@ -1542,11 +1602,13 @@ int pack_query_layout(gahhg_ctx* c) {
       const int64_t nl0 = l0_offset(2 * Bmax);
       if (use_buf(c, q.b_l0, nl0 * 4, (void**)&l0)) return GAHHG_E_NOMEM;
       hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((nl0 + 255) / 256)), dim3(256), 0, c->stream, g,
+                         (const int32_t*)q.nodes,
                          c->pd[0].cell0, Bmax, l0);
       rc = hip_check(c, hipGetLastError(), "level-0 table");
       if (rc) return rc;
     }
     hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
+                       (const int32_t*)q.nodes,
                        (const PolyDev*)c->d_pd, np, (const int32_t*)q.tstart, (const int32_t*)l0, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
     // sub-tables for one polygon only: on the 256-polygon batch they cost
