@@ -2470,6 +2470,10 @@ int build_classify_dev(gahhg_ctx* c, bool* replayed, int32_t* attempts) {
   c->tot_r = s.R;
   c->und_snap = (int64_t)s.und_first;
   c->und_final = (int64_t)s.undefined_leaves;
+  // a polygon set built before by this context (same shape) is likely to be
+  // built again: only then is the build recorded, so that a one-off build
+  // does not pay for the capture and instantiation
+  const bool again = d.shape_n == c->n_edges && d.shape_np == np && d.used_levels > 0;
   d.used_v = s.tot_v;
   d.used_b = s.tot_b;
   d.used_c = s.tot_c;
@@ -2480,7 +2484,7 @@ int build_classify_dev(gahhg_ctx* c, bool* replayed, int32_t* attempts) {
   // record the build for the next one of this polygon set (the levels it
   // needed; a polygon set that needs more stops the replay short, and the
   // next build runs eagerly and records again)
-  if (!*replayed && c->prm.build_graph) {
+  if (!*replayed && c->prm.build_graph && again) {
     drop_build_graph(c);
     const int gl = s.gnum;
     hipGraph_t gr = nullptr;
