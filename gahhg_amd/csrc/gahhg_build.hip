@@ -423,8 +423,11 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 // dv != 0 (the device-driven build): the record levels come from
 // st->rec_levels, and the launch does nothing once the level loop has
 // stopped (mark walk) or the build stops short (dispatch walks).
+// Waves per SIMD: the walks wait on memory most of their time, so the
+// grid-array variants are held to 6 (80 VGPRs, a few spilled; the 256-polygon
+// batch's mark walks 8.25 -> 7.91 ms against 5); the record variants keep 4.
 template <int MODE, bool REC>
-__global__ void __launch_bounds__(kBlock) k_walk(GridView g, const WalkRec* __restrict__ rec, uint32_t rec_levels,
+__global__ void __launch_bounds__(kBlock, REC ? 4 : 6) k_walk(GridView g, const WalkRec* __restrict__ rec, uint32_t rec_levels,
                                                  const double* __restrict__ edge,
                                                  const int32_t* __restrict__ epoly, const PolyDev* __restrict__ pd,
                                                  int Ntot, int level, int32_t* buf, int32_t* buf_prev,

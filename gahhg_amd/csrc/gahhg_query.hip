@@ -1205,8 +1205,11 @@ __global__ void k_zero_queue(unsigned long long* queue) {
 // Thread t of the launch takes position t; with xcd > 0 each XCD takes
 // runs of xcd consecutive blocks (xcd_block).  With d_n the count is *d_n
 // (at most n, the launch bound): blocks past it leave at once.
+// 8 waves per SIMD (64 VGPRs, none spilled; 7 at the compiler's choice):
+// the walk is bound by the latency of its dependent gathers, and the extra
+// wave takes fractal_1m's pass from 8.83 to 8.70 ms.
 template <bool TRACE>
-__global__ void __launch_bounds__(kQBlock) k_query(GridView g, const PolyDev* __restrict__ pd,
+__global__ void __launch_bounds__(kQBlock, 8) k_query(GridView g, const PolyDev* __restrict__ pd,
                                                    const int32_t* __restrict__ table,
                                                    const int32_t* __restrict__ sub, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
