@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(256) k_cls_level(const PolyDev* __restrict__ p
   }
 }
 
-__global__ void __launch_bounds__(kCBlock) k_ne_dv(GridView g, const PolyDev* __restrict__ pd, const double* pv,
+__global__ void __launch_bounds__(kCBlock, 6) k_ne_dv(GridView g, const PolyDev* __restrict__ pd, const double* pv,
                                                    const int32_t* pe, const double* edge, const DevState* st, int lv) {
   if (cls_skip(st, lv)) return;
   const int G = st->cls_G[lv];

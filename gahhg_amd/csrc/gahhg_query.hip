@@ -613,7 +613,7 @@ __global__ void k_sub_list(int64_t n, const int32_t* __restrict__ has, const int
 
 // One 64-lane block per bin with a sub-table: its S^2 entries, each
 // descending from the bin's own entry.
-__global__ void __launch_bounds__(64) k_sub_build(GridView g, const int32_t* __restrict__ nodes,
+__global__ void __launch_bounds__(64, 6) k_sub_build(GridView g, const int32_t* __restrict__ nodes,
                                                   const PolyDev* __restrict__ pd, int B, int64_t per,
                                                   const int32_t* __restrict__ list,
                                                   const int32_t* __restrict__ table,
@@ -880,7 +880,7 @@ __global__ void k_index_list(const int32_t* __restrict__ flag, const int32_t* __
 }
 
 // One wave per listed leaf.
-__global__ void __launch_bounds__(256) k_index_size(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
+__global__ void __launch_bounds__(256, 6) k_index_size(const double* __restrict__ qc, const int32_t* __restrict__ hdr,
                                                     const double* __restrict__ sv, const uint8_t* __restrict__ sf,
                                                     const int32_t* __restrict__ list, int nlist,
                                                     int32_t* __restrict__ isize, uint16_t* __restrict__ code) {
