@@ -161,9 +161,21 @@ __device__ __forceinline__ CpChoice cp_select(const int32_t* __restrict__ refs, 
         lean = dot(a1, sub(center, pnv));
       }
     }
-    const int m = min(G, rn - base);
-    for (int j = 0; j < m; j++) {
-      const int src = gr.gbase + j;
+    // Only the refs at the chunk's least distance can change the choice:
+    // one at a larger distance either loses to that ref later in the chunk
+    // or (at or past the current minimum) changes nothing, and the
+    // shared-vertex rule links refs of one nearest vertex, which are at the
+    // same distance.  So the selection is replayed over those refs alone,
+    // and a chunk whose least distance exceeds the current one is skipped.
+    double cmin = i < rn ? curdist : __builtin_inf();
+    for (int o = 1; o < G; o <<= 1)
+      cmin = fmin(cmin, __hiloint2double(__shfl_xor(__double2hiint(cmin), o, G), __shfl_xor(__double2loint(cmin), o, G)));
+    if (!(cmin <= mindist)) continue;  // uniform within the group
+    unsigned long long cand = __ballot(i < rn && curdist == cmin) & gr.mask;
+    while (cand) {
+      const int src = __ffsll((long long)cand) - 1;
+      cand &= cand - 1;
+      const int j = src - gr.gbase;
       const double cd = shfl_d(curdist, src), sd = shfl_d(side, src), ln = shfl_d(lean, src);
       const int cn = __shfl(cur_nv, src, 64);
       if (nv_index != -1 && nv_index == cn) {
