@@ -54,26 +54,6 @@ __device__ __forceinline__ V3 mean_center(const GridView& g, int cid) {
   return unit(c);
 }
 
-// Crossing parity from the centre of `from` to the midpoint of border `mb`
-// over the refs of `from` (dev_isInterSeg, :174-205), giving the label on
-// the far side.
-__device__ int parity_label(const GridView& g, const double* __restrict__ edge, int from, int mb,
-                            int from_type) {
-  const V3 m = unit(mul(add(cvtx(g, g.b_v0[mb]), cvtx(g, g.b_v1[mb])), 0.5));
-  const V3 c = mean_center(g, from);
-  const V3 n1 = arc_normal(c, m);
-  int cnt = 0;
-  const int rs = g.c_rstart[from], rn = g.c_rnum[from];
-  for (int i = 0; i < rn; i++) {
-    const double* er = edge + 9 * (int64_t)g.refs[rs + i];
-    const V3 u0 = load3(er), u1 = load3(er + 3), n0 = load3(er + 6);
-    if (same_side(dot(n0, c), dot(n0, m))) continue;
-    if (same_side(dot(n1, u0), dot(n1, u1))) continue;
-    cnt++;
-  }
-  return (cnt % 2 == 0) ? from_type : 3 - from_type;
-}
-
 // Cells of one level whose polygon bottoms out there (bottom) or deeper
 // (upper); every polygon of the batch follows its own reference sequence.
 __device__ __forceinline__ int cell_phase(const GridView& g, const PolyDev* pd, int cid, int lv) {
@@ -250,8 +230,10 @@ __global__ void k_snap(const int32_t* type, int cstart, int cnum, int32_t* snap)
   if (t < cnum) snap[t] = type[cstart + t];
 }
 
-// Crossing parity as parity_label by a lane group: the lanes split the refs
-// of `from` and the count is the popcount of the group's ballots.
+// Crossing parity from the centre of `from` to the midpoint of border `mb`
+// over the refs of `from` (dev_isInterSeg, :174-205), giving the label on
+// the far side; a lane group splits the refs and the count is the popcount
+// of the group's ballots.
 __device__ int parity_label_group(const GridView& g, const double* __restrict__ edge, int from, int mb,
                                   int from_type, const Group& gr) {
   const V3 m = unit(mul(add(cvtx(g, g.b_v0[mb]), cvtx(g, g.b_v1[mb])), 0.5));
@@ -331,34 +313,56 @@ __global__ void __launch_bounds__(kCBlock) k_ie(GridView g, int cstart, int cnum
 }
 
 // setDECell: an empty unlabelled cell above the bottom takes its label from
-// the leaf reached through the first refined neighbour.
-__device__ __forceinline__ void de_upper_cell(const GridView& g, const PolyDev* __restrict__ pd, const double* edge,
-                                              int cid, int lv) {
-  if (cell_phase(g, pd, cid, lv) != 1) return;
-  if (!(g.c_rnum[cid] == 0 && g.c_type[cid] == kUndefined)) return;
-  const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
-  int pe_ = -1, pc = -1;
-  for (int i = 0; i < bn; i++) {
-    pe_ = g.b_partner[bs + i];
-    if (pe_ != -1 && g.c_subcell[g.b_parent[pe_]] != -1) {
-      pc = g.b_parent[pe_];
-      break;
+// the leaf reached through the first refined neighbour.  The lookup is one
+// lane per cell; the crossing parity over the leaf's refs (for leaves with
+// refs) is counted by the whole wave, one pending cell after another, its
+// 64 lanes splitting the refs (most cells of a level need none, and a leaf
+// holds tens of refs that one lane would fetch in turn).
+__device__ __forceinline__ void de_upper_wave(const GridView& g, const PolyDev* __restrict__ pd, const double* edge,
+                                              int cstart, int64_t t, int64_t cnum, int lv) {
+  int cid = -1, target = -1, mb = -1, ttype = 0;
+  bool need = false;
+  if (t < cnum) {
+    cid = cstart + (int)t;
+    if (cell_phase(g, pd, cid, lv) == 1 && g.c_rnum[cid] == 0 && g.c_type[cid] == kUndefined) {
+      const int bs = g.c_bstart[cid], bn = g.c_bnum[cid];
+      int pe_ = -1, pc = -1;
+      for (int i = 0; i < bn; i++) {
+        pe_ = g.b_partner[bs + i];
+        if (pe_ != -1 && g.c_subcell[g.b_parent[pe_]] != -1) {
+          pc = g.b_parent[pe_];
+          break;
+        }
+      }
+      if (pc != -1 && (target = g.b_subcell[pe_]) >= 0) {
+        while (g.c_subcell[target] != -1) target = g.c_subcell[target];
+        ttype = g.c_type[target];
+        if (g.c_rnum[target] == 0) {
+          g.c_type[cid] = ttype;
+        } else {
+          need = true;
+          mb = pe_;
+        }
+      }
     }
   }
-  if (pc == -1) return;
-  int target = g.b_subcell[pe_];
-  if (target < 0) return;
-  while (g.c_subcell[target] != -1) target = g.c_subcell[target];
-  if (g.c_rnum[target] == 0)
-    g.c_type[cid] = g.c_type[target];
-  else
-    g.c_type[cid] = parity_label(g, edge, target, pe_, g.c_type[target]);
+  const Group gr = group_of(64);
+  unsigned long long pend = __ballot(need);
+  while (pend) {
+    const int src = __ffsll((long long)pend) - 1;
+    pend &= pend - 1;
+    const int lab = parity_label_group(g, edge, __shfl(target, src, 64), __shfl(mb, src, 64),
+                                       __shfl(ttype, src, 64), gr);
+    if (gr.lane == src) g.c_type[cid] = lab;
+  }
 }
 
+// Every lane of a wave runs de_upper_wave the same number of times (the
+// loop bound is the wave's first cell).
 __global__ void __launch_bounds__(kCBlock) k_de_upper(GridView g, const PolyDev* __restrict__ pd,
                                                       const double* edge, int cstart, int cnum, int lv) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < cnum) de_upper_cell(g, pd, edge, cstart + t, lv);
+  const int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  de_upper_wave(g, pd, edge, cstart, w0 + (threadIdx.x & 63), cnum, lv);
 }
 
 // setTopECell on one polygon's 42 level-0 cells (block = polygon),
@@ -470,7 +474,9 @@ __global__ void __launch_bounds__(kCBlock) k_de_upper_dv(GridView g, const PolyD
                                                          const double* edge, const DevState* st, int lv) {
   if (cls_skip(st, lv)) return;
   const LevelDesc L = st->lv[lv];
-  FOR_LEVEL(L, t) de_upper_cell(g, pd, edge, L.cstart + t, lv);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); w0 < L.cnum; w0 += stride)
+    de_upper_wave(g, pd, edge, L.cstart, w0 + (threadIdx.x & 63), L.cnum, lv);
 }
 
 // setIECell's first round over the level.
