@@ -27,7 +27,9 @@
 
 namespace gahhg {
 
-namespace {
+// A named namespace (not an anonymous one) so that profilers show the
+// kernels' names.
+namespace learn_kernels {
 
 // The reference's inside test: every border plane n.p >= -eps.
 __device__ __forceinline__ bool inside(const GridView& g, V3 q, int cid) {
@@ -384,7 +386,8 @@ int ensure_learn(gahhg_ctx* c) {
 
 unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-}  // namespace
+}  // namespace learn_kernels
+using namespace learn_kernels;
 
 // Edge statistics and expected points per (polygon, top-level cell); needs
 // the level-0 grid and the polygon records on the device.

@@ -30,7 +30,9 @@
 
 namespace gahhg {
 
-namespace {
+// A named namespace (not an anonymous one) so that profilers show the
+// kernels' names.
+namespace mlp_kernels {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -242,7 +244,8 @@ __global__ void k_mlp_adam(float* __restrict__ master, const float* __restrict__
   W[k] = f2bf(w);
 }
 
-}  // namespace
+}  // namespace mlp_kernels
+using namespace mlp_kernels;
 
 int mlp_forward(gahhg_ctx* c, const float* d_x, int64_t rows, const uint16_t* d_w, float* d_y) {
   if (rows <= 0) return GAHHG_OK;

@@ -14,7 +14,9 @@
 
 namespace gahhg {
 
-namespace {
+// A named namespace (not an anonymous one) so that profilers show the
+// kernels' names.
+namespace trace_kernels {
 
 __device__ __forceinline__ int level_of_cell(const LevelTab& lt, int cid) {
   int l = 0;
@@ -46,7 +48,8 @@ __global__ void k_cell_trace(GridView g, const PolyDev* __restrict__ pd, LevelTa
   wave_add<int32_t>(out, key >= 0 ? key + 3 : -1, refs);
 }
 
-}  // namespace
+}  // namespace trace_kernels
+using namespace trace_kernels;
 
 // Edge aggregates per top-level cell of the edge's first endpoint (first
 // matching level-0 cell): out[(p * 42 + r) * 3 + k] = {edges, sum of arc
