@@ -2512,4 +2512,10 @@ int build_classify_dev(gahhg_ctx* c, bool* replayed, int32_t* attempts) {
   return GAHHG_OK;
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_build() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_zero));
+}
+
 }  // namespace gahhg

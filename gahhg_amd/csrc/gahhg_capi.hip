@@ -312,6 +312,15 @@ int gahhg_create(gahhg_ctx** out, const gahhg_params* p) {
   }
   for (int i = 0; i < 4; i++) (void)hipEventCreate(&c->ev[i]);
   noalloc_init(c->device);
+  load_code_build();
+  load_code_classify();
+  load_code_query();
+  load_code_scan();
+  load_code_sort();
+  load_code_learn();
+  load_code_mlp();
+  load_code_load();
+  load_code_trace();
   if (hipMalloc((void**)&c->d_st, sizeof(DevState)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, 64, hipHostMallocDefault) != hipSuccess ||

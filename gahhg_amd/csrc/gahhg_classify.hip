@@ -686,4 +686,10 @@ int classify_hierarchy(gahhg_ctx* c) {
   return GAHHG_OK;
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_classify() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_snap));
+}
+
 }  // namespace gahhg

@@ -719,4 +719,17 @@ void free_buf(DevBuf& b);
 int fill_i32(gahhg_ctx* c, int32_t* p, int64_t n, int32_t v);
 int copy_i32(gahhg_ctx* c, int32_t* dst, const int32_t* src, int64_t n);
 
+// Each kernel file's code object, loaded onto the current device when a
+// context is created (HIP loads a code object at its first use, which would
+// otherwise fall into a context's first build).
+void load_code_build();
+void load_code_classify();
+void load_code_query();
+void load_code_scan();
+void load_code_sort();
+void load_code_learn();
+void load_code_mlp();
+void load_code_load();
+void load_code_trace();
+
 }  // namespace gahhg

@@ -596,4 +596,10 @@ void learn_free(gahhg_ctx* c) {
   L = LearnState();
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_learn() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&learn_kernels::k_hint_hist));
+}
+
 }  // namespace gahhg

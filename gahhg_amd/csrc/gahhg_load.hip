@@ -652,4 +652,10 @@ void free_text_load(gahhg_ctx* c) {
   L = TextLoad();
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_load() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_nl_count));
+}
+
 }  // namespace gahhg

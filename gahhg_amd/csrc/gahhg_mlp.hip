@@ -284,4 +284,10 @@ int mlp_adam(gahhg_ctx* c, float* d_master, const float* d_grad, float* d_m, flo
   return hip_check(c, hipGetLastError(), "mlp adam");
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_mlp() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&mlp_kernels::k_mlp_forward));
+}
+
 }  // namespace gahhg

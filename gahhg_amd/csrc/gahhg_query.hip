@@ -2133,4 +2133,10 @@ int run_kat(gahhg_ctx* c, int op, const double* d_in, int64_t n, double* d_out) 
   return hip_check(c, hipGetLastError(), "kat kernel");
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_query() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_pack_nodes));
+}
+
 }  // namespace gahhg

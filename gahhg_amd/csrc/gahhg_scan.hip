@@ -259,4 +259,10 @@ int scan_reserve(gahhg_ctx* c, int64_t n_bound) {
   return GAHHG_OK;
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_scan() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_tile_reduce));
+}
+
 }  // namespace gahhg

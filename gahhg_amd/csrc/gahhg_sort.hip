@@ -182,4 +182,10 @@ int radix_sort_pairs(gahhg_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uin
   return GAHHG_OK;
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_sort() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rs_hist));
+}
+
 }  // namespace gahhg

@@ -129,4 +129,10 @@ int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges) {
   return rc;
 }
 
+// Loads this file's code object onto the current device (context set-up).
+void load_code_trace() {
+  hipFuncAttributes a;
+  (void)hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&trace_kernels::k_cell_trace));
+}
+
 }  // namespace gahhg
