@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -39,27 +40,32 @@ int main(int argc, char** argv) {
     fprintf(stderr, "Error. No usable GPU.\n");
     return -1;
   }
+  // polygon and query text are parsed on the GPU (values equal the host
+  // loaders' on every line; the query points stay in device memory)
   printf("Load spherical polygon from %s.\n", poly.c_str());
-  if (gahhg_load_polygon_txt(ctx, poly.c_str()) != GAHHG_OK) {
+  if (gahhg_load_polygon_txt_device(ctx, poly.c_str(), nullptr) != GAHHG_OK) {
     printf("Error. %s\n", gahhg_last_error(ctx));
     gahhg_destroy(ctx);
     return -1;
   }
   printf("Load test points from a CSV file: %s.\n", tp.c_str());
+  const double* d_xyz = nullptr;
   int64_t n = 0;
-  if (gahhg_read_query_csv(tp.c_str(), nullptr, nullptr, 0, &n) != GAHHG_OK) {
+  if (gahhg_load_query_csv_device(ctx, tp.c_str(), &d_xyz, &n, nullptr) != GAHHG_OK) {
     printf("Error. Can not read %s\n", tp.c_str());
     gahhg_destroy(ctx);
     return -1;
   }
-  std::vector<double> lon((size_t)n), lat((size_t)n);
   std::vector<int32_t> labels((size_t)n);
-  gahhg_read_query_csv(tp.c_str(), lon.data(), lat.data(), n, &n);
-  if (gahhg_build(ctx) != GAHHG_OK || gahhg_query_lonlat(ctx, lon.data(), lat.data(), n, labels.data()) != GAHHG_OK) {
+  void* d_labels = nullptr;
+  if (gahhg_build(ctx) != GAHHG_OK || gahhg_device_alloc(ctx, std::max<int64_t>(n, 1) * 4, &d_labels) != GAHHG_OK ||
+      gahhg_query_xyz_device(ctx, d_xyz, n, (int32_t*)d_labels) != GAHHG_OK ||
+      gahhg_memcpy_d2h(ctx, labels.data(), d_labels, n * 4) != GAHHG_OK) {
     printf("Error. %s\n", gahhg_last_error(ctx));
     gahhg_destroy(ctx);
     return -1;
   }
+  gahhg_device_free(ctx, d_labels);
   auto base = [](const std::string& s) {
     size_t k = s.find_last_of("/\\");
     return k == std::string::npos ? s : s.substr(k + 1);
