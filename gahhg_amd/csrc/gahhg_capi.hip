@@ -825,7 +825,7 @@ int gahhg_load_grid(gahhg_ctx* c, const char* path) {
   c->tot_r = cnt[3];
   c->und_snap = und[0];
   c->und_final = und[1];
-  for (PolyDev& P : c->pd) P.tbl = -1, P.tbins = 0;
+  for (PolyDev& P : c->pd) P.tbl = -1, P.tbins = 0, P.trel = 0;
   memset(c->h_st, 0, sizeof(DevState));
   RCK(hip_check(c, hipMemcpyAsync(c->d_st, c->h_st, sizeof(DevState), hipMemcpyHostToDevice, c->stream), "state"));
   RCK(hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,

@@ -52,6 +52,7 @@ struct PolyDev {
   int32_t nvb, nbb0, ncb;   // next level: first vertex, border, cell
   int32_t tbl;          // first entry of this polygon's start-cell table, -1 none
   int32_t tbins;        // bins per cube-face side of that table (0 = none)
+  int32_t trel;         // 1: the table is the shared level-0 lookup (entries + cell0)
   int32_t depth;        // fixed depth for this polygon (0 = density rule)
   int32_t tsub;         // sub-table side S of this polygon's start table (1 = none)
   int32_t qg;           // lanes per edge-test group of this polygon's queries

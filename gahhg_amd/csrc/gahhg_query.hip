@@ -516,6 +516,10 @@ __device__ int32_t cap_descend(const GridView& g, const int32_t* __restrict__ no
   return cur;
 }
 
+// Bins per cube-face side of the shared level-0 lookup of a batch's
+// table-less polygons (24576 entries, 96 KB).
+constexpr int kL0Bins = 64;
+
 // First entry of the level-0 table of B bins per side (B = 16 * 2^k): the
 // tables of 16, 32, ... bins follow one another.
 __host__ __device__ __forceinline__ int64_t l0_offset(int B) {
@@ -524,14 +528,17 @@ __host__ __device__ __forceinline__ int64_t l0_offset(int B) {
   return off;
 }
 
-// Level-0 part of every bin's entry for 16, 32, ..., Bmax bins per side.
-// Level-0 geometry is the same for every polygon (polygon 0's cells are
-// used), so a batch computes it once per resolution, not once per polygon.
-__global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, int cell0, int Bmax,
+// Level-0 part of every bin's entry for Bmin, 2 Bmin, ..., Bmax bins per
+// side (powers of two >= 16), written from out[0] on: a relative level-0
+// cell (0..41) that every point of the bin reaches first, or -1.  Level-0
+// geometry is the same for every polygon (polygon 0's cells are used), so a
+// batch computes it once per resolution, not once per polygon.
+__global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, int cell0, int Bmin, int Bmax,
                                int32_t* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = t0 + l0_offset(Bmin);
   if (t >= l0_offset(2 * Bmax)) return;
-  int B = 16;
+  int B = Bmin;
   while (t >= l0_offset(2 * B)) B *= 2;
   const int64_t u = t - l0_offset(B);
   const int f = (int)(u / ((int64_t)B * B));
@@ -540,7 +547,7 @@ __global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, in
   V3 cen;
   double rho;
   bin_cap(f, B, i, j, 1, 0, 0, cen, rho);
-  out[t] = cap_level0(g, nodes, cell0, cen, cos(rho), sin(rho));
+  out[t0] = cap_level0(g, nodes, cell0, cen, cos(rho), sin(rho));
 }
 
 // Entry tt of all polygons' tables: the polygon owning it is found by a
@@ -1124,6 +1131,7 @@ __device__ __forceinline__ void query_lanes(int64_t t, int64_t n, const GridView
         labels[q] = -e - 2;
         done = true;
       }
+      if (P.trel && e >= 0) e += P.cell0;  // the shared level-0 lookup
       cur = e;
       if (TRACE) start_level = e <= -2 ? -2 : (e >= 0 ? level_of(lt, e) : -1);
     }
@@ -1548,6 +1556,7 @@ int pack_query_layout(gahhg_ctx* c) {
     // ~500 edges: 30 ms of table build at 128 bins)
     const int64_t tb = c->prm.query_table_budget;
     std::vector<int32_t> ts((size_t)np + 1, 0);
+    int nrel = 0;
     for (int p = 0; p < np; p++) {
       int Bp = q.bins;
       if (np > 1 && cand[p] >= 0) {
@@ -1558,16 +1567,26 @@ int pack_query_layout(gahhg_ctx* c) {
         Bp = std::min(Bp, q.bins);
       }
       const int64_t next = (int64_t)ts[p] + 6LL * Bp * Bp;
-      if (next > INT32_MAX) return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
+      if (next > INT32_MAX - 6LL * kL0Bins * kL0Bins)
+        return set_err(c, GAHHG_E_LIMIT, "start tables exceed int32 indexing");
       c->pd[p].tbl = ts[p];
       c->pd[p].tbins = Bp;
       c->pd[p].tsub = 1;
+      c->pd[p].trel = 0;
+      nrel += Bp == 0;
       ts[p + 1] = (int32_t)next;
     }
     const int64_t nb = ts[np];
+    // polygons of a batch without a table of their own share one level-0
+    // lookup appended to the tables (kL0Bins per face side, entries relative
+    // to the polygon's first level-0 cell), in place of scanning up to 42
+    // level-0 cells per point
+    const int64_t nl0r = nb > 0 && nrel > 0 ? 6LL * kL0Bins * kL0Bins : 0;
+    for (int p = 0; p < np && nl0r > 0; p++)
+      if (c->pd[p].tbins == 0) c->pd[p].tbl = (int32_t)nb, c->pd[p].tbins = kL0Bins, c->pd[p].trel = 1;
     if (nb == 0) {  // no polygon gets a table: the query starts at level 0
       q.bins = 0;
-      for (int p = 0; p < np; p++) c->pd[p].tbl = -1, c->pd[p].tbins = 0, c->pd[p].tsub = 1;
+      for (int p = 0; p < np; p++) c->pd[p].tbl = -1, c->pd[p].tbins = 0, c->pd[p].tsub = 1, c->pd[p].trel = 0;
       rc = hip_check(c, hipMemcpyAsync(c->d_pd, c->pd.data(), c->pd.size() * sizeof(PolyDev), hipMemcpyHostToDevice,
                                        c->stream),
                      "polygon upload");
@@ -1576,10 +1595,10 @@ int pack_query_layout(gahhg_ctx* c) {
         for (int p = 0; p < np; p++) c->table_used[2 * p] = 0, c->table_used[2 * p + 1] = 1;
       return rc;
     }
-    if (use_buf(c, q.b_table, nb * 4, (void**)&q.table) ||
+    if (use_buf(c, q.b_table, (nb + nl0r) * 4, (void**)&q.table) ||
         use_buf(c, q.b_tstart, ((int64_t)np + 1) * 4, (void**)&q.tstart))
       return GAHHG_E_NOMEM;
-    q.bytes += nb * 4;
+    q.bytes += (nb + nl0r) * 4;
     rc = hip_check(c, hipMemcpyAsync(q.tstart, ts.data(), ((size_t)np + 1) * 4, hipMemcpyHostToDevice, c->stream),
                    "table starts");
     if (!rc)
@@ -1596,7 +1615,7 @@ int pack_query_layout(gahhg_ctx* c) {
     bool pow2 = np > 1;
     for (int p = 0; p < np && pow2; p++) {
       const int B = c->pd[p].tbins;
-      if (B == 0) continue;  // no table for this polygon
+      if (B == 0 || c->pd[p].trel) continue;  // no table of its own
       pow2 = B >= 16 && B <= 4096 && (B & (B - 1)) == 0;
       Bmax = std::max(Bmax, B);
     }
@@ -1605,8 +1624,7 @@ int pack_query_layout(gahhg_ctx* c) {
       const int64_t nl0 = l0_offset(2 * Bmax);
       if (use_buf(c, q.b_l0, nl0 * 4, (void**)&l0)) return GAHHG_E_NOMEM;
       hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((nl0 + 255) / 256)), dim3(256), 0, c->stream, g,
-                         (const int32_t*)q.nodes,
-                         c->pd[0].cell0, Bmax, l0);
+                         (const int32_t*)q.nodes, c->pd[0].cell0, 16, Bmax, l0);
       rc = hip_check(c, hipGetLastError(), "level-0 table");
       if (rc) return rc;
     }
@@ -1614,14 +1632,20 @@ int pack_query_layout(gahhg_ctx* c) {
                        (const int32_t*)q.nodes,
                        (const PolyDev*)c->d_pd, np, (const int32_t*)q.tstart, (const int32_t*)l0, q.table);
     rc = hip_check(c, hipGetLastError(), "table build");
+    if (!rc && nl0r > 0) {
+      hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((nl0r + 255) / 256)), dim3(256), 0, c->stream, g,
+                         (const int32_t*)q.nodes, c->pd[0].cell0, kL0Bins, kL0Bins, q.table + nb);
+      rc = hip_check(c, hipGetLastError(), "level-0 lookup");
+    }
     // sub-tables for one polygon only: on the 256-polygon batch they cost
     // 11.5 ms of build for 0.9 ms less per 64M points
     if (!rc && sub_cap > 1 && np == 1 && nb > 0) rc = build_sub_tables(c, per, nb, sub_cap, sub_budget);
   }
   if (!rc && q.bins == 0)
-    for (int p = 0; p < np; p++) c->pd[p].tbins = 0, c->pd[p].tsub = 1;  // no table this time
+    for (int p = 0; p < np; p++) c->pd[p].tbins = 0, c->pd[p].tsub = 1, c->pd[p].trel = 0;  // no table this time
   if (!rc)
-    for (int p = 0; p < np; p++) c->table_used[2 * p] = c->pd[p].tbins, c->table_used[2 * p + 1] = c->pd[p].tsub;
+    for (int p = 0; p < np; p++)
+      c->table_used[2 * p] = c->pd[p].trel ? 0 : c->pd[p].tbins, c->table_used[2 * p + 1] = c->pd[p].tsub;
   return rc;
 }
 // Spreads the low 16 bits of v to the even bit positions.
@@ -1712,7 +1736,8 @@ __global__ void __launch_bounds__(256) k_query_table(const PolyDev* __restrict__
   for (int j = 0; j < kTableSub; j++) {
     const PolyDev& P = pd[poly[j]];
     fx[j] = fy[j] = 0.0;
-    const int bin = P.tbins > 0 ? bin_of(p[j], P.tbins, fx[j], fy[j]) : -1;
+    // the shared level-0 lookup never answers a point: left to the walk
+    const int bin = P.tbins > 0 && !P.trel ? bin_of(p[j], P.tbins, fx[j], fy[j]) : -1;
     e[j] = (ok[j] && bin >= 0) ? table[(int64_t)P.tbl + bin] : -1;
   }
 #pragma unroll
