@@ -5,7 +5,9 @@ without the model; the table the build reports is the one asked for.
 L1  one polygon, each of the 8 candidates fixed: labels equal the oracle's on
     uniform and near-edge points, table layout as requested;
 L2  a batch with per-polygon candidates 0..4: labels equal the per-polygon
-    oracle, per-polygon bins as requested;
+    oracle, per-polygon bins as requested; the polygons without a table
+    (candidate 0) start most points from the shared level-0 lookup (a
+    level-0 cell) instead of the level-0 scan, with the same labels;
 L3  the table model (trained one when present, else random weights) with a
     query hint: labels equal the oracle's; the features are finite.
 """
@@ -67,6 +69,15 @@ def test_batch_candidates_same_labels():
         assert [int(b) for b in lay[:, 0]] == [gahhg_amd.TABLE_BINS[c] for c in cand]
         say("L2 batch of 10 with candidates 0..4")
         got = g.query_batch_xyz(xyz, pid)
+        say("L2 traced: start levels of the table-less polygons")
+        tlab, tr = g.trace_xyz(xyz, pid)
+        assert (tlab == got).all()
+        none = cand[pid] == 0
+        assert none.any()
+        start = tr[none, 3]
+        # level 0 = the shared lookup's cell, -1 = the level-0 scan; a bin
+        # straddling level-0 cells falls back to the scan
+        assert np.count_nonzero(start == 0) >= 0.8 * np.count_nonzero(start >= -1), np.unique(start)
     for k, it in enumerate(items):
         sel = pid == k
         ref = ol.Oracle().set_lonlat(*it).build().query_xyz(np.ascontiguousarray(xyz[sel]))
