@@ -86,18 +86,25 @@ __device__ __forceinline__ int descend(const GridView& g, V3 p, int cur, int& te
 template <bool TRACE>
 __device__ __forceinline__ bool node_inside(const int32_t* __restrict__ nodes, int x, V3 p, int& sub, int& meta,
                                             int& tests) {
+  // all six plane slots are loaded at once (one round trip instead of one
+  // per plane); the slots past bn hold zero normals, which pass (0 >= -eps)
   const int32_t* r = nodes + (int64_t)x * kNodeInts;
   const int2 h = *reinterpret_cast<const int2*>(r);
-  const int bn = h.y & 0xff;
   const double* n = reinterpret_cast<const double*>(r + 10);
-  for (int i = 0; i < bn; i++) {
-    const double v = n[3 * i] * p.x + n[3 * i + 1] * p.y + n[3 * i + 2] * p.z;
-    if (v < -kEps) {
-      if (TRACE) tests += i + 1;
-      return false;
-    }
+  double w[18];
+#pragma unroll
+  for (int i = 0; i < 18; i++) w[i] = n[i];
+  int fail = -1;
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    const double v = w[3 * i] * p.x + w[3 * i + 1] * p.y + w[3 * i + 2] * p.z;
+    if (v < -kEps) fail = i;
   }
-  if (TRACE) tests += bn;
+  if (fail >= 0) {
+    if (TRACE) tests += fail + 1;
+    return false;
+  }
+  if (TRACE) tests += h.y & 0xff;
   sub = h.x;
   meta = h.y;
   return true;
@@ -1213,11 +1220,12 @@ __global__ void k_zero_queue(unsigned long long* queue) {
 // Thread t of the launch takes position t; with xcd > 0 each XCD takes
 // runs of xcd consecutive blocks (xcd_block).  With d_n the count is *d_n
 // (at most n, the launch bound): blocks past it leave at once.
-// 8 waves per SIMD (64 VGPRs, none spilled; 7 at the compiler's choice):
-// the walk is bound by the latency of its dependent gathers, and the extra
-// wave takes fractal_1m's pass from 8.83 to 8.70 ms.
+// 7 waves per SIMD (72 VGPRs): the walk is bound by the latency of its
+// dependent gathers; the registers that holding a node record's six planes
+// at once needs are worth more than an eighth wave (8 waves: 2 VGPRs
+// spilled, fractal_1m's pass 8.52-8.61 ms against 8.36-8.45 ms).
 template <bool TRACE>
-__global__ void __launch_bounds__(kQBlock, 8) k_query(GridView g, const PolyDev* __restrict__ pd,
+__global__ void __launch_bounds__(kQBlock, 7) k_query(GridView g, const PolyDev* __restrict__ pd,
                                                    const int32_t* __restrict__ table,
                                                    const int32_t* __restrict__ sub, int B,
                                                    const double* __restrict__ qc, const int32_t* __restrict__ hdr,
