@@ -9,9 +9,10 @@
 //
 // Per 8-bit digit (the last one cut at end_bit), three steps over tiles of
 // 4096 pairs (256 threads x 16):
-//   1. k_rs_hist: per tile, the count of each digit (wave-level digit match
-//      by 8 ballots, one LDS atomic per distinct digit per 64 pairs),
-//      written digit-major: hist[d * tiles + tile];
+//   1. k_rs_hist: per tile, the count of each digit (one LDS atomic per
+//      pair: the digits of a tile are spread, so a wave-level digit match
+//      first costs more than the atomics' conflicts), written digit-major:
+//      hist[d * tiles + tile];
 //   2. the device scan (gahhg_scan.hip) of that table, in place, over the
 //      256 * tiles entries in use: the first output slot of (digit, tile);
 //   3. k_rs_scatter: each tile ranks its pairs stably (wave order, then
@@ -59,9 +60,7 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_hist(const uint32_t* __restri
   for (int r = 0; r < kRsItems; r++) {
     const int64_t i = base + r * 64 + lane;
     const bool ok = i < m;
-    const uint32_t d = ok ? (key[i] >> shift) & mask : 0u;
-    const unsigned long long mm = digit_match(d, __ballot(ok));
-    if (ok && (mm & ((1ull << lane) - 1ull)) == 0) atomicAdd(&cnt[d], __popcll(mm));
+    if (ok) atomicAdd(&cnt[(key[i] >> shift) & mask], 1);
   }
   __syncthreads();
   hist[(int64_t)threadIdx.x * nt + blockIdx.x] = cnt[threadIdx.x];
