@@ -63,6 +63,7 @@ void reset_grid(gahhg_ctx* c) {
   kept.b_qc = q.b_qc, kept.b_hdr = q.b_hdr, kept.b_ib = q.b_ib, kept.b_idx = q.b_idx;
   kept.b_sv = q.b_sv, kept.b_sn = q.b_sn, kept.b_sf = q.b_sf, kept.b_iscr = q.b_iscr;
   kept.b_tstart = q.b_tstart, kept.b_l0 = q.b_l0, kept.b_nodes = q.b_nodes;
+  kept.l0_have = q.l0_have;
   q = kept;
   c->qepoch++;
   c->built = false;

@@ -307,6 +307,10 @@ struct QueryLayout {
   // of the layout is not built)
   DevBuf b_table, b_sub, b_tmp, b_pcnt, b_qc, b_hdr, b_ib, b_idx, b_sv, b_sn, b_sf, b_iscr, b_tstart, b_l0, b_nodes;
   int32_t* tstart = nullptr;  // per polygon: first table entry (np + 1 values, ascending)
+  // resolutions whose level-0 table part b_l0 holds (bit k: 16 << k bins per
+  // side); the level-0 grid is the same for every polygon and build, so a
+  // part is computed once and kept until b_l0 is reallocated or freed
+  uint32_t l0_have = 0;
   // per cell, kNodeInts int32 (192 B): the descent's view of the cell in one
   // record -- [0] centre child (-1 = leaf), [1] borders | border children
   // << 8 | type << 16 | (refs != 0) << 24, [2..7] border children in border

@@ -538,8 +538,8 @@ __host__ __device__ __forceinline__ int64_t l0_offset(int B) {
 // Level-0 part of every bin's entry for Bmin, 2 Bmin, ..., Bmax bins per
 // side (powers of two >= 16), written from out[0] on: a relative level-0
 // cell (0..41) that every point of the bin reaches first, or -1.  Level-0
-// geometry is the same for every polygon (polygon 0's cells are used), so a
-// batch computes it once per resolution, not once per polygon.
+// geometry is the same for every polygon and build (polygon 0's cells are
+// used), so it is computed once per resolution and context.
 __global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, int cell0, int Bmin, int Bmax,
                                int32_t* __restrict__ out) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -559,7 +559,8 @@ __global__ void k_level0_table(GridView g, const int32_t* __restrict__ nodes, in
 
 // Entry tt of all polygons' tables: the polygon owning it is found by a
 // binary search over the tables' first entries (tstart, np + 1 values).
-// With l0 (a batch), the level-0 part comes from k_level0_table.
+// With l0 (power-of-two bins per side), the level-0 part comes from
+// k_level0_table.
 __global__ void k_build_table(GridView g, const int32_t* __restrict__ nodes, const PolyDev* __restrict__ pd,
                               int npoly, const int32_t* __restrict__ tstart, const int32_t* __restrict__ l0,
                               int32_t* __restrict__ table_all) {
@@ -1616,25 +1617,35 @@ int pack_query_layout(gahhg_ctx* c) {
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "table starts");  // ts leaves scope
     if (rc) return rc;
     const int64_t per = 6LL * q.bins * q.bins;
-    // a batch whose tables have 16, 32, ... bins per side: level-0 parts
-    // computed once per resolution and shared by the polygons
+    // tables of 16, 32, ... bins per side: the level-0 part of their entries
+    // comes from b_l0, computed once per resolution and shared by the
+    // polygons and by later builds (level-0 geometry never changes)
     int32_t* l0 = nullptr;
     int Bmax = 0;
-    bool pow2 = np > 1;
+    bool pow2 = true;
+    uint32_t need = 0;
     for (int p = 0; p < np && pow2; p++) {
       const int B = c->pd[p].tbins;
       if (B == 0 || c->pd[p].trel) continue;  // no table of its own
       pow2 = B >= 16 && B <= 4096 && (B & (B - 1)) == 0;
+      if (pow2) need |= 1u << (31 - __builtin_clz((unsigned)(B / 16)));
       Bmax = std::max(Bmax, B);
     }
     pow2 = pow2 && Bmax > 0;
     if (pow2) {
       const int64_t nl0 = l0_offset(2 * Bmax);
+      if (q.b_l0.p == nullptr || std::max<int64_t>(nl0 * 4, 256) > q.b_l0.cap) q.l0_have = 0;  // reallocated
       if (use_buf(c, q.b_l0, nl0 * 4, (void**)&l0)) return GAHHG_E_NOMEM;
-      hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((nl0 + 255) / 256)), dim3(256), 0, c->stream, g,
-                         (const int32_t*)q.nodes, c->pd[0].cell0, 16, Bmax, l0);
-      rc = hip_check(c, hipGetLastError(), "level-0 table");
-      if (rc) return rc;
+      for (int k = 0; (16 << k) <= Bmax; k++) {
+        if (!((need >> k) & 1u) || ((q.l0_have >> k) & 1u)) continue;
+        const int B = 16 << k;
+        const int64_t n = 6LL * B * B;
+        hipLaunchKernelGGL(k_level0_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g,
+                           (const int32_t*)q.nodes, c->pd[0].cell0, B, B, l0 + l0_offset(B));
+        rc = hip_check(c, hipGetLastError(), "level-0 table");
+        if (rc) return rc;
+        q.l0_have |= 1u << k;
+      }
     }
     hipLaunchKernelGGL(k_build_table, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g,
                        (const int32_t*)q.nodes,

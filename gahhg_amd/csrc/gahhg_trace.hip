@@ -55,41 +55,74 @@ using namespace trace_kernels;
 // matching level-0 cell): out[(p * 42 + r) * 3 + k] = {edges, sum of arc
 // lengths, sum of turning angles to the next edge of the ring}, the sums in
 // fixed point (2^-40 rad) so that they do not depend on summation order.
+// Each wave takes kTraceEdges consecutive edges (kTraceEdges / 64 per
+// lane, strided by 64 so that the loads stay coalesced) and keeps a running
+// sum per lane while its edges stay in one (polygon, top-level cell); the
+// wave adds its sums to the totals only when some lane's key changes and at
+// its end.  Consecutive edges of a ring mostly share a key, so the totals'
+// few addresses take one atomic per counter per kTraceEdges edges instead
+// of one per 64 (those atomics, serialised on a handful of addresses, were
+// the kernel's time).  The sums are integers, so the order does not matter.
+constexpr int kTraceEdges = 1024;
 __global__ void k_edge_trace(GridView g, const PolyDev* __restrict__ pd, const double* __restrict__ edge,
                              const int32_t* __restrict__ epoly, const int32_t* __restrict__ pe, int64_t Nt,
                              unsigned long long* __restrict__ out) {
   // every lane reaches the wave-aggregated adds (invalid lanes add to key -1)
   __shared__ Level0Lds l0;  // level-0 geometry is the same for every polygon
   level0_load(g, pd[0].cell0, &l0);
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int key = -1;
-  unsigned long long len_fx = 0, turn_fx = 0;
-  if (e < Nt) {
-    const int p = epoly[e];
-    const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
-    const int root = level0_cell(&l0, p0);
-    if (root >= 0) {
-      const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
-      double turn = 0.0;
-      if (e + 1 < Nt && epoly[e + 1] == p && pe[2 * e + 1] == pe[2 * (e + 1)]) {
-        const V3 n2 = load3(edge + 9 * (e + 1) + 6);
-        turn = acos(fmin(1.0, fmax(-1.0, dot(ne, n2))));
+  const int lane = threadIdx.x & 63;
+  const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kTraceEdges;
+  int acc = -1;
+  unsigned long long n_acc = 0, len_acc = 0, turn_acc = 0;
+  for (int j = 0; j < kTraceEdges; j += 64) {
+    const int64_t e = base + j + lane;
+    int key = -1;
+    unsigned long long len_fx = 0, turn_fx = 0;
+    if (e < Nt) {
+      const int p = epoly[e];
+      const V3 p0 = load3(edge + 9 * e), p1 = load3(edge + 9 * e + 3), ne = load3(edge + 9 * e + 6);
+      const int root = level0_cell(&l0, p0);
+      if (root >= 0) {
+        const double len = acos(fmin(1.0, fmax(-1.0, dot(p0, p1))));
+        double turn = 0.0;
+        if (e + 1 < Nt && epoly[e + 1] == p && pe[2 * e + 1] == pe[2 * (e + 1)]) {
+          const V3 n2 = load3(edge + 9 * (e + 1) + 6);
+          turn = acos(fmin(1.0, fmax(-1.0, dot(ne, n2))));
+        }
+        key = (p * kGrid0Cells + root) * 3;
+        len_fx = (unsigned long long)(len * kFix);
+        turn_fx = (unsigned long long)(turn * kFix);
       }
-      key = (p * kGrid0Cells + root) * 3;
-      len_fx = (unsigned long long)(len * kFix);
-      turn_fx = (unsigned long long)(turn * kFix);
+    }
+    if (__ballot(key >= 0 && acc >= 0 && key != acc)) {  // wave-uniform
+      wave_add<unsigned long long>(out, acc, n_acc);
+      wave_add<unsigned long long>(out, acc >= 0 ? acc + 1 : -1, len_acc);
+      wave_add<unsigned long long>(out, acc >= 0 ? acc + 2 : -1, turn_acc);
+      acc = -1;
+      n_acc = len_acc = turn_acc = 0;
+    }
+    if (key >= 0) {
+      acc = key;
+      n_acc += 1;
+      len_acc += len_fx;
+      turn_acc += turn_fx;
     }
   }
-  wave_add<unsigned long long>(out, key, key >= 0 ? 1ull : 0ull);
-  wave_add<unsigned long long>(out, key >= 0 ? key + 1 : -1, len_fx);
-  wave_add<unsigned long long>(out, key >= 0 ? key + 2 : -1, turn_fx);
+  wave_add<unsigned long long>(out, acc, n_acc);
+  wave_add<unsigned long long>(out, acc >= 0 ? acc + 1 : -1, len_acc);
+  wave_add<unsigned long long>(out, acc >= 0 ? acc + 2 : -1, turn_acc);
+}
+
+static inline unsigned edge_trace_blocks(int64_t Nt) {
+  const int64_t per_block = 4LL * kTraceEdges;  // 256 threads = 4 waves
+  return (unsigned)std::max<int64_t>((Nt + per_block - 1) / per_block, 1);
 }
 
 int edge_stats(gahhg_ctx* c, unsigned long long* d_out) {
   const int64_t Nt = c->n_edges;
   int rc = fill_i32(c, (int32_t*)d_out, (int64_t)c->polys.size() * kGrid0Cells * 3 * 2, 0);
   if (rc || Nt <= 0) return rc;
-  hipLaunchKernelGGL(k_edge_trace, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, c->stream, view(c),
+  hipLaunchKernelGGL(k_edge_trace, dim3(edge_trace_blocks(Nt)), dim3(256), 0, c->stream, view(c),
                      (const PolyDev*)c->d_pd, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                      (const int32_t*)c->d_pe, Nt, d_out);
   return hip_check(c, hipGetLastError(), "edge stats");
@@ -113,7 +146,7 @@ int build_trace(gahhg_ctx* c, int32_t* h_cells, uint64_t* h_edges) {
     rc = hip_check(c, hipGetLastError(), "cell trace");
     const int64_t Nt = c->n_edges;
     if (!rc && Nt > 0) {
-      hipLaunchKernelGGL(k_edge_trace, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, c->stream, g,
+      hipLaunchKernelGGL(k_edge_trace, dim3(edge_trace_blocks(Nt)), dim3(256), 0, c->stream, g,
                          (const PolyDev*)c->d_pd, (const double*)c->d_edge, (const int32_t*)c->d_epoly,
                          (const int32_t*)c->d_pe, Nt, d_edges);
       rc = hip_check(c, hipGetLastError(), "edge trace");

@@ -90,7 +90,47 @@ def test_build_trace_counts():
         n_edges = len(items[k][0]) - (len(items[k][2]) - 1)
         assert edges[k, :, 0].sum() == n_edges
         assert (edges[k, :, 1] >= 0).all() and (edges[k, :, 2] >= 0).all()
+        check_edge_aggregates(edges[k], pg, items[k])
     g.close()
+
+
+def check_edge_aggregates(got, pg, item):
+    """Edge aggregates of one polygon against a host restatement: each edge
+    goes to the first level-0 cell whose border planes all pass the inside
+    test at its first endpoint; per cell the edge count (exact), the sum of
+    arc lengths and the sum of turning angles to the ring's next edge, in
+    units of 2^-40 rad, each edge truncated (at most one unit per edge
+    apart, from the last bit of acos)."""
+    pv, pe = ol.Oracle().set_lonlat(*item).polygon()
+    a, b = pv[pe[:, 0]], pv[pe[:, 1]]
+    c, nrm = pg["cells"], pg["normals"]
+    root = np.full(len(pe), -1)
+    for i in range(41, -1, -1):  # backwards: the first matching cell wins
+        inside = np.ones(len(pe), dtype=bool)
+        for k in range(c[i, 1]):
+            n = nrm[c[i, 0] + k]
+            inside &= ~((n[0] * a[:, 0] + n[1] * a[:, 1] + n[2] * a[:, 2]) < -1e-8)
+        root[inside] = i
+    ok = root >= 0
+    cnt = np.bincount(root[ok], minlength=42)
+    assert got[:, 0].astype(np.int64).tolist() == cnt.tolist()
+    fix = float(2 ** 40)
+    ln = np.arccos(np.clip(a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2], -1.0, 1.0))
+    d = b - a
+    ne = np.cross(a, d)
+    ne /= np.sqrt((ne * ne).sum(axis=1))[:, None]
+    turn = np.zeros(len(pe))
+    nxt = np.zeros(len(pe), dtype=bool)
+    nxt[:-1] = pe[:-1, 1] == pe[1:, 0]
+    dn = (ne[:-1] * ne[1:]).sum(axis=1)
+    turn[:-1] = np.where(nxt[:-1], np.arccos(np.clip(dn, -1.0, 1.0)), 0.0)
+    for col, val, tol in ((1, ln, 0.0), (2, turn, 1e-9)):
+        exp = np.zeros(42, dtype=np.int64)
+        np.add.at(exp, root[ok], np.floor(val[ok] * fix).astype(np.int64))
+        # the binding returns the sums in rad: the fixed-point integers
+        # (below 2^53) times 2^-40, exactly
+        dif = np.abs(np.rint(got[:, col] * fix).astype(np.int64) - exp)
+        assert (dif <= cnt + 1 + tol * exp).all(), (col, dif.max())
 
 
 def test_per_polygon_fixed_depths():
