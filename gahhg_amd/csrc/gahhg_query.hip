@@ -739,7 +739,7 @@ __global__ void __launch_bounds__(256) k_leaf_poly_sums(const PolyDev* __restric
 // ref i goes to record k0 + i + (chain breaks before i), and a closing
 // record (the chain's last vertex) follows every break.
 __global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* __restrict__ pd,
-                                                    const double* __restrict__ pv, const int32_t* __restrict__ pe,
+                                                    const int32_t* __restrict__ pe,
                                                     const double* __restrict__ edge, int cnum,
                                                     const int32_t* __restrict__ roff, double* __restrict__ qc,
                                                     int32_t* __restrict__ hdr, double* __restrict__ sv,
@@ -772,14 +772,17 @@ __global__ void __launch_bounds__(256) k_leaf_write(GridView g, const PolyDev* _
     if (i < rn) {
       const int before = carry + __popcll(bal & ((1ull << lane) - 1ull));
       const int k = k0 + i + before;
+      // the edge record holds the edge's vertices (copies of pv at pe[2e],
+      // pe[2e + 1]) beside its normal: one fetch instead of pe then pv
       const int e = r[i];
-      const V3 ne = load3(edge + 9 * (int64_t)e + 6);
+      const double* er = edge + 9 * (int64_t)e;
+      const V3 ne = load3(er + 6);
       const double r2 = dot(ne, c);
-      store3(sv + 3 * (int64_t)k, load3(pv + 3 * (int64_t)pe[2 * e]));
+      store3(sv + 3 * (int64_t)k, load3(er));
       store3(sn + 3 * (int64_t)k, ne);
       sf[k] = (uint8_t)(4 | (lt0(r2) ? 1 : 0) | (gt0(r2) ? 2 : 0));
       if (brk) {
-        store3(sv + 3 * (int64_t)(k + 1), load3(pv + 3 * (int64_t)pe[2 * e + 1]));
+        store3(sv + 3 * (int64_t)(k + 1), load3(er + 3));
         store3(sn + 3 * (int64_t)(k + 1), v3(0.0, 0.0, 0.0));
         sf[k + 1] = 0;
       }
@@ -1528,7 +1531,7 @@ int pack_query_layout(gahhg_ctx* c) {
     return GAHHG_E_NOMEM;
   q.bytes = nc * 32 + q.nrec * 49;
   hipLaunchKernelGGL(k_leaf_write, dim3(wb), dim3(256), 0, c->stream, g, (const PolyDev*)c->d_pd,
-                     (const double*)c->d_pv, (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
+                     (const int32_t*)c->d_pe, (const double*)c->d_edge, (int)c->tot_c,
                      (const int32_t*)c->d_o0, q.qc, q.hdr, q.sv, q.sn, q.sf, G);
   rc = hip_check(c, hipGetLastError(), "leaf write");
   if (rc) return rc;
