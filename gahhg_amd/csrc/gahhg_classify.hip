@@ -95,8 +95,7 @@ struct CpChoice {
 // so the chosen feature -- including the shared-vertex tie rule, whose
 // comparison needs only each ref's own values -- is the reference's.
 // Conditions are uniform within a group; every lane returns the choice.
-__device__ __forceinline__ CpChoice cp_select(const int32_t* __restrict__ refs, int rn,
-                                              const double* __restrict__ pv, const int32_t* __restrict__ pe,
+__device__ __forceinline__ CpChoice cp_select(const int32_t* __restrict__ refs, int rn, const int32_t* __restrict__ pe,
                                               const double* __restrict__ edge, V3 center, const Group& gr) {
   const int G = gr.G;
   double mindist = FLT_MAX;
@@ -134,10 +133,12 @@ __device__ __forceinline__ CpChoice cp_select(const int32_t* __restrict__ refs, 
       side = dot(n0, center);
       if (cur_nv != -1) {
         // the tie rule's lean of this edge: its far end towards the centre,
-        // seen from the shared vertex
-        const int oth = (pe[2 * se] == cur_nv) ? pe[2 * se + 1] : pe[2 * se];
-        const V3 pnv = load3(pv + 3 * (int64_t)cur_nv);
-        const V3 a1 = unit(sub(load3(pv + 3 * (int64_t)oth), pnv));
+        // seen from the shared vertex.  The vertices are v0 = pv[pe[2 se]]
+        // and v1 = pv[pe[2 se + 1]] (the edge record holds copies), so
+        // pv[cur_nv] and pv[other end] are taken from there.
+        const V3 pnv = cur_nv == pe[2 * se] ? v0 : v1;
+        const V3 oth = (pe[2 * se] == cur_nv) ? v1 : v0;
+        const V3 a1 = unit(sub(oth, pnv));
         lean = dot(a1, sub(center, pnv));
       }
     }
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(kCBlock) k_ne(GridView g, const PolyDev* __res
   const int rn = g.c_rnum[cid];
   if (rn == 0) return;
   const V3 center = mean_center(g, cid);
-  const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pv, pe, edge, center, gr);
+  const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pe, edge, center, gr);
   if (gr.gl == 0) g.c_type[cid] = cp_type(ch);
 }
 
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(kCBlock) k_kat_cp_prop(const double* pv, const
   const int w = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   if (w >= n) return;
   const V3 center = load3(centers + 3 * (int64_t)w);
-  const CpChoice ch = cp_select(refs + rs_rn[2 * w], rs_rn[2 * w + 1], pv, pe, edge, center, gr);
+  const CpChoice ch = cp_select(refs + rs_rn[2 * w], rs_rn[2 * w + 1], pe, edge, center, gr);
   if (gr.gl == 0) {
     out3[3 * w] = cp_type(ch);
     out3[3 * w + 1] = ch.slot;
@@ -447,7 +448,7 @@ __global__ void __launch_bounds__(kCBlock, 6) k_ne_dv(GridView g, const PolyDev*
     const int rn = g.c_rnum[cid];
     if (rn == 0) continue;
     const V3 center = mean_center(g, cid);
-    const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pv, pe, edge, center, gr);
+    const CpChoice ch = cp_select(g.refs + g.c_rstart[cid], rn, pe, edge, center, gr);
     if (gr.gl == 0) g.c_type[cid] = cp_type(ch);
   }
 }
